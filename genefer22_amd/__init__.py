@@ -1,0 +1,226 @@
+"""genefer22_amd -- gfx950 modular-squaring engine for b^(2^n) + 1 (genefer's transform path).
+
+Python mirror of the reference's transform interface (src/transform.h:42-256) over the C ABI of
+include/gfe.h and include/gfe_driver.h (libgfe.so, built by genefer22_amd.build).  Every method
+maps to one gfe_* call; errors raise GfeError with the library's message.  There is no CPU fallback:
+importing works without a GPU, but creating an engine without libgfe.so or a HIP device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgfe.so")
+_LIB = None
+
+I32P = ctypes.POINTER(ctypes.c_int32)
+U32P = ctypes.POINTER(ctypes.c_uint32)
+U64P = ctypes.POINTER(ctypes.c_uint64)
+
+
+class GfeError(RuntimeError):
+    pass
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("res64", ctypes.c_uint64), ("old64", ctypes.c_uint64), ("hash64", ctypes.c_uint64),
+                ("pkey", ctypes.c_uint64), ("esize", ctypes.c_uint64), ("iters", ctypes.c_uint64),
+                ("is_prp", ctypes.c_int32), ("gl_ok", ctypes.c_int32), ("seconds", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# (name, restype, argtypes) of every exported entry point; tests check each is present.
+ABI = [
+    ("gfe_create", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("gfe_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_set", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gfe_square_dup", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("gfe_square_mul", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    ("gfe_init_multiplicand", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gfe_mul", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_copy", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    ("gfe_get_digits", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, I32P]),
+    ("gfe_set_digits", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, I32P]),
+    ("gfe_mem_size", ctypes.c_size_t, [ctypes.c_void_p]),
+    ("gfe_cache_size", ctypes.c_size_t, [ctypes.c_void_p]),
+    ("gfe_context_size", ctypes.c_size_t, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gfe_save", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t]),
+    ("gfe_load", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_size_t]),
+    ("gfe_get_error", ctypes.c_double, [ctypes.c_void_p]),
+    ("gfe_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+    ("gfe_sync", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_select", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    ("gfe_last_error", ctypes.c_char_p, []),
+    ("gfe_time_squarings", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    ("gfe_tune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gfe_run_quick", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(Result)]),
+    ("gfe_run_prp", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_int32, I32P, ctypes.POINTER(Result)]),
+    ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),
+    ("gfe_residue", ctypes.c_int, [I32P, ctypes.c_uint32, ctypes.c_uint32, I32P, ctypes.POINTER(Result)]),
+]
+
+
+def lib():
+    """Loads libgfe.so; raises GfeError when it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise GfeError(f"{LIB_PATH} is missing: build it with `python -m genefer22_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in ABI:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _ck(rc: int) -> None:
+    if rc != 0:
+        raise GfeError(f"gfe error {rc}: {lib().gfe_last_error().decode(errors='replace')}")
+
+
+def _i32(a: np.ndarray):
+    return a.ctypes.data_as(I32P)
+
+
+def select(b: int, n: int) -> tuple[int, bool]:
+    """Prime set of create_ocl (src/transform_ocl.cpp:35-50): (2 or 3 primes, 32-bit primes?)."""
+    rns, is32 = ctypes.c_int(), ctypes.c_int()
+    _ck(lib().gfe_select(b, n, ctypes.byref(rns), ctypes.byref(is32)))
+    return rns.value, bool(is32.value)
+
+
+class Transform:
+    """The reference's transform interface on the HIP engine (one candidate, one device)."""
+
+    def __init__(self, b: int, n: int, num_regs: int = 3, device: int = 0):
+        self.b, self.n, self.N, self.num_regs = b, n, 1 << n, num_regs
+        h = ctypes.c_void_p()
+        _ck(lib().gfe_create(b, n, num_regs, device, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gfe_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # src/transform.h:57-73
+    def set(self, a: int): _ck(lib().gfe_set(self._h, a))
+    def square_dup(self, dup: bool = False): _ck(lib().gfe_square_dup(self._h, int(bool(dup))))
+    def square_mul(self, a: int): _ck(lib().gfe_square_mul(self._h, a))
+    def init_multiplicand(self, src: int): _ck(lib().gfe_init_multiplicand(self._h, src))
+
+    def mul(self, src: int | None = None):
+        """mul() (r0 *= rm) or, with src, transform::mul(src) = initMultiplicand(src) + mul()."""
+        if src is not None:
+            self.init_multiplicand(src)
+        _ck(lib().gfe_mul(self._h))
+
+    def copy(self, dst: int, src: int): _ck(lib().gfe_copy(self._h, dst, src))
+    def sync(self): _ck(lib().gfe_sync(self._h))
+    def mem_size(self) -> int: return lib().gfe_mem_size(self._h)
+    def cache_size(self) -> int: return lib().gfe_cache_size(self._h)
+    def error(self) -> float: return lib().gfe_get_error(self._h)
+
+    def info(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        _ck(lib().gfe_info(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    def get_digits(self, reg: int = 0) -> np.ndarray:
+        d = np.zeros(self.N, dtype=np.int32)
+        _ck(lib().gfe_get_digits(self._h, reg, _i32(d)))
+        return d
+
+    def set_digits(self, d, reg: int = 0):
+        d = np.ascontiguousarray(d, dtype=np.int32)
+        if d.shape != (self.N,):
+            raise ValueError("setInt")
+        _ck(lib().gfe_set_digits(self._h, reg, _i32(d)))
+
+    def save_context(self, num_regs: int = 0) -> bytes:
+        size = lib().gfe_context_size(self._h, num_regs or self.num_regs)
+        buf = ctypes.create_string_buffer(size)
+        _ck(lib().gfe_save(self._h, num_regs, buf, size))
+        return buf.raw
+
+    def read_context(self, data: bytes, num_regs: int = 0) -> bool:
+        """readContext: False for a context of another engine kind (src/transformGPU.h:1127-1139)."""
+        buf = ctypes.create_string_buffer(data, len(data))
+        rc = lib().gfe_load(self._h, num_regs, buf, len(data))
+        if rc == -4:
+            return False
+        _ck(rc)
+        return True
+
+    def time_squarings(self, count: int, dup_bits: int = 0, per_kernel: bool = False):
+        ms = ctypes.c_double()
+        km = (ctypes.c_double * 5)()
+        _ck(lib().gfe_time_squarings(self._h, count, dup_bits & (2**64 - 1), ctypes.byref(ms), km if per_kernel else None))
+        return (ms.value, list(km)) if per_kernel else ms.value
+
+    def tune(self, reps: int = 8) -> str:
+        _ck(lib().gfe_tune(self._h, reps))
+        return self.info()
+
+    def residue(self, reg: int = 0) -> dict:
+        return residue(self.get_digits(reg), self.n, self.b)
+
+
+def residue(digits, n: int, b: int, canonical: bool = False):
+    """RES64 / old64 / hash64 / isOne of a digit vector by the library's gint (src/gint.h:40-225)."""
+    d = np.ascontiguousarray(digits, dtype=np.int32)
+    r = Result()
+    canon = np.zeros(1 << n, dtype=np.int32) if canonical else None
+    _ck(lib().gfe_residue(_i32(d), n, b, _i32(canon) if canonical else None, ctypes.byref(r)))
+    out = r.as_dict()
+    if canonical:
+        out["digits"] = canon
+    return out
+
+
+def run_quick(b: int, n: int, device: int = 0) -> dict:
+    """genefer -q: PRP + Gerbicz-Li check (quick(), src/genefer.h:736-748)."""
+    r = Result()
+    _ck(lib().gfe_run_quick(b, n, device, ctypes.byref(r)))
+    return r.as_dict()
+
+
+def run_prp(b: int, n: int, iters: int = 0, b_gl: int = 0, device: int = 0, digits: bool = False) -> dict:
+    """The first `iters` squarings of the PRP loop (src/genefer.h:479-552)."""
+    r = Result()
+    d = np.zeros(1 << n, dtype=np.int32) if digits else None
+    _ck(lib().gfe_run_prp(b, n, device, iters, b_gl, _i32(d) if digits else None, ctypes.byref(r)))
+    out = r.as_dict()
+    if digits:
+        out["digits"] = d
+    return out
+
+
+def run_proof(b: int, n: int, depth: int, device: int = 0) -> dict:
+    """proof(): PRP with Pietrzak-Li checkpoints, GL, proof key (src/genefer.h:750-766)."""
+    r = Result()
+    mu = (ctypes.c_uint64 * (depth + 1))()
+    _ck(lib().gfe_run_proof(b, n, device, depth, mu, ctypes.byref(r)))
+    out = r.as_dict()
+    out["mu_hash64"] = list(mu)
+    return out

@@ -1,0 +1,189 @@
+// genefer.h -- the Fermat PRP driver of the squaring engine: the caller side of the transform
+// interface, restating the reference driver's sequencing (src/genefer.h:441-766):
+//   prp   : r0 = 2^E, E = b^(2^n), one squareDup(bit) per exponent bit, with the Gerbicz-Li
+//           product d(t) updated every B_GL iterations and Pietrzak-Li checkpoints every B_PL;
+//   GL    : the Gerbicz-Li check d(t)^(2^B) * 2^res == d(t+1);
+//   PL    : the Pietrzak-Li proof (mu_k residues and the proof key) from the checkpoints;
+//   power / powerz : r0 = r_reg^e by square-and-multiply.
+// GMP provides E and its bits as in the reference.
+#pragma once
+
+#include <gmp.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "transform_hip.h"
+
+namespace gfe {
+
+inline int ilog2_32(uint32_t x) { return 31 - __builtin_clz(x); }
+
+class genefer {
+public:
+    genefer(uint32_t b, uint32_t n, int device, size_t num_regs)
+        : _t(new transformHIP(b, n, device, num_regs)), _g(size_t(1) << n, b) {
+        mpz_init(_E);
+        mpz_ui_pow_ui(_E, b, 1ul << n);
+    }
+    ~genefer() { mpz_clear(_E); }
+
+    transformHIP &t() { return *_t; }
+    gint &gi() { return _g; }
+    const mpz_t &exponent() const { return _E; }
+    size_t esize() const { return mpz_sizeinbase(_E, 2); }
+
+    // src/genefer.h:467-476
+    static int B_GerbiczLi(size_t esize) {
+        const size_t L = size_t(2) << (ilog2_32((uint32_t)esize) / 2);
+        return (int)((esize - 1) / L) + 1;
+    }
+    static int B_PietrzakLi(size_t esize, int depth) { return (int)((esize - 1) >> depth) + 1; }
+
+    // src/genefer.h:441-465
+    void power(size_t reg, uint32_t e) {
+        transformHIP &T = *_t;
+        T.initMultiplicand(reg);
+        T.set(1);
+        if (e == 0) return;
+        for (int i = ilog2_32(e); i >= 0; --i) {
+            T.squareDup(false);
+            if ((e >> i) & 1u) T.mul();
+        }
+    }
+    void powerz(size_t reg, mpz_srcptr e) {
+        transformHIP &T = *_t;
+        T.initMultiplicand(reg);
+        T.set(1);
+        if (mpz_sgn(e) == 0) return;
+        for (int i = (int)mpz_sizeinbase(e, 2) - 1; i >= 0; --i) {
+            T.squareDup(false);
+            if (mpz_tstbit(e, (mp_bitcnt_t)i)) T.mul();
+        }
+    }
+
+    // The PRP loop, src/genefer.h:479-552.  Runs exponent bits esize-1 .. i_end (i_end = 0: all).
+    // Pietrzak-Li checkpoints go to registers 3 + i/B_PL when ckpt_in_regs (the reference's
+    // fast_checkpoints), else to host digit vectors.
+    void prp(int B_GL, int B_PL, bool ckpt_in_regs, long i_end = 0) {
+        transformHIP &T = *_t;
+        T.set(1);
+        T.copy(1, 0);
+        const long i0 = (long)esize() - 1;
+        _ckpts.clear();
+        for (long i = i0; i >= i_end; --i) {
+            T.squareDup(mpz_tstbit(_E, (mp_bitcnt_t)i) != 0);
+            if (B_GL > 0 && (i % B_GL == 0) && (i / B_GL != 0)) {
+                T.copy(2, 0);
+                T.mul(1);
+                T.copy(1, 0);
+                T.copy(0, 2);
+            }
+            if (B_PL != 0 && (i % B_PL == 0)) {
+                if (ckpt_in_regs) T.copy(3 + size_t(i / B_PL), 0);
+                else {
+                    T.getInt(_g);
+                    _ckpts[size_t(i / B_PL)].assign(_g.data(), _g.data() + _g.getSize());
+                }
+            }
+        }
+    }
+
+    // The Gerbicz-Li check, src/genefer.h:557-615.  In: r0 = 2^E, r1 = d(t).
+    bool GL(int B_GL) {
+        transformHIP &T = *_t;
+        T.mul(1);
+        T.copy(2, 0);
+        T.copy(0, 1);
+        for (int i = B_GL - 1; i >= 0; --i) T.squareDup(false);
+        T.copy(1, 0);
+        mpz_t res, e, tt;
+        mpz_init_set_ui(res, 0);
+        mpz_init_set(e, _E);
+        mpz_init(tt);
+        while (mpz_sgn(e) != 0) {
+            mpz_mod_2exp(tt, e, (unsigned long)B_GL);
+            mpz_add(res, res, tt);
+            mpz_div_2exp(e, e, (unsigned long)B_GL);
+        }
+        T.set(1);
+        for (int i = (int)mpz_sizeinbase(res, 2) - 1; i >= 0; --i) T.squareDup(mpz_tstbit(res, (mp_bitcnt_t)i) != 0);
+        mpz_clear(res); mpz_clear(e); mpz_clear(tt);
+        T.mul(1);
+        T.getInt(_g);
+        const uint64_t h1 = _g.gethash64();
+        T.copy(0, 2);
+        T.getInt(_g);
+        const uint64_t h2 = _g.gethash64();
+        return h1 == h2;
+    }
+
+    // The Pietrzak-Li proof, src/genefer.h:620-734, returning the mu_k residues (canonical
+    // digits) and the proof key hash64(v1).  Needs the checkpoints of prp(.., B_PL, ..).
+    uint64_t PL(int depth, bool ckpt_in_regs, std::vector<std::vector<int32_t>> &mus) {
+        transformHIP &T = *_t;
+        mus.clear();
+        load_ckpt(0, ckpt_in_regs);  // mu[0] = ckpt[0]
+        _g.unbalance();
+        mus.emplace_back(_g.data(), _g.data() + _g.getSize());
+        const uint32_t q0 = _g.gethash32();
+        T.setInt(_g);
+        power(0, q0);
+        T.copy(2, 0);
+        const size_t L = size_t(1) << depth;
+        std::vector<__mpz_struct> w(L / 2);
+        for (auto &x : w) mpz_init(&x);
+        mpz_set_ui(&w[0], q0);
+        for (int k = 1; k <= depth; ++k) {
+            const size_t i = size_t(1) << (depth - k);
+            load_ckpt(i, ckpt_in_regs);
+            if (!ckpt_in_regs) T.setInt(_g);
+            powerz(0, &w[0]);
+            T.copy(1, 0);
+            for (size_t j = i; j < L / 2; j += i) {
+                load_ckpt(i + 2 * j, ckpt_in_regs);
+                if (!ckpt_in_regs) T.setInt(_g);
+                powerz(0, &w[j]);
+                T.mul(1);
+                T.copy(1, 0);
+            }
+            T.getInt(_g);
+            _g.unbalance();
+            mus.emplace_back(_g.data(), _g.data() + _g.getSize());
+            const uint32_t q = _g.gethash32();
+            power(0, q);
+            T.mul(2);
+            T.copy(2, 0);
+            if (i > 1)
+                for (size_t j = 0; j < L / 2; j += i) mpz_mul_ui(&w[i / 2 + j], &w[j], q);
+        }
+        for (auto &x : w) mpz_clear(&x);
+        T.copy(0, 2);
+        T.getInt(_g);
+        return _g.gethash64();
+    }
+
+private:
+    std::unique_ptr<transformHIP> _t;
+    gint _g;
+    mpz_t _E;
+    std::map<size_t, std::vector<int32_t>> _ckpts;
+
+    // Checkpoint i into r0 (registers) or into the gint (host copies).
+    void load_ckpt(size_t i, bool in_regs) {
+        if (in_regs) {
+            _t->copy(0, 3 + i);
+            _t->getInt(_g);
+        } else {
+            const std::vector<int32_t> &c = _ckpts.at(i);
+            std::copy(c.begin(), c.end(), _g.data());
+            _g.reset();
+        }
+    }
+};
+
+}  // namespace gfe

@@ -1,0 +1,125 @@
+// genefer_driver.cpp -- C ABI of the PRP driver (include/gfe_driver.h).
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "../../include/gfe_driver.h"
+#include "genefer.h"
+
+namespace gfe {
+int set_error(int code, const std::string &msg);
+}
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void fill_residue(gfe::gint &g, gfe_result *out) {
+    uint64_t r = 0, o = 0;
+    const bool one = g.isOne(r, o);
+    out->res64 = r;
+    out->old64 = o;
+    out->is_prp = one ? 1 : 0;
+    out->hash64 = one ? 0 : g.gethash64();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gfe_run_quick(uint32_t b, uint32_t n, int device, gfe_result *out) {
+    if (!out) return gfe::set_error(GFE_EINVAL, "null output");
+    memset(out, 0, sizeof(*out));
+    try {
+        const double t0 = now_s();
+        gfe::genefer G(b, n, device, 3);
+        const size_t es = G.esize();
+        const int B_GL = gfe::genefer::B_GerbiczLi(es);
+        G.prp(B_GL, 0, false);
+        G.t().getInt(G.gi());
+        fill_residue(G.gi(), out);
+        out->esize = es;
+        out->iters = es;
+        out->gl_ok = G.GL(B_GL) ? 1 : 0;
+        out->seconds = now_s() - t0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EHIP, ex.what());
+    }
+}
+
+int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl, int32_t *digits_out,
+                gfe_result *out) {
+    if (!out) return gfe::set_error(GFE_EINVAL, "null output");
+    memset(out, 0, sizeof(*out));
+    try {
+        const double t0 = now_s();
+        gfe::genefer G(b, n, device, 3);
+        const size_t es = G.esize();
+        const int B_GL = (b_gl == 0) ? gfe::genefer::B_GerbiczLi(es) : (b_gl < 0 ? 0 : b_gl);
+        const long i_end = (iters == 0 || iters >= es) ? 0 : (long)(es - iters);
+        G.prp(B_GL, 0, false, i_end);
+        G.t().getInt(G.gi());
+        fill_residue(G.gi(), out);
+        if (digits_out) memcpy(digits_out, G.gi().data(), (size_t(1) << n) * sizeof(int32_t));
+        out->esize = es;
+        out->iters = es - (size_t)i_end;
+        out->gl_ok = -1;
+        out->seconds = now_s() - t0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EHIP, ex.what());
+    }
+}
+
+int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_hash64, gfe_result *out) {
+    if (!out || depth < 1 || depth > 12) return gfe::set_error(GFE_EINVAL, "proof arguments");
+    memset(out, 0, sizeof(*out));
+    try {
+        const double t0 = now_s();
+        const bool fast = (n <= 17);  // src/genefer.h:1377-1385
+        gfe::genefer G(b, n, device, fast ? 3 + (size_t(1) << depth) : 3);
+        const size_t es = G.esize();
+        const int B_GL = gfe::genefer::B_GerbiczLi(es), B_PL = gfe::genefer::B_PietrzakLi(es, depth);
+        G.prp(B_GL, B_PL, fast);
+        G.t().getInt(G.gi());
+        fill_residue(G.gi(), out);
+        out->esize = es;
+        out->iters = es;
+        out->gl_ok = G.GL(B_GL) ? 1 : 0;
+        std::vector<std::vector<int32_t>> mus;
+        out->pkey = G.PL(depth, fast, mus);
+        if (mu_hash64) {
+            gfe::gint g(size_t(1) << n, b);
+            for (size_t k = 0; k < mus.size(); ++k) {
+                std::copy(mus[k].begin(), mus[k].end(), g.data());
+                g.reset();
+                mu_hash64[k] = g.gethash64();
+            }
+        }
+        out->seconds = now_s() - t0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EHIP, ex.what());
+    }
+}
+
+int gfe_residue(const int32_t *digits, uint32_t n, uint32_t b, int32_t *canon, gfe_result *out) {
+    if (!digits || !out || n > 24) return gfe::set_error(GFE_EINVAL, "residue arguments");
+    memset(out, 0, sizeof(*out));
+    try {
+        gfe::gint g(size_t(1) << n, b);
+        memcpy(g.data(), digits, (size_t(1) << n) * sizeof(int32_t));
+        fill_residue(g, out);
+        if (canon) memcpy(canon, g.data(), (size_t(1) << n) * sizeof(int32_t));
+        out->gl_ok = -1;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EINVAL, ex.what());
+    }
+}
+
+}  // extern "C"

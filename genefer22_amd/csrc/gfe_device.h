@@ -1,0 +1,131 @@
+// gfe_device.h -- device-side modular arithmetic, CRT and carry for the gfx950 squaring engine.
+//
+// Residues are plain integers in [0, p) (no Montgomery form between operations).
+//  * S primes (p < 2^31): twiddle products use Shoup's method with a precomputed quotient
+//    w' = floor(w 2^32 / p): one mul_hi, one mul_lo and a v_mad_u64_u32 (all half-rate on gfx950)
+//    plus a min-based correction.
+//  * U primes (2^31 < p < 2^32): 2p does not fit in 32 bits, so twiddle products use Montgomery
+//    reduction with the twiddle stored as w 2^32 mod p, which returns the plain product a w.
+//  * data x data products (the pointwise square / product) use Montgomery reduction for both
+//    sets; the resulting factor 2^-32 is folded into the per-prime scale applied before the CRT.
+// The reference's arithmetic for the same primes is ocl/kernel.cl:94-126 (Montgomery on data kept
+// in Montgomery form); the residues computed here are the same integers mod p.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gfe {
+
+// ---------------------------------------------------------------------------------------------
+// Modular arithmetic
+
+struct ModS {  // p < 2^31
+    uint32_t p, q;  // q = p^-1 mod 2^32
+    __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const {
+        const uint32_t s = a + b;
+        return min(s, s - p);
+    }
+    __device__ __forceinline__ uint32_t sub(uint32_t a, uint32_t b) const {
+        const uint32_t d = a - b;
+        return min(d, d + p);
+    }
+    // a * w mod p; a is any 32-bit value, tw = (w, floor(w 2^32 / p)).
+    __device__ __forceinline__ uint32_t mulw(uint32_t a, uint2 tw) const {
+        const uint32_t qq = __umulhi(a, tw.y);
+        const uint32_t r = a * tw.x - qq * p;
+        return min(r, r - p);
+    }
+    // a * b * 2^-32 mod p, a < 2^32, b < p.
+    __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b) const {
+        const uint64_t t = (uint64_t)a * b;
+        const uint32_t m = (uint32_t)t * q;
+        const uint32_t r = (uint32_t)(t >> 32) - __umulhi(m, p);
+        return min(r, r + p);
+    }
+};
+
+struct ModU {  // 2^31 < p < 2^32
+    uint32_t p, q;
+    __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const {
+        const uint32_t s = a + b;
+        return ((s < a) | (s >= p)) ? s - p : s;
+    }
+    __device__ __forceinline__ uint32_t sub(uint32_t a, uint32_t b) const {
+        const uint32_t d = a - b;
+        return (a < b) ? d + p : d;
+    }
+    __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b) const {
+        const uint64_t t = (uint64_t)a * b;
+        const uint32_t m = (uint32_t)t * q;
+        const uint32_t hi = (uint32_t)(t >> 32), mh = __umulhi(m, p);
+        const uint32_t r = hi - mh;
+        return (hi < mh) ? r + p : r;
+    }
+    // a * w mod p with tw.y = w 2^32 mod p (Montgomery form of w).
+    __device__ __forceinline__ uint32_t mulw(uint32_t a, uint2 tw) const { return mont(a, tw.y); }
+};
+
+// Residue of a signed digit.
+__device__ __forceinline__ uint32_t to_res(int32_t d, uint32_t p) {
+    return (d < 0) ? (uint32_t)d + p : (uint32_t)d;
+}
+
+// ---------------------------------------------------------------------------------------------
+// CRT and carry
+
+// Per-candidate normalisation constants (host-computed).
+struct CrtConst {
+    uint32_t p[3];
+    uint32_t q[3];
+    uint2 scale[3];     // multiplies the raw inverse-transform output: 2^32 N^-1 (see header)
+    uint2 inv_p1_m2;    // 1/p1 mod p2
+    uint2 p1_m3;        // p1 mod p3
+    uint2 inv_p12_m3;   // 1/(p1 p2) mod p3
+    uint64_t p12;       // p1 p2
+    double p12_d;       // (double)(p1 p2)
+};
+
+// Balanced division of the exact integer f (given as f mod 2^64 and an approximation f_d with
+// |f_d - f| small relative to b) by b: returns d in (-b/2, b/2] with f = q b + d, q in *quot.
+__device__ __forceinline__ int32_t div_bal(uint64_t f64, double f_d, uint32_t b, double binv, int64_t *quot) {
+    int64_t qi = (int64_t)rint(f_d * binv);
+    int64_t r = (int64_t)(f64 - (uint64_t)qi * (uint64_t)b);
+    const int64_t k = (int64_t)rint((double)r * binv);
+    r -= k * (int64_t)b;
+    qi += k;
+    const int64_t hi = (int64_t)(b / 2), lo = -(int64_t)((b - 1) / 2);
+    if (r > hi) { r -= b; qi += 1; }
+    if (r < lo) { r += b; qi -= 1; }
+    *quot = qi;
+    return (int32_t)r;
+}
+
+// Exact coefficient from residues (already scaled), as (x mod 2^64, approximate x).
+template <int RNS, class M1, class M2, class M3>
+__device__ __forceinline__ void garner(const CrtConst &c, const M1 &m1, const M2 &m2, const M3 &m3,
+                                       uint32_t a1, uint32_t a2, uint32_t a3, uint64_t *x64, double *xd) {
+    // y2 = (a2 - a1) / p1 mod p2
+    const uint32_t a1m2 = (a1 >= c.p[1]) ? a1 - c.p[1] : a1;
+    const uint32_t y2 = m2.mulw(m2.sub(a2, a1m2), c.inv_p1_m2);
+    const uint64_t lo = (uint64_t)a1 + (uint64_t)c.p[0] * y2;  // in [0, p1 p2)
+    if (RNS == 2) {
+        const bool neg = (y2 > (c.p[1] - 1) / 2) || ((y2 == (c.p[1] - 1) / 2) && (a1 > (c.p[0] - 1) / 2));
+        const uint64_t x = neg ? lo - c.p12 : lo;
+        *x64 = x;
+        *xd = (double)(int64_t)x;
+    } else {
+        // y3 = (a3 - y1 - p1 y2) / (p1 p2) mod p3
+        const uint32_t a1m3 = (a1 >= c.p[2]) ? a1 - c.p[2] : a1;  // p1 < 2 p3
+        const uint32_t t = m3.sub(m3.sub(a3, a1m3), m3.mulw(y2, c.p1_m3));
+        const uint32_t y3 = m3.mulw(t, c.inv_p12_m3);
+        const bool neg = (y3 > (c.p[2] - 1) / 2) ||
+                         ((y3 == (c.p[2] - 1) / 2) && ((y2 > (c.p[1] - 1) / 2) ||
+                                                       ((y2 == (c.p[1] - 1) / 2) && (a1 > (c.p[0] - 1) / 2))));
+        const int64_t Y3 = neg ? (int64_t)y3 - (int64_t)c.p[2] : (int64_t)y3;
+        *x64 = lo + c.p12 * (uint64_t)Y3;
+        *xd = (double)lo + c.p12_d * (double)Y3;
+    }
+}
+
+}  // namespace gfe

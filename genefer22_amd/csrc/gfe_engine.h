@@ -1,0 +1,64 @@
+// gfe_engine.h -- host-side engine object behind the C ABI of include/gfe.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "gfe_kernels.h"
+
+namespace gfe {
+
+// Pass plan of one n: a column levels (2^a-point column transforms, col_ng columns per
+// workgroup, 2^col_loge elements per thread) and n - a row levels (row_ng rows per workgroup,
+// 2^row_loge elements per thread); norm_s digits per thread in the normalisation.
+struct Plan {
+    int a = 0, col_loge = 4, col_ng = 16, row_loge = 4, row_ng = 1, norm_s = 8;
+};
+
+Plan default_plan(uint32_t n);
+bool plan_valid(const Plan &P, uint32_t n);
+std::string plan_str(const Plan &P, uint32_t n);
+void select_primes(uint32_t b, uint32_t n, int *rns, int *is32);
+uint32_t prime_of(int is32, int k);
+uint2 mulw_const(uint32_t w, uint32_t p);
+CrtConst make_crt(int rns, int is32, uint32_t n);
+int set_error(int code, const std::string &msg);
+extern thread_local std::string g_last_error;
+
+struct Engine {
+    uint32_t b = 0, n = 0, num_regs = 0;
+    int device = 0, rns = 0, is32 = 0;
+    size_t N = 0;
+    Plan plan;
+    PrimeArgs pr;
+    hipStream_t stream = nullptr;
+    int32_t *regs = nullptr;
+    uint32_t *ws = nullptr, *zp = nullptr;
+    int64_t *carry = nullptr;
+    uint2 *tw = nullptr;
+    CrtConst *crt = nullptr;
+    uint32_t *bdev = nullptr;
+    std::vector<hipEvent_t> events;
+    bool dev_ok = false;
+
+    ~Engine();
+    int init(uint32_t b, uint32_t n, uint32_t num_regs, int device);
+    size_t lds_col() const;
+    size_t lds_row() const;
+    void launch_col_fwd(uint32_t src, uint32_t *dst);
+    void launch_col_inv();
+    void launch_row(int mode, uint32_t *data);
+    void launch_norm(int dup);
+    void launch_carry();
+    void square(int dup);
+    void mul();
+    void init_multiplicand(uint32_t src);
+    void mul_scalar(int32_t a);
+    void set(uint32_t a);
+    int check(const char *what);
+};
+
+}  // namespace gfe

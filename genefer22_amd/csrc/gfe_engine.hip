@@ -1,0 +1,674 @@
+// gfe_engine.hip -- host side of the gfx950 squaring engine and its C ABI (include/gfe.h).
+//
+// Replaces the reference's engines<RNS, is32> / transformGPUs<RNS, is32> (src/transformGPU.h:
+// 210-1195) and create_ocl (src/transform_ocl.cpp:20-52).  Device state of one engine:
+//   regs  : num_regs x N int32 digits (register 0 = r0), balanced, |d| <= b/2 + 4
+//   ws    : rns x N uint32 residues (transform-domain work space of one squaring)
+//   zp    : rns x N uint32 residues (multiplicand, transform domain)
+//   carry : N / S int64 segment carries
+//   tw    : rns x N uint2 twiddles (w, Shoup quotient or Montgomery form)
+// One squaring = k_col_fwd, k_row, k_col_inv, k_norm, k_carry on the engine's stream.
+
+#include "gfe_engine.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/gfe.h"
+
+namespace gfe {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host modular arithmetic
+
+static const uint32_t k_primes[6] = {127u * (1u << 24) + 1u, 63u * (1u << 25) + 1u, 15u * (1u << 27) + 1u,
+                                     125u * (1u << 25) + 1u, 243u * (1u << 24) + 1u, 235u * (1u << 24) + 1u};
+static const uint32_t k_gen[6] = {3, 5, 31, 3, 7, 3};
+
+uint32_t prime_of(int is32, int k) { return k_primes[(is32 ? 3 : 0) + k]; }
+
+static uint32_t hmul(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)(((uint64_t)a * b) % p); }
+static uint32_t hpow(uint32_t a, uint64_t e, uint32_t p) {
+    uint32_t r = 1;
+    while (e) {
+        if (e & 1) r = hmul(r, a, p);
+        a = hmul(a, a, p);
+        e >>= 1;
+    }
+    return r;
+}
+static uint32_t hinv(uint32_t a, uint32_t p) { return hpow(a % p, p - 2, p); }
+static uint32_t pinv32(uint32_t p) {
+    uint32_t q = 1;
+    for (int i = 0; i < 5; ++i) q *= 2u - p * q;
+    return q;
+}
+
+// Operand of ModS::mulw / ModU::mulw for the constant w.
+uint2 mulw_const(uint32_t w, uint32_t p) {
+    uint2 r;
+    r.x = w;
+    if (p < (1u << 31)) r.y = (uint32_t)((((uint64_t)w) << 32) / p);
+    else r.y = (uint32_t)((((uint64_t)w) << 32) % p);
+    return r;
+}
+
+// create_ocl's rule, src/transform_ocl.cpp:35-50.
+void select_primes(uint32_t b, uint32_t n, int *rns, int *is32) {
+    const uint64_t bb = (uint64_t)b * b, N = (uint64_t)1 << n;
+    if (bb < ((uint64_t)k_primes[0] * k_primes[1] / 2) / N) { *rns = 2; *is32 = 0; }
+    else if (bb < ((uint64_t)k_primes[3] * k_primes[4] / 2) / N) { *rns = 2; *is32 = 1; }
+    else if (b <= 1000000000u) { *rns = 3; *is32 = 0; }
+    else { *rns = 3; *is32 = 1; }
+}
+
+// Twiddles Psi[k] = psi^bitrev_n(k), k in [1, N), psi = g^((p-1)/2N), in mulw form.
+static void make_twiddles(int is32, int k, uint32_t n, uint2 *out) {
+    const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k];
+    const size_t N = (size_t)1 << n;
+    const uint32_t psi = hpow(g, (p - 1) >> (n + 1), p);
+    std::vector<uint32_t> pw(N);
+    pw[0] = 1;
+    for (size_t e = 1; e < N; ++e) pw[e] = hmul(pw[e - 1], psi, p);
+    out[0] = make_uint2(0, 0);
+    for (size_t j = 1; j < N; ++j) {
+        size_t r = 0;
+        for (uint32_t bit = 0; bit < n; ++bit) r |= ((j >> bit) & 1) << (n - 1 - bit);
+        out[j] = mulw_const(pw[r], p);
+    }
+}
+
+CrtConst make_crt(int rns, int is32, uint32_t n) {
+    CrtConst c;
+    memset(&c, 0, sizeof(c));
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t p = prime_of(is32, k);
+        c.p[k] = p;
+        c.q[k] = pinv32(p);
+        // output of the inverse transform = N 2^-32 (product); multiply by 2^32 / N
+        const uint32_t s = hmul((uint32_t)((((uint64_t)1) << 32) % p), hinv((uint32_t)(((uint64_t)1 << n) % p), p), p);
+        c.scale[k] = mulw_const(s, p);
+    }
+    const uint32_t p1 = c.p[0], p2 = c.p[1], p3 = c.p[2];
+    c.inv_p1_m2 = mulw_const(hinv(p1 % p2, p2), p2);
+    c.p1_m3 = mulw_const(p1 % p3, p3);
+    c.inv_p12_m3 = mulw_const(hinv(hmul(p1 % p3, p2 % p3, p3), p3), p3);
+    c.p12 = (uint64_t)p1 * p2;
+    c.p12_d = (double)c.p12;
+    (void)rns;
+    return c;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Plans
+
+Plan default_plan(uint32_t n) {
+    Plan P;
+    P.a = (int)n / 2;
+    if (n >= 22) P.a = (int)n - 12;
+    P.col_loge = 4;
+    P.row_loge = 4;
+    const int E = 1 << P.col_loge;
+    const int G = 1 << P.a;
+    P.col_ng = std::min(16, std::max(1, 1024 * E / G));
+    while (P.col_ng * G / E < 64 && P.col_ng < (1 << (n - P.a))) P.col_ng *= 2;
+    const int Gr = 1 << (n - P.a), Er = 1 << P.row_loge;
+    const int R = 1 << P.a;
+    P.row_ng = std::min(R, std::max(1, 256 / (Gr / Er)));
+    P.norm_s = 8;
+    return P;
+}
+
+bool plan_valid(const Plan &P, uint32_t n) {
+    const int gc = P.a, gr = (int)n - P.a;
+    if (gc < P.col_loge || gr < P.row_loge) return false;
+    const int Tc = P.col_ng * (1 << gc) / (1 << P.col_loge);
+    const int Tr = P.row_ng * (1 << gr) / (1 << P.row_loge);
+    if (Tc < 32 || Tc > 1024 || Tr < 32 || Tr > 1024) return false;
+    if (P.col_ng > (1 << gr) || P.row_ng > (1 << gc)) return false;
+    const size_t lc = (size_t)P.col_ng * ((1 << gc) + ((1 << gc) >> P.col_loge)) * 4;
+    const size_t lr = (size_t)P.row_ng * ((1 << gr) + ((1 << gr) >> P.row_loge)) * 4;
+    if (lc > 160 * 1024 || lr > 160 * 1024) return false;
+    return true;
+}
+
+std::string plan_str(const Plan &P, uint32_t n) {
+    std::ostringstream ss;
+    ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (n - P.a)
+       << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg), norm " << P.norm_s << " digits/thread";
+    return ss.str();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Kernel dispatch
+
+typedef void (*ColFn)(ColArgs);
+typedef void (*RowFn)(RowArgs);
+typedef void (*NormFn)(NormArgs);
+
+template <class M> static ColFn col_fwd_fn(int loge) {
+    switch (loge) {
+        case 3: return k_col_fwd<M, 3, true>;
+        case 4: return k_col_fwd<M, 4, true>;
+        default: return k_col_fwd<M, 5, true>;
+    }
+}
+template <class M> static ColFn col_inv_fn(int loge) {
+    switch (loge) {
+        case 3: return k_col_inv<M, 3>;
+        case 4: return k_col_inv<M, 4>;
+        default: return k_col_inv<M, 5>;
+    }
+}
+template <class M, int MODE> static RowFn row_fn(int loge) {
+    switch (loge) {
+        case 3: return k_row<M, 3, MODE>;
+        case 4: return k_row<M, 4, MODE>;
+        default: return k_row<M, 5, MODE>;
+    }
+}
+
+struct KernelSet {
+    ColFn col_fwd, col_inv;
+    RowFn row_sq, row_mul, row_fwd;
+    NormFn norm;
+};
+
+static KernelSet kernels_for(int rns, int is32, const Plan &P) {
+    KernelSet K;
+    if (is32) {
+        K.col_fwd = col_fwd_fn<ModU>(P.col_loge);
+        K.col_inv = col_inv_fn<ModU>(P.col_loge);
+        K.row_sq = row_fn<ModU, ROW_SQUARE>(P.row_loge);
+        K.row_mul = row_fn<ModU, ROW_MUL>(P.row_loge);
+        K.row_fwd = row_fn<ModU, ROW_FWD>(P.row_loge);
+        K.norm = (rns == 2) ? k_norm<2, ModU, 8> : k_norm<3, ModU, 8>;
+    } else {
+        K.col_fwd = col_fwd_fn<ModS>(P.col_loge);
+        K.col_inv = col_inv_fn<ModS>(P.col_loge);
+        K.row_sq = row_fn<ModS, ROW_SQUARE>(P.row_loge);
+        K.row_mul = row_fn<ModS, ROW_MUL>(P.row_loge);
+        K.row_fwd = row_fn<ModS, ROW_FWD>(P.row_loge);
+        K.norm = (rns == 2) ? k_norm<2, ModS, 8> : k_norm<3, ModS, 8>;
+    }
+    return K;
+}
+
+static std::once_flag g_attr_once;
+
+// Kernels may use up to 160 KiB of LDS (gfx950).
+static void set_lds_attributes() {
+    std::call_once(g_attr_once, []() {
+        const int lds = 160 * 1024;
+        const int loges[3] = {3, 4, 5};
+        for (int L : loges) {
+            Plan P;
+            P.col_loge = L;
+            P.row_loge = L;
+            for (int is32 = 0; is32 < 2; ++is32) {
+                KernelSet K = kernels_for(3, is32, P);
+                (void)hipFuncSetAttribute((const void *)K.col_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                (void)hipFuncSetAttribute((const void *)K.col_inv, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                (void)hipFuncSetAttribute((const void *)K.row_sq, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                (void)hipFuncSetAttribute((const void *)K.row_mul, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                (void)hipFuncSetAttribute((const void *)K.row_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            }
+        }
+        (void)hipGetLastError();
+    });
+}
+
+// ------------------------------------------------------------------------------------------------
+// Engine
+
+Engine::~Engine() {
+    if (dev_ok) {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        (void)hipFree(regs); (void)hipFree(ws); (void)hipFree(zp); (void)hipFree(carry);
+        (void)hipFree(tw); (void)hipFree(crt); (void)hipFree(bdev);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (auto &ev : events) (void)hipEventDestroy(ev);
+    }
+}
+
+int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
+    b = b_; n = n_; num_regs = num_regs_; device = device_;
+    N = (size_t)1 << n;
+    select_primes(b, n, &rns, &is32);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(GFE_ENODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return set_error(GFE_ENODEV, "device index out of range");
+    if (hipSetDevice(device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice failed");
+    set_lds_attributes();
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
+    plan = default_plan(n);
+    const size_t nseg = N / plan.norm_s;
+    if (hipMalloc(&regs, (size_t)num_regs * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)rns * N * 4) != hipSuccess ||
+        hipMalloc(&zp, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&carry, nseg * 8) != hipSuccess ||
+        hipMalloc(&tw, (size_t)rns * N * 8) != hipSuccess || hipMalloc(&crt, sizeof(CrtConst)) != hipSuccess ||
+        hipMalloc(&bdev, 4) != hipSuccess) {
+        dev_ok = true;
+        return set_error(GFE_ENOMEM, "device allocation failed");
+    }
+    dev_ok = true;
+    std::vector<uint2> htw(N);
+    for (int k = 0; k < rns; ++k) {
+        make_twiddles(is32, k, n, htw.data());
+        if (hipMemcpy(tw + (size_t)k * N, htw.data(), N * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(GFE_EHIP, "twiddle upload");
+    }
+    const CrtConst c = make_crt(rns, is32, n);
+    if (hipMemcpy(crt, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "crt upload");
+    if (hipMemcpy(bdev, &b, 4, hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "b upload");
+    for (int k = 0; k < 3; ++k) { pr.p[k] = prime_of(is32, k); pr.q[k] = pinv32(pr.p[k]); }
+    if (hipMemsetAsync(regs, 0, (size_t)num_regs * N * 4, stream) != hipSuccess) return set_error(GFE_EHIP, "memset");
+    for (int i = 0; i < 12; ++i) { hipEvent_t ev; if (hipEventCreate(&ev) != hipSuccess) return set_error(GFE_EHIP, "event"); events.push_back(ev); }
+    if (hipStreamSynchronize(stream) != hipSuccess) return set_error(GFE_EHIP, "init sync");
+    return GFE_OK;
+}
+
+size_t Engine::lds_col() const {
+    return (size_t)plan.col_ng * ((1u << plan.a) + ((1u << plan.a) >> plan.col_loge)) * 4;
+}
+size_t Engine::lds_row() const {
+    const uint32_t G = 1u << (n - plan.a);
+    return (size_t)plan.row_ng * (G + (G >> plan.row_loge)) * 4;
+}
+
+// Forward column pass from register `src` into `dst` (ws or zp).
+void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
+    const KernelSet K = kernels_for(rns, is32, plan);
+    ColArgs A;
+    A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
+    A.src_stride = 0; A.ws_stride = 0; A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng; A.rns = rns;
+    const uint32_t C = 1u << (n - plan.a);
+    const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+    hipLaunchKernelGGL(K.col_fwd, dim3(C / plan.col_ng, rns, 1), dim3(T), lds_col(), stream, A);
+}
+
+void Engine::launch_col_inv() {
+    const KernelSet K = kernels_for(rns, is32, plan);
+    ColArgs A;
+    A.src = nullptr; A.ws = ws; A.tw = tw; A.pr = pr;
+    A.src_stride = 0; A.ws_stride = 0; A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng; A.rns = rns;
+    const uint32_t C = 1u << (n - plan.a);
+    const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+    hipLaunchKernelGGL(K.col_inv, dim3(C / plan.col_ng, rns, 1), dim3(T), lds_col(), stream, A);
+}
+
+void Engine::launch_row(int mode, uint32_t *data) {
+    const KernelSet K = kernels_for(rns, is32, plan);
+    RowArgs A;
+    A.ws = data; A.zp = zp; A.tw = tw; A.pr = pr; A.ws_stride = 0; A.zp_stride = 0;
+    A.n = (int)n; A.a = plan.a; A.ng = plan.row_ng;
+    const uint32_t R = 1u << plan.a, G = 1u << (n - plan.a);
+    const int T = plan.row_ng * G / (1 << plan.row_loge);
+    RowFn f = (mode == ROW_SQUARE) ? K.row_sq : (mode == ROW_MUL) ? K.row_mul : K.row_fwd;
+    hipLaunchKernelGGL(f, dim3(R / plan.row_ng, rns, 1), dim3(T), lds_row(), stream, A);
+}
+
+void Engine::launch_norm(int dup) {
+    const KernelSet K = kernels_for(rns, is32, plan);
+    NormArgs A;
+    A.ws = ws; A.dst = regs; A.carry = carry; A.crt = crt; A.b = bdev; A.dup = nullptr;
+    A.ws_stride = 0; A.dst_stride = 0; A.carry_stride = 0; A.crt_stride = 0; A.n = (int)n; A.dup_all = dup;
+    const size_t nseg = N / plan.norm_s;
+    const int T = 256;
+    hipLaunchKernelGGL(K.norm, dim3((unsigned)((nseg + T - 1) / T), 1, 1), dim3(T), 0, stream, A);
+}
+
+void Engine::launch_carry() {
+    CarryArgs A;
+    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0; A.n = (int)n;
+    A.seg_log = 3;
+    const size_t nseg = N / plan.norm_s;
+    const int T = 256;
+    hipLaunchKernelGGL(k_carry, dim3((unsigned)((nseg + T - 1) / T), 1, 1), dim3(T), 0, stream, A);
+}
+
+void Engine::square(int dup) {
+    launch_col_fwd(0, ws);
+    launch_row(ROW_SQUARE, ws);
+    launch_col_inv();
+    launch_norm(dup);
+    launch_carry();
+}
+
+void Engine::mul() {
+    launch_col_fwd(0, ws);
+    launch_row(ROW_MUL, ws);
+    launch_col_inv();
+    launch_norm(0);
+    launch_carry();
+}
+
+void Engine::init_multiplicand(uint32_t src) {
+    launch_col_fwd(src, zp);
+    launch_row(ROW_FWD, zp);
+}
+
+void Engine::mul_scalar(int32_t a) {
+    MulScalarArgs A;
+    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0; A.n = (int)n; A.seg_log = 3;
+    A.a = a;
+    const size_t nseg = N / plan.norm_s;
+    const int T = 256;
+    hipLaunchKernelGGL(k_mulscalar, dim3((unsigned)((nseg + T - 1) / T), 1, 1), dim3(T), 0, stream, A);
+    launch_carry();
+}
+
+void Engine::set(uint32_t a) {
+    const int T = 256;
+    hipLaunchKernelGGL(k_set, dim3((unsigned)((N + T - 1) / T), 1, 1), dim3(T), 0, stream, regs, 0ll, N, (int32_t)a);
+}
+
+int Engine::check(const char *what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(GFE_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return GFE_OK;
+}
+
+}  // namespace gfe
+
+// ================================================================================================
+// C ABI
+
+using gfe::Engine;
+using gfe::set_error;
+
+struct gfe_engine {
+    Engine eng;
+};
+
+#define GFE_GUARD(e)                                                         \
+    do {                                                                     \
+        if (!(e)) return set_error(GFE_EINVAL, "null engine");               \
+        if (hipSetDevice((e)->eng.device) != hipSuccess)                     \
+            return set_error(GFE_EHIP, "hipSetDevice failed");               \
+    } while (0)
+
+extern "C" {
+
+const char *gfe_last_error(void) { return gfe::g_last_error.c_str(); }
+
+int gfe_select(uint32_t b, uint32_t n, int *rns, int *is32) {
+    if (!rns || !is32) return set_error(GFE_EINVAL, "null output");
+    gfe::select_primes(b, n, rns, is32);
+    return GFE_OK;
+}
+
+int gfe_create(uint32_t b, uint32_t n, uint32_t num_regs, int device, gfe_engine **out) {
+    if (!out) return set_error(GFE_EINVAL, "null output");
+    *out = nullptr;
+    if (n < 12 || n > 23) return set_error(GFE_EINVAL, "n must be in [12, 23]");
+    if (b < 1024 || b > 2000000000u) return set_error(GFE_EINVAL, "b must be in [1024, 2000000000]");
+    if (num_regs < 1 || num_regs > 4096) return set_error(GFE_EINVAL, "num_regs must be in [1, 4096]");
+    gfe_engine *e = new gfe_engine;
+    const int rc = e->eng.init(b, n, num_regs, device);
+    if (rc != GFE_OK) {
+        const std::string msg = gfe::g_last_error;
+        delete e;
+        return set_error(rc, msg);
+    }
+    *out = e;
+    return GFE_OK;
+}
+
+int gfe_destroy(gfe_engine *e) {
+    if (!e) return GFE_OK;
+    delete e;
+    return GFE_OK;
+}
+
+int gfe_set(gfe_engine *e, uint32_t a) {
+    GFE_GUARD(e);
+    e->eng.set(a);
+    return e->eng.check("set");
+}
+
+int gfe_square_dup(gfe_engine *e, int dup) {
+    GFE_GUARD(e);
+    e->eng.square(dup != 0);
+    return e->eng.check("squareDup");
+}
+
+int gfe_square_mul(gfe_engine *e, int32_t a) {
+    GFE_GUARD(e);
+    // transformGPUs::squareMul, src/transformGPU.h:1163-1171 (a <= 2 -> squareDup(a == 2))
+    if (a <= 2) {
+        e->eng.square(a == 2);
+    } else {
+        e->eng.square(0);
+        e->eng.mul_scalar(a);
+    }
+    return e->eng.check("squareMul");
+}
+
+int gfe_init_multiplicand(gfe_engine *e, uint32_t src) {
+    GFE_GUARD(e);
+    if (src >= e->eng.num_regs) return set_error(GFE_EINVAL, "register index out of range");
+    e->eng.init_multiplicand(src);
+    return e->eng.check("initMultiplicand");
+}
+
+int gfe_mul(gfe_engine *e) {
+    GFE_GUARD(e);
+    e->eng.mul();
+    return e->eng.check("mul");
+}
+
+int gfe_copy(gfe_engine *e, uint32_t dst, uint32_t src) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (dst >= g.num_regs || src >= g.num_regs) return set_error(GFE_EINVAL, "register index out of range");
+    if (dst == src) return GFE_OK;
+    if (hipMemcpyAsync(g.regs + (size_t)dst * g.N, g.regs + (size_t)src * g.N, g.N * 4, hipMemcpyDeviceToDevice,
+                       g.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "copy");
+    return GFE_OK;
+}
+
+int gfe_get_digits(gfe_engine *e, uint32_t reg, int32_t *out) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (!out) return set_error(GFE_EINVAL, "null output");
+    if (reg >= g.num_regs) return set_error(GFE_EINVAL, "register index out of range");
+    if (hipMemcpyAsync(out, g.regs + (size_t)reg * g.N, g.N * 4, hipMemcpyDeviceToHost, g.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "read");
+    if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "read sync");
+    return GFE_OK;
+}
+
+int gfe_set_digits(gfe_engine *e, uint32_t reg, const int32_t *in) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (!in) return set_error(GFE_EINVAL, "null input");
+    if (reg >= g.num_regs) return set_error(GFE_EINVAL, "register index out of range");
+    // digits must be bounded as the engine keeps them (|d| <= b/2 + 4); larger values are the
+    // caller's to balance first (transform::setInt balances, src/transform.h:241-246)
+    const int64_t bound = (int64_t)(g.b / 2) + 4;
+    for (size_t i = 0; i < g.N; ++i)
+        if ((int64_t)in[i] > bound || (int64_t)in[i] < -bound) return set_error(GFE_EINVAL, "digit out of balanced range");
+    if (hipMemcpyAsync(g.regs + (size_t)reg * g.N, in, g.N * 4, hipMemcpyHostToDevice, g.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "write");
+    if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "write sync");
+    return GFE_OK;
+}
+
+size_t gfe_mem_size(const gfe_engine *e) {
+    if (!e) return 0;
+    const Engine &g = e->eng;
+    return (size_t)g.num_regs * g.N * 4 + 2 * (size_t)g.rns * g.N * 4 + (size_t)g.rns * g.N * 8 + g.N / 8 * 8;
+}
+
+size_t gfe_cache_size(const gfe_engine *e) {
+    if (!e) return 0;
+    const Engine &g = e->eng;
+    return (size_t)g.rns * g.N * 4 + g.N * 4;
+}
+
+static int engine_kind(const Engine &g) { return 0x47460000 | (g.rns << 4) | g.is32; }
+
+size_t gfe_context_size(const gfe_engine *e, uint32_t num_regs) {
+    if (!e) return 0;
+    return sizeof(int) + (size_t)num_regs * e->eng.N * 4;
+}
+
+int gfe_save(gfe_engine *e, uint32_t num_regs, void *buf, size_t size) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (num_regs == 0) num_regs = g.num_regs;
+    if (num_regs > g.num_regs) return set_error(GFE_EINVAL, "num_regs");
+    if (!buf || size < gfe_context_size(e, num_regs)) return set_error(GFE_EINVAL, "buffer too small");
+    const int kind = engine_kind(g);
+    memcpy(buf, &kind, sizeof(kind));
+    if (hipMemcpyAsync((char *)buf + sizeof(int), g.regs, (size_t)num_regs * g.N * 4, hipMemcpyDeviceToHost, g.stream) !=
+        hipSuccess)
+        return set_error(GFE_EHIP, "save");
+    if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "save sync");
+    return GFE_OK;
+}
+
+int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (num_regs == 0) num_regs = g.num_regs;
+    if (num_regs > g.num_regs) return set_error(GFE_EINVAL, "num_regs");
+    if (!buf || size < sizeof(int)) return set_error(GFE_EINVAL, "short context");
+    int kind;
+    memcpy(&kind, buf, sizeof(kind));
+    if (kind != engine_kind(g)) return set_error(GFE_EKIND, "context of another engine kind");
+    if (size < gfe_context_size(e, num_regs)) return set_error(GFE_EINVAL, "short context");
+    if (hipMemcpyAsync(g.regs, (const char *)buf + sizeof(int), (size_t)num_regs * g.N * 4, hipMemcpyHostToDevice,
+                       g.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "load");
+    if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "load sync");
+    return GFE_OK;
+}
+
+double gfe_get_error(const gfe_engine *) { return 0.0; }
+
+int gfe_info(const gfe_engine *e, char *buf, size_t len) {
+    if (!e || !buf || len == 0) return set_error(GFE_EINVAL, "info");
+    const Engine &g = e->eng;
+    std::ostringstream ss;
+    ss << "NTT" << g.rns << (g.is32 ? "u" : "s") << " n=" << g.n << " b=" << g.b << "; " << gfe::plan_str(g.plan, g.n);
+    snprintf(buf, len, "%s", ss.str().c_str());
+    return GFE_OK;
+}
+
+int gfe_sync(gfe_engine *e) {
+    GFE_GUARD(e);
+    if (hipStreamSynchronize(e->eng.stream) != hipSuccess) return set_error(GFE_EHIP, "sync");
+    return e->eng.check("sync");
+}
+
+int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double *ms, double *kernel_ms) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (!ms || count == 0) return set_error(GFE_EINVAL, "time_squarings");
+    hipEvent_t e0 = g.events[0], e1 = g.events[1];
+    (void)hipEventRecord(e0, g.stream);
+    for (uint32_t i = 0; i < count; ++i) g.square((dup_bits >> (i % 64)) & 1);
+    (void)hipEventRecord(e1, g.stream);
+    if (hipEventSynchronize(e1) != hipSuccess) return set_error(GFE_EHIP, "event sync");
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t;
+    if (kernel_ms) {
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (uint32_t i = 0; i < count; ++i) {
+            const int dup = (dup_bits >> (i % 64)) & 1;
+            (void)hipEventRecord(g.events[2], g.stream);
+            g.launch_col_fwd(0, g.ws);
+            (void)hipEventRecord(g.events[3], g.stream);
+            g.launch_row(gfe::ROW_SQUARE, g.ws);
+            (void)hipEventRecord(g.events[4], g.stream);
+            g.launch_col_inv();
+            (void)hipEventRecord(g.events[5], g.stream);
+            g.launch_norm(dup);
+            (void)hipEventRecord(g.events[6], g.stream);
+            g.launch_carry();
+            (void)hipEventRecord(g.events[7], g.stream);
+            if (hipEventSynchronize(g.events[7]) != hipSuccess) return set_error(GFE_EHIP, "event sync");
+            for (int k = 0; k < 5; ++k) {
+                float d = 0;
+                (void)hipEventElapsedTime(&d, g.events[2 + k], g.events[3 + k]);
+                acc[k] += d;
+            }
+        }
+        for (int k = 0; k < 5; ++k) kernel_ms[k] = acc[k] / count;
+    }
+    return g.check("time_squarings");
+}
+
+// The autotuner: every admissible plan is timed on `reps` squarings of a pseudo-random balanced
+// residue (register 0 is saved and restored); the fastest is kept.
+int gfe_tune(gfe_engine *e, uint32_t reps) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (reps == 0) reps = 8;
+    std::vector<int32_t> saved(g.N), trial(g.N);
+    if (hipMemcpyAsync(saved.data(), g.regs, g.N * 4, hipMemcpyDeviceToHost, g.stream) != hipSuccess ||
+        hipStreamSynchronize(g.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "tune save");
+    uint64_t x = 0x9E3779B97F4A7C15ull ^ g.b;
+    const int64_t half = g.b / 2;
+    for (size_t i = 0; i < g.N; ++i) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        trial[i] = (int32_t)((int64_t)(x % g.b) - half + ((g.b & 1) ? 0 : 1));
+    }
+    std::vector<gfe::Plan> plans;
+    for (int a = 4; a <= (int)g.n - 4; ++a)
+        for (int cl = 3; cl <= 5; ++cl)
+            for (int cng = 1; cng <= 64; cng *= 2)
+                for (int rl = 3; rl <= 5; ++rl)
+                    for (int rng = 1; rng <= 64; rng *= 2) {
+                        gfe::Plan P;
+                        P.a = a; P.col_loge = cl; P.col_ng = cng; P.row_loge = rl; P.row_ng = rng; P.norm_s = 8;
+                        const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (g.n - a)) / (1 << rl);
+                        if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 512) continue;
+                        if (!gfe::plan_valid(P, g.n)) continue;
+                        plans.push_back(P);
+                    }
+    const gfe::Plan orig = g.plan;
+    gfe::Plan best = orig;
+    double best_ms = 1e30;
+    for (const gfe::Plan &P : plans) {
+        g.plan = P;
+        if (hipMemcpyAsync(g.regs, trial.data(), g.N * 4, hipMemcpyHostToDevice, g.stream) != hipSuccess)
+            return set_error(GFE_EHIP, "tune upload");
+        g.square(0);
+        (void)hipEventRecord(g.events[0], g.stream);
+        for (uint32_t i = 0; i < reps; ++i) g.square(0);
+        (void)hipEventRecord(g.events[1], g.stream);
+        if (hipEventSynchronize(g.events[1]) != hipSuccess) {
+            g.plan = orig;
+            return set_error(GFE_EHIP, "tune sync");
+        }
+        if (hipGetLastError() != hipSuccess) continue;
+        float t = 0;
+        (void)hipEventElapsedTime(&t, g.events[0], g.events[1]);
+        if (t < best_ms) { best_ms = t; best = P; }
+    }
+    g.plan = best;
+    if (hipMemcpyAsync(g.regs, saved.data(), g.N * 4, hipMemcpyHostToDevice, g.stream) != hipSuccess ||
+        hipStreamSynchronize(g.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "tune restore");
+    return g.check("tune");
+}
+
+}  // extern "C"

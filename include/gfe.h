@@ -1,0 +1,93 @@
+/* gfe.h -- C ABI of the gfx950 modular-squaring engine for b^(2^n) + 1.
+ *
+ * Drop-in boundary for genefer's GPU transform (the reference's `transform` interface,
+ * src/transform.h:42-256, implemented on the GPU by transformGPUs, src/transformGPU.h:931-1195).
+ * Plain C types only; every entry point returns GFE_OK (0) or a negative error code and never
+ * throws; gfe_last_error() describes the last failure of the calling thread.
+ *
+ * Register model (src/transform.h:54-73): num_regs registers of N = 2^n digits; register 0 is the
+ * working value r0; rm is the multiplicand set by gfe_init_multiplicand.  Registers hold signed
+ * digits in base b with |digit| <= b/2 + 4 (a bounded, non-canonical representation, as the
+ * reference's GPU registers are); gfe_get_digits returns them as they are and the caller
+ * canonicalises (gint::unbalance, src/gint.h:40-102).
+ *
+ * All work is queued on the engine's own HIP stream; gfe_get_digits / gfe_save / gfe_sync wait
+ * for it.  One engine must not be used from two threads at once (as the reference's engines,
+ * src/transformGPU.h:205-208).
+ */
+#ifndef GFE_H
+#define GFE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFE_OK 0
+#define GFE_EINVAL (-1)     /* bad argument (n, b, register index, size) */
+#define GFE_EHIP (-2)       /* HIP runtime error */
+#define GFE_ENOMEM (-3)     /* device allocation failed */
+#define GFE_EKIND (-4)      /* context of another engine kind */
+#define GFE_ENODEV (-5)     /* no such device / no HIP device */
+
+typedef struct gfe_engine gfe_engine;
+
+/* transform::create_gpu (src/transform.h:114-120) -> create_ocl (src/transform_ocl.cpp:20-52):
+ * the prime set (2 or 3 primes of 31 or 32 bits) is chosen from (b, n) by the reference's rule.
+ * 12 <= n <= 23 as the reference's command line allows (src/main.cpp:258-259), 1024 <= b <= 2e9. */
+int gfe_create(uint32_t b, uint32_t n, uint32_t num_regs, int device, gfe_engine **out);
+int gfe_destroy(gfe_engine *e);
+
+/* transform::set, src/transform.h:57 -- r0 = a. */
+int gfe_set(gfe_engine *e, uint32_t a);
+/* transform::squareDup, src/transform.h:58 -- r0 = r0^2, or 2 r0^2 when dup != 0. */
+int gfe_square_dup(gfe_engine *e, int dup);
+/* transform::squareMul, src/transform.h:59 -- r0 = a r0^2. */
+int gfe_square_mul(gfe_engine *e, int32_t a);
+/* transform::initMultiplicand, src/transform.h:60 -- rm = transform(r_src). */
+int gfe_init_multiplicand(gfe_engine *e, uint32_t src);
+/* transform::mul, src/transform.h:61 -- r0 = r0 * rm. */
+int gfe_mul(gfe_engine *e);
+/* transform::copy, src/transform.h:63 -- r_dst = r_src. */
+int gfe_copy(gfe_engine *e, uint32_t dst, uint32_t src);
+/* transform::getZi / setZi, src/transform.h:53-54 (getInt/setInt 234-246 add the size/base checks
+ * and gint canonicalisation on the caller side); any register. */
+int gfe_get_digits(gfe_engine *e, uint32_t reg, int32_t *out);
+int gfe_set_digits(gfe_engine *e, uint32_t reg, const int32_t *in);
+/* transform::getMemSize / getCacheSize, src/transform.h:65-66. */
+size_t gfe_mem_size(const gfe_engine *e);
+size_t gfe_cache_size(const gfe_engine *e);
+/* transform::saveContext / readContext, src/transform.h:68-69 (payload of the context file after
+ * the driver's header: int kind, then the num_regs registers).  gfe_context_size gives the bytes
+ * gfe_save writes.  gfe_load returns GFE_EKIND for a payload of another kind (the reference's
+ * readContext returns false, src/transformGPU.h:1127-1139). */
+size_t gfe_context_size(const gfe_engine *e, uint32_t num_regs);
+int gfe_save(gfe_engine *e, uint32_t num_regs, void *buf, size_t size);
+int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size);
+/* transform::getError, src/transform.h:71 -- always 0: the arithmetic is exact. */
+double gfe_get_error(const gfe_engine *e);
+/* transform::info, src/transform.h:72 -- prime set and pass plan as text. */
+int gfe_info(const gfe_engine *e, char *buf, size_t len);
+/* Waits for the engine's queued work. */
+int gfe_sync(gfe_engine *e);
+/* Prime-set selection without creating an engine: *rns = 2 or 3, *is32 = 0 (31-bit) or 1 (32-bit). */
+int gfe_select(uint32_t b, uint32_t n, int *rns, int *is32);
+const char *gfe_last_error(void);
+
+/* Times `count` squarings (dup pattern: dup_bits bit i for squaring i mod 64) with HIP events on
+ * the engine's stream, after the queued work.  *ms = total; kernel_ms[0..4] = average duration of
+ * one launch of each pass (column forward, row, column inverse, normalise, carry), measured in a
+ * second, per-launch-timed run of the same count (kernel_ms may be null). */
+int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double *ms, double *kernel_ms);
+
+/* Autotuner: times every admissible pass plan for this engine's n on the device and keeps the
+ * fastest (the reference's splitter + engines::tune, src/ocl.h:31-90, src/transformGPU.h:851-900).
+ * `reps` squarings per plan.  The chosen plan is reported by gfe_info. */
+int gfe_tune(gfe_engine *e, uint32_t reps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,49 @@
+/* gfe_driver.h -- C ABI of the PRP driver that runs on the squaring engine (genefer's caller side,
+ * src/genefer.h:441-766).  Used by tests and bench.py; a genefer-style C++ driver uses the
+ * transform interface (genefer22_amd/csrc/transform_hip.h) directly instead.
+ * Same conventions as gfe.h: GFE_OK or a negative code, gfe_last_error() for the message. */
+#ifndef GFE_DRIVER_H
+#define GFE_DRIVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gfe_result {
+    uint64_t res64;    /* canonical residue mod 2^64 (gint::isOne, src/gint.h:169-202) */
+    uint64_t old64;    /* low bytes of the 8 top canonical digits */
+    uint64_t hash64;   /* gint::gethash64 of the canonical residue, 0 when it is 1 */
+    uint64_t pkey;     /* Pietrzak-Li proof key (gfe_run_proof), else 0 */
+    uint64_t esize;    /* bits of E = b^(2^n) */
+    uint64_t iters;    /* squarings of the PRP loop that ran */
+    int32_t is_prp;    /* residue == 1 */
+    int32_t gl_ok;     /* Gerbicz-Li check: 1 pass, 0 fail, -1 not run */
+    double seconds;    /* wall time of the run */
+} gfe_result;
+
+/* genefer's quick test (quick(), src/genefer.h:736-748): PRP of b^(2^n)+1 in base 2 with the
+ * Gerbicz-Li check at the reference's cadence B_GL = B_GerbiczLi(esize). */
+int gfe_run_quick(uint32_t b, uint32_t n, int device, gfe_result *out);
+
+/* The first `iters` squarings of the PRP loop (all when iters == 0 or >= esize), with the d(t)
+ * update every b_gl iterations (0 -> the reference's cadence, < 0 -> none).  When digits_out is not
+ * null it receives the canonical digits of r0 (N int32). */
+int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl, int32_t *digits_out,
+                gfe_result *out);
+
+/* proof(), src/genefer.h:750-766: PRP with Pietrzak-Li checkpoints (in registers for n <= 17 as the
+ * reference's fast_checkpoints, host copies otherwise), the Gerbicz-Li check, then the proof.
+ * mu_hash64 (may be null) receives gethash64 of mu_0..mu_depth (depth + 1 values). */
+int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_hash64, gfe_result *out);
+
+/* Canonical residue of a digit vector with the product's gint (src/gint.h:40-225):
+ * canon (may be null) receives the canonical digits. */
+int gfe_residue(const int32_t *digits, uint32_t n, uint32_t b, int32_t *canon, gfe_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+"""GPU parity of the HIP squaring engine against the oracle (C restatement of the reference's
+exact path) and the reference's known-answer RES64 values.  Bit-exact: canonical residues equal."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# (b, n, expected prime set) -- one case per prime set of create_ocl (src/transform_ocl.cpp:35-50)
+CASES = [
+    (1534, 12, (2, False)),        # NTT2, 31-bit primes
+    (30000000, 12, (2, True)),     # NTT2, 32-bit primes (2.34e7 <= b < 4.57e7)
+    (999999618, 12, (3, False)),   # NTT3, 31-bit primes
+    (1999999266, 12, (3, True)),   # NTT3, 32-bit primes
+]
+
+
+def prp_bits(b, n, iters):
+    E = b ** (1 << n)
+    es = E.bit_length()
+    return [((E >> i) & 1) for i in range(es - 1, es - 1 - iters, -1)]
+
+
+def canon(O, d, n, b):
+    return O.unbalance(d, n, b)
+
+
+@pytest.mark.parametrize("b,n,sel", CASES)
+def test_prime_set_selection(gfe, oracle, b, n, sel):
+    assert gfe.select(b, n) == sel
+    assert oracle.select(b, n) == sel
+
+
+@pytest.mark.parametrize("b,n,sel", CASES)
+def test_square_dup_matches_oracle(gfe, oracle, b, n, sel):
+    """r0 after k PRP iterations (squareDup with the exponent bits) equals the oracle's."""
+    iters = 300
+    bits = prp_bits(b, n, iters)
+    t = gfe.Transform(b, n)
+    o = oracle.Engine(b, n)
+    t.set(1); o.set(1)
+    for k, bit in enumerate(bits):
+        t.square_dup(bit)
+        o.square_dup(bit)
+        if k in (0, 1, 2, 9, 99, iters - 1):
+            got = t.get_digits(0)
+            assert np.abs(got).max() <= b // 2 + 4, "digits left the balanced bound"
+            assert np.array_equal(canon(oracle, got, n, b), canon(oracle, o.get_digits(0), n, b)), f"iteration {k}"
+
+
+@pytest.mark.parametrize("b,n", [(1000000000, 13), (40000, 14), (123456790, 16), (2000000000, 16), (8000000, 18), (30000000, 20)])
+def test_larger_n_matches_oracle(gfe, oracle, b, n):
+    iters = {13: 40, 14: 30, 16: 12, 18: 5, 20: 3}[n]
+    bits = prp_bits(b, n, iters)
+    t = gfe.Transform(b, n)
+    o = oracle.Engine(b, n)
+    t.set(1); o.set(1)
+    for bit in bits:
+        t.square_dup(bit); o.square_dup(bit)
+    assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b))
+
+
+@pytest.mark.parametrize("b,n", [(1534, 12), (999999618, 12), (1999999266, 12)])
+def test_mul_copy_multiplicand(gfe, oracle, b, n):
+    """initMultiplicand / mul / copy / squareMul on random balanced residues."""
+    rs = np.random.RandomState(b % 1000)
+    N = 1 << n
+    a = oracle.balance(rs.randint(0, b, N).astype(np.int32), n, b)
+    c = oracle.balance(rs.randint(0, b, N).astype(np.int32), n, b)
+    t = gfe.Transform(b, n, num_regs=3)
+    o = oracle.Engine(b, n, num_regs=3)
+    t.set_digits(a, 0); o.set_digits(a, 0)
+    t.set_digits(c, 1); o.set_digits(c, 1)
+    t.mul(1); o.mul(1)
+    assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b))
+    t.copy(2, 0); o.copy(2, 0)
+    t.square_mul(7); o.square_mul(7)
+    assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b))
+    assert np.array_equal(canon(oracle, t.get_digits(2), n, b), canon(oracle, o.get_digits(2), n, b))
+
+
+def test_known_answer_quick_n12(gfe):
+    """Full quick test (PRP + Gerbicz-Li) against the reference's KATs (src/main.cpp:424-425)."""
+    for b, want in [(1000000000, 0x4E43A5B93273C649), (2000000000, 0xE5F3EA34F6C68EBD)]:
+        r = gfe.run_quick(b, 12)
+        assert r["res64"] == want, hex(r["res64"])
+        assert r["gl_ok"] == 1
