@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Headline benchmark of the gfx950 squaring engine for b^(2^n) + 1.
+
+One JSON line on stdout (progress on stderr):
+  * value      : ns per squaring of one candidate at n = 22 (N = 4,194,304 digits), b seeded in
+                 [1e7, 3.5e7] (prime set NTT3 31-bit), residue resident in HBM, W warm-up then K timed
+                 squarings (one step = one squaring = column pass, row pass, column pass, CRT + carry),
+                 timed with HIP events on the engine's stream;
+  * roofline   : the dominant kernel's algorithmic bytes (one read + one write of the rns x N uint32
+                 residues, 8 rns N bytes per launch) over its average launch duration, against 8 TB/s;
+  * valu       : the same kernel's butterfly rate and estimated VALU-slot use (integer-bound path);
+  * batched_n15: candidates/hour at n = 15, 512 seeded candidates in one batch vs one candidate per
+                 launch, from timed iteration blocks (full-test length = esize squarings);
+  * parity     : RES64 after P PRP iterations on the GPU vs the reference's own CPU transform;
+  * cpu_baseline: the reference's CPU transform (compiled from its sources into oracle/_ref) timed on
+                 this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ns per squaring at n=22; candidates/hour at n=15 batched vs plain, bit-exact"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def splitmix64(state: int):
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield z ^ (z >> 31)
+
+
+def seeded_bases(seed: int, lo: int, hi: int, count: int) -> list[int]:
+    """Even b uniform in [lo, hi), b >= 1024, not a power of two (src/main.cpp:249-253)."""
+    out, g = [], splitmix64(seed)
+    while len(out) < count:
+        b = lo + 2 * ((next(g) % ((hi - lo) // 2)))
+        if b >= 1024 and (b & (b - 1)) != 0:
+            out.append(b)
+    return out
+
+
+def esize_of(b: int, n: int) -> int:
+    """Bits of E = b^(2^n) (the number of squarings of a full test)."""
+    return (b ** (1 << n)).bit_length()
+
+
+def bench_n22(G, steps: int, warmup: int):
+    b = seeded_bases(22, 10_000_000, 35_000_000, 1)[0]
+    n = 22
+    t = G.Transform(b, n)
+    log(f"[bench] n=22 b={b}: tuning")
+    t.tune(6)
+    plan = t.info()
+    log(f"[bench] plan: {plan}")
+    t.set(1)
+    dup_pattern = 0x9B4D_63A1_5C27_E8F0
+    t.time_squarings(max(warmup, 1), dup_pattern)
+    ms = t.time_squarings(steps, dup_pattern)
+    kreps = min(steps, 200)
+    _, km = t.time_squarings(kreps, dup_pattern, per_kernel=True)
+    names = ["col_fwd", "row", "col_inv", "norm"]
+    rns = 3 if "NTT3" in plan else 2
+    N = 1 << n
+    dom = max(range(4), key=lambda i: km[i])
+    alg_bytes = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 8 * rns * N, 3: 4 * rns * N + 4 * N}[dom]
+    achieved = alg_bytes / (km[dom] * 1e-3) / 1e9
+    # butterflies of the dominant pass (row: (n - a) levels fwd + inv, N/2 per level per prime)
+    a = int(plan.split("cols 2^")[1].split(" ")[0])
+    levels = {0: a, 1: 2 * (n - a), 2: a}.get(dom, 0)
+    bfly = rns * levels * (N // 2)
+    bfly_rate = bfly / (km[dom] * 1e-3) if levels else 0.0
+    slots = 14.0  # VALU issue slots of one butterfly (2 half-rate mul_lo/mul_hi + 1 mad_u64 + 8 adds/mins)
+    valu_peak_slots = 256 * 128 * 2.4e9
+    res = {
+        "b": b, "plan": plan, "ms_per_step": ms / steps, "ns_per_squaring": ms / steps * 1e6,
+        "kernel_us": {names[i]: round(km[i] * 1e3, 2) for i in range(4)},
+        "roofline": {
+            "bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "alg_bytes_per_launch": alg_bytes,
+        },
+        "valu": {
+            "kernel": names[dom], "butterflies_per_s": bfly_rate, "slots_per_butterfly": slots,
+            "achieved_slots_per_s": bfly_rate * slots, "peak_slots_per_s": valu_peak_slots,
+            "frac": round(bfly_rate * slots / valu_peak_slots, 4),
+        },
+    }
+    t.close()
+    return res
+
+
+def bench_batched_n15(G, count: int, block: int):
+    """Candidates/hour at n = 15: `count` seeded candidates in one batch vs one per launch."""
+    if not hasattr(G, "Batch"):
+        return None
+    n = 15
+    bases = seeded_bases(15, 100_000_000, 2_000_000_000, count)
+    mean_es = sum(esize_of(b, n) for b in bases) / count
+    log(f"[bench] n=15 batched: {count} candidates, mean esize {mean_es:.0f}")
+    bt = G.Batch(n, bases)
+    bt.start_prp()
+    bt.step_squarings(block)  # warm-up
+    t_b = bt.time_squarings(4 * block)
+    per_iter_b = t_b / (4 * block)
+    bt.close()
+    # plain: one candidate per launch (the transform engine), same seeds, sampled
+    plain_ms = []
+    for b in bases[:4]:
+        t = G.Transform(b, n)
+        t.tune(4)
+        t.set(1)
+        t.time_squarings(block)
+        plain_ms.append(t.time_squarings(4 * block) / (4 * block))
+        t.close()
+    per_sq_plain = sum(plain_ms) / len(plain_ms)
+    cand_h_batched = 3600.0 / (per_iter_b * 1e-3 * mean_es) * count
+    cand_h_plain = 3600.0 / (per_sq_plain * 1e-3 * mean_es)
+    return {
+        "candidates": count, "mean_esize": round(mean_es), "block": block,
+        "batched_ms_per_iteration": per_iter_b, "plain_ms_per_squaring": per_sq_plain,
+        "candidates_per_hour_batched": round(cand_h_batched, 1),
+        "candidates_per_hour_plain": round(cand_h_plain, 1),
+        "speedup": round(cand_h_batched / cand_h_plain, 2),
+    }
+
+
+def parity_n22(G, O, b: int, iters: int, threads: int):
+    """RES64 after `iters` PRP iterations: GPU driver vs the reference CPU transform."""
+    g = G.run_prp(b, 22, iters=iters, b_gl=-1)
+    ref = O.RefEngine(b, 22, threads=threads)
+    es, r, _ = ref.prp(iters=iters, b_gl=-1)
+    err = ref.error()
+    return {"iters": iters, "gpu_res64": f"{g['res64']:016X}", "ref_res64": f"{r.res64:016X}",
+            "match": g["res64"] == r.res64, "ref_impl": ref.impl, "ref_max_error": err}
+
+
+def cpu_baseline(O, b: int, threads: int, budget_s: float):
+    """The reference's CPU transform at n = 22 (same b), squarings timed for about budget_s."""
+    ref = O.RefEngine(b, 22, threads=threads)
+    ref.set(1)
+    for _ in range(3):
+        ref.square_dup(False)
+    count, t0 = 0, time.perf_counter()
+    while True:
+        ref.square_dup(count & 1)
+        count += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or count >= 5000:
+            break
+    return {"value": round(el / count * 1e9, 1), "unit": "ns/squaring", "cores": threads, "kind": "reference",
+            "sample": f"{count} squarings of one n=22 candidate (b={b}) with the reference's '{ref.impl}' CPU "
+                      f"transform, {threads} thread(s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the n=15 batched leg")
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): candidates are independent and shard "
+              f"across GPUs with no exchange; run one process per GPU instead.", file=sys.stderr)
+        sys.exit(2)
+
+    import genefer22_amd as G
+    G.lib()
+    res = bench_n22(G, args.steps, args.warmup)
+    log(f"[bench] n=22: {res['ns_per_squaring']:.0f} ns/squaring; kernels {res['kernel_us']}")
+
+    batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64)
+    if batched:
+        log(f"[bench] n=15: {batched}")
+
+    threads = min(16, os.cpu_count() or 1)
+    parity = cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        if O.RefEngine.available():
+            parity = parity_n22(G, O, res["b"], 200, threads)
+            log(f"[bench] parity: {parity}")
+            cpu = cpu_baseline(O, res["b"], threads, args.cpu_seconds)
+            log(f"[bench] cpu baseline: {cpu}")
+        else:
+            log("[bench] oracle/_ref/libgfref.so missing: no CPU baseline")
+
+    line = {
+        "metric": METRIC,
+        "value": round(res["ns_per_squaring"], 1),
+        "unit": "ns/squaring",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 6),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded b; residue 2^k mod b^N+1 from set(1) and a fixed dup pattern)",
+        "config": {
+            "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] (NTT3, 31-bit primes)",
+            "n": 22, "b": res["b"], "plan": res["plan"], "kernel_us": res["kernel_us"],
+            "batched_n15": batched, "parity": parity,
+        },
+        "roofline": res["roofline"],
+        "valu": res["valu"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

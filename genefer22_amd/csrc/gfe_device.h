@@ -1,13 +1,14 @@
 // gfe_device.h -- device-side modular arithmetic, CRT and carry for the gfx950 squaring engine.
 //
 // Residues are plain integers in [0, p) (no Montgomery form between operations).
-//  * S primes (p < 2^31): twiddle products use Shoup's method with a precomputed quotient
-//    w' = floor(w 2^32 / p): one mul_hi, one mul_lo and a v_mad_u64_u32 (all half-rate on gfx950)
-//    plus a min-based correction.
-//  * U primes (2^31 < p < 2^32): 2p does not fit in 32 bits, so twiddle products use Montgomery
-//    reduction with the twiddle stored as w 2^32 mod p, which returns the plain product a w.
-//  * data x data products (the pointwise square / product) use Montgomery reduction for both
-//    sets; the resulting factor 2^-32 is folded into the per-prime scale applied before the CRT.
+//  * products by a constant (twiddles, CRT constants) take the constant in Montgomery form
+//    w 2^32 mod p, so that Montgomery reduction returns the plain product a w: one v_mad_u64_u32,
+//    one mul_lo and one mul_hi (all half-rate on gfx950) plus a correction -- the same issue cost
+//    as Shoup's method, with 4-byte instead of 8-byte twiddles;
+//  * data x data products (the pointwise square / product) use the same reduction; the resulting
+//    factor 2^-32 is folded into the per-prime scale applied before the CRT.
+//  * S primes (p < 2^31) correct with unsigned min() tricks; U primes (p > 2^31, 2p >= 2^32) with
+//    compares.
 // The reference's arithmetic for the same primes is ocl/kernel.cl:94-126 (Montgomery on data kept
 // in Montgomery form); the residues computed here are the same integers mod p.
 #pragma once
@@ -30,12 +31,8 @@ struct ModS {  // p < 2^31
         const uint32_t d = a - b;
         return min(d, d + p);
     }
-    // a * w mod p; a is any 32-bit value, tw = (w, floor(w 2^32 / p)).
-    __device__ __forceinline__ uint32_t mulw(uint32_t a, uint2 tw) const {
-        const uint32_t qq = __umulhi(a, tw.y);
-        const uint32_t r = a * tw.x - qq * p;
-        return min(r, r - p);
-    }
+    // a * w mod p; a is any 32-bit value, wm = w 2^32 mod p.
+    __device__ __forceinline__ uint32_t mulw(uint32_t a, uint32_t wm) const { return mont(a, wm); }
     // a * b * 2^-32 mod p, a < 2^32, b < p.
     __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b) const {
         const uint64_t t = (uint64_t)a * b;
@@ -62,8 +59,8 @@ struct ModU {  // 2^31 < p < 2^32
         const uint32_t r = hi - mh;
         return (hi < mh) ? r + p : r;
     }
-    // a * w mod p with tw.y = w 2^32 mod p (Montgomery form of w).
-    __device__ __forceinline__ uint32_t mulw(uint32_t a, uint2 tw) const { return mont(a, tw.y); }
+    // a * w mod p with wm = w 2^32 mod p (Montgomery form of w).
+    __device__ __forceinline__ uint32_t mulw(uint32_t a, uint32_t wm) const { return mont(a, wm); }
 };
 
 // Residue of a signed digit.
@@ -78,10 +75,10 @@ __device__ __forceinline__ uint32_t to_res(int32_t d, uint32_t p) {
 struct CrtConst {
     uint32_t p[3];
     uint32_t q[3];
-    uint2 scale[3];     // multiplies the raw inverse-transform output: 2^32 N^-1 (see header)
-    uint2 inv_p1_m2;    // 1/p1 mod p2
-    uint2 p1_m3;        // p1 mod p3
-    uint2 inv_p12_m3;   // 1/(p1 p2) mod p3
+    uint32_t scale[3];     // multiplies the raw inverse-transform output by 2^32 N^-1 (Montgomery form)
+    uint32_t inv_p1_m2;    // 1/p1 mod p2 (Montgomery form)
+    uint32_t p1_m3;        // p1 mod p3 (Montgomery form)
+    uint32_t inv_p12_m3;   // 1/(p1 p2) mod p3 (Montgomery form)
     uint64_t p12;       // p1 p2
     double p12_d;       // (double)(p1 p2)
 };

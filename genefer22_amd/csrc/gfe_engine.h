@@ -23,7 +23,7 @@ bool plan_valid(const Plan &P, uint32_t n);
 std::string plan_str(const Plan &P, uint32_t n);
 void select_primes(uint32_t b, uint32_t n, int *rns, int *is32);
 uint32_t prime_of(int is32, int k);
-uint2 mulw_const(uint32_t w, uint32_t p);
+uint32_t mulw_const(uint32_t w, uint32_t p);
 CrtConst make_crt(int rns, int is32, uint32_t n);
 int set_error(int code, const std::string &msg);
 extern thread_local std::string g_last_error;
@@ -38,7 +38,7 @@ struct Engine {
     int32_t *regs = nullptr;
     uint32_t *ws = nullptr, *zp = nullptr;
     int64_t *carry = nullptr;
-    uint2 *tw = nullptr;
+    uint32_t *tw = nullptr;
     CrtConst *crt = nullptr;
     uint32_t *bdev = nullptr;
     std::vector<hipEvent_t> events;
@@ -58,6 +58,7 @@ struct Engine {
     void init_multiplicand(uint32_t src);
     void mul_scalar(int32_t a);
     void set(uint32_t a);
+    void copy(uint32_t dst, uint32_t src);
     int check(const char *what);
 };
 

@@ -5,9 +5,9 @@
 //   regs  : num_regs x N int32 digits (register 0 = r0), balanced, |d| <= b/2 + 4
 //   ws    : rns x N uint32 residues (transform-domain work space of one squaring)
 //   zp    : rns x N uint32 residues (multiplicand, transform domain)
-//   carry : N / S int64 segment carries
-//   tw    : rns x N uint2 twiddles (w, Shoup quotient or Montgomery form)
-// One squaring = k_col_fwd, k_row, k_col_inv, k_norm, k_carry on the engine's stream.
+//   carry : N / (S 256) int64 workgroup carries of the normalisation
+//   tw    : rns x N uint32 twiddles (Montgomery form)
+// One squaring = k_col_fwd, k_row, k_col_inv, k_norm, k_carry_wg on the engine's stream.
 
 #include "gfe_engine.h"
 
@@ -56,14 +56,8 @@ static uint32_t pinv32(uint32_t p) {
     return q;
 }
 
-// Operand of ModS::mulw / ModU::mulw for the constant w.
-uint2 mulw_const(uint32_t w, uint32_t p) {
-    uint2 r;
-    r.x = w;
-    if (p < (1u << 31)) r.y = (uint32_t)((((uint64_t)w) << 32) / p);
-    else r.y = (uint32_t)((((uint64_t)w) << 32) % p);
-    return r;
-}
+// Operand of ModS::mulw / ModU::mulw for the constant w: its Montgomery form w 2^32 mod p.
+uint32_t mulw_const(uint32_t w, uint32_t p) { return (uint32_t)((((uint64_t)w) << 32) % p); }
 
 // create_ocl's rule, src/transform_ocl.cpp:35-50.
 void select_primes(uint32_t b, uint32_t n, int *rns, int *is32) {
@@ -75,14 +69,14 @@ void select_primes(uint32_t b, uint32_t n, int *rns, int *is32) {
 }
 
 // Twiddles Psi[k] = psi^bitrev_n(k), k in [1, N), psi = g^((p-1)/2N), in mulw form.
-static void make_twiddles(int is32, int k, uint32_t n, uint2 *out) {
+static void make_twiddles(int is32, int k, uint32_t n, uint32_t *out) {
     const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k];
     const size_t N = (size_t)1 << n;
     const uint32_t psi = hpow(g, (p - 1) >> (n + 1), p);
     std::vector<uint32_t> pw(N);
     pw[0] = 1;
     for (size_t e = 1; e < N; ++e) pw[e] = hmul(pw[e - 1], psi, p);
-    out[0] = make_uint2(0, 0);
+    out[0] = 0;
     for (size_t j = 1; j < N; ++j) {
         size_t r = 0;
         for (uint32_t bit = 0; bit < n; ++bit) r |= ((j >> bit) & 1) << (n - 1 - bit);
@@ -255,19 +249,20 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     set_lds_attributes();
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
     plan = default_plan(n);
-    const size_t nseg = N / plan.norm_s;
+    const size_t nwg = N / (plan.norm_s * 256);
     if (hipMalloc(&regs, (size_t)num_regs * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)rns * N * 4) != hipSuccess ||
-        hipMalloc(&zp, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&carry, nseg * 8) != hipSuccess ||
-        hipMalloc(&tw, (size_t)rns * N * 8) != hipSuccess || hipMalloc(&crt, sizeof(CrtConst)) != hipSuccess ||
+        hipMalloc(&zp, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&carry, nwg * 8) != hipSuccess ||
+
+        hipMalloc(&tw, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&crt, sizeof(CrtConst)) != hipSuccess ||
         hipMalloc(&bdev, 4) != hipSuccess) {
         dev_ok = true;
         return set_error(GFE_ENOMEM, "device allocation failed");
     }
     dev_ok = true;
-    std::vector<uint2> htw(N);
+    std::vector<uint32_t> htw(N);
     for (int k = 0; k < rns; ++k) {
         make_twiddles(is32, k, n, htw.data());
-        if (hipMemcpy(tw + (size_t)k * N, htw.data(), N * 8, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(tw + (size_t)k * N, htw.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess)
             return set_error(GFE_EHIP, "twiddle upload");
     }
     const CrtConst c = make_crt(rns, is32, n);
@@ -325,18 +320,16 @@ void Engine::launch_norm(int dup) {
     NormArgs A;
     A.ws = ws; A.dst = regs; A.carry = carry; A.crt = crt; A.b = bdev; A.dup = nullptr;
     A.ws_stride = 0; A.dst_stride = 0; A.carry_stride = 0; A.crt_stride = 0; A.n = (int)n; A.dup_all = dup;
-    const size_t nseg = N / plan.norm_s;
-    const int T = 256;
-    hipLaunchKernelGGL(K.norm, dim3((unsigned)((nseg + T - 1) / T), 1, 1), dim3(T), 0, stream, A);
+    const unsigned nwg = (unsigned)(N / (plan.norm_s * 256));
+    hipLaunchKernelGGL(K.norm, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
+    launch_carry();
 }
 
 void Engine::launch_carry() {
     CarryArgs A;
-    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0; A.n = (int)n;
-    A.seg_log = 3;
-    const size_t nseg = N / plan.norm_s;
-    const int T = 256;
-    hipLaunchKernelGGL(k_carry, dim3((unsigned)((nseg + T - 1) / T), 1, 1), dim3(T), 0, stream, A);
+    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0;
+    A.nwg = (unsigned)(N / (8 * 256));
+    hipLaunchKernelGGL(k_carry_wg, dim3((A.nwg + 255) / 256, 1, 1), dim3(256), 0, stream, A);
 }
 
 void Engine::square(int dup) {
@@ -344,7 +337,6 @@ void Engine::square(int dup) {
     launch_row(ROW_SQUARE, ws);
     launch_col_inv();
     launch_norm(dup);
-    launch_carry();
 }
 
 void Engine::mul() {
@@ -352,7 +344,6 @@ void Engine::mul() {
     launch_row(ROW_MUL, ws);
     launch_col_inv();
     launch_norm(0);
-    launch_carry();
 }
 
 void Engine::init_multiplicand(uint32_t src) {
@@ -362,12 +353,19 @@ void Engine::init_multiplicand(uint32_t src) {
 
 void Engine::mul_scalar(int32_t a) {
     MulScalarArgs A;
-    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0; A.n = (int)n; A.seg_log = 3;
-    A.a = a;
-    const size_t nseg = N / plan.norm_s;
-    const int T = 256;
-    hipLaunchKernelGGL(k_mulscalar, dim3((unsigned)((nseg + T - 1) / T), 1, 1), dim3(T), 0, stream, A);
+    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0;
+    A.n = (int)n; A.a = a;
+    const unsigned nwg = (unsigned)(N / (8 * 256));
+    hipLaunchKernelGGL(k_mulscalar, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
     launch_carry();
+}
+
+void Engine::copy(uint32_t dst, uint32_t src) {
+    const size_t n4 = N / 4;
+    const int T = 256;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)((n4 + T - 1) / T), 1, 1), dim3(T), 0, stream,
+                       reinterpret_cast<int4 *>(regs + (size_t)dst * N), reinterpret_cast<const int4 *>(regs + (size_t)src * N),
+                       n4, 0ll, 0ll);
 }
 
 void Engine::set(uint32_t a) {
@@ -475,10 +473,8 @@ int gfe_copy(gfe_engine *e, uint32_t dst, uint32_t src) {
     Engine &g = e->eng;
     if (dst >= g.num_regs || src >= g.num_regs) return set_error(GFE_EINVAL, "register index out of range");
     if (dst == src) return GFE_OK;
-    if (hipMemcpyAsync(g.regs + (size_t)dst * g.N, g.regs + (size_t)src * g.N, g.N * 4, hipMemcpyDeviceToDevice,
-                       g.stream) != hipSuccess)
-        return set_error(GFE_EHIP, "copy");
-    return GFE_OK;
+    g.copy(dst, src);
+    return g.check("copy");
 }
 
 int gfe_get_digits(gfe_engine *e, uint32_t reg, int32_t *out) {
@@ -511,7 +507,7 @@ int gfe_set_digits(gfe_engine *e, uint32_t reg, const int32_t *in) {
 size_t gfe_mem_size(const gfe_engine *e) {
     if (!e) return 0;
     const Engine &g = e->eng;
-    return (size_t)g.num_regs * g.N * 4 + 2 * (size_t)g.rns * g.N * 4 + (size_t)g.rns * g.N * 8 + g.N / 8 * 8;
+    return (size_t)g.num_regs * g.N * 4 + 3 * (size_t)g.rns * g.N * 4 + g.N / 2048 * 8;
 }
 
 size_t gfe_cache_size(const gfe_engine *e) {
@@ -590,6 +586,7 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
     *ms = t;
     if (kernel_ms) {
         double acc[5] = {0, 0, 0, 0, 0};
+        const int nk = 4;
         for (uint32_t i = 0; i < count; ++i) {
             const int dup = (dup_bits >> (i % 64)) & 1;
             (void)hipEventRecord(g.events[2], g.stream);
@@ -601,16 +598,14 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
             (void)hipEventRecord(g.events[5], g.stream);
             g.launch_norm(dup);
             (void)hipEventRecord(g.events[6], g.stream);
-            g.launch_carry();
-            (void)hipEventRecord(g.events[7], g.stream);
-            if (hipEventSynchronize(g.events[7]) != hipSuccess) return set_error(GFE_EHIP, "event sync");
-            for (int k = 0; k < 5; ++k) {
+            if (hipEventSynchronize(g.events[6]) != hipSuccess) return set_error(GFE_EHIP, "event sync");
+            for (int k = 0; k < nk; ++k) {
                 float d = 0;
                 (void)hipEventElapsedTime(&d, g.events[2 + k], g.events[3 + k]);
                 acc[k] += d;
             }
         }
-        for (int k = 0; k < 5; ++k) kernel_ms[k] = acc[k] / count;
+        for (int k = 0; k < 5; ++k) kernel_ms[k] = (k < nk) ? acc[k] / count : 0.0;
     }
     return g.check("time_squarings");
 }

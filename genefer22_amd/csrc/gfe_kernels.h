@@ -4,8 +4,8 @@
 // Transform (per prime p, length N = 2^n, psi a primitive 2N-th root of unity mod p):
 // negacyclic number-theoretic transform, Cooley-Tukey forward with roots
 // Psi[m + j] = psi^bitrev_n(m + j) at level l (m = 2^l, block j), pointwise product in the
-// bit-reversed domain, Gentleman-Sande inverse with root -Psi[2m - 1 - j]^-1... written as
-// (u, v) -> (u + v, (v - u) Psi[2m - 1 - j]) since Psi[m + j]^-1 = -Psi[2m - 1 - j].
+// bit-reversed domain, Gentleman-Sande inverse (u, v) -> (u + v, (u - v) Psi[m + j]^-1), computed
+// as (u + v, (v - u) Psi[2m - 1 - j]) because Psi[m + j]^-1 = -Psi[2m - 1 - j] (psi^N = -1).
 // The reference factors x^N + 1 the same way down to degree-1 residues (ocl/kernel.cl:235-273,
 // "z-transform"); here the last level is also split and the product is pointwise.
 //
@@ -15,8 +15,9 @@
 //   k_row     : levels a..n-1 on each row (C contiguous elements), pointwise square / product,
 //               inverse levels n-1..a;
 //   k_col_inv : inverse levels a-1..0;
-//   k_norm    : scale, Garner CRT, balanced carry over S consecutive digits per thread;
-//   k_carry   : adds each segment's carry-out to the next segment (negated across x^N = -1).
+//   k_norm    : scale, Garner CRT, balanced carry over S consecutive digits per thread; carries
+//               cross threads through LDS;
+//   k_carry_wg: adds each workgroup's carry-out into the next one (negated across x^N = -1).
 // Inside a pass, a thread holds E = 2^LOGE elements of one sub-transform in registers and runs
 // LOGE levels per chunk; chunks exchange through LDS.
 #pragma once
@@ -35,18 +36,16 @@ struct PrimeArgs {
 // Forward CT levels mu = 0..ne-1 of a 2^LOGE-point sub-transform held in x[].  The twiddle of
 // level mu, register block bi is tw[(t0 << mu) + bi].
 template <class M, int LOGE>
-__device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint2 *__restrict__ tw,
+__device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
                                            uint32_t t0, int ne) {
     constexpr int E = 1 << LOGE;
 #pragma unroll
     for (int mu = 0; mu < LOGE; ++mu) {
         if (mu < ne) {
-            constexpr int dummy = 0;
-            (void)dummy;
             const int h = E >> (mu + 1);
 #pragma unroll
             for (int bi = 0; bi < (1 << mu); ++bi) {
-                const uint2 w = tw[(t0 << mu) + bi];
+                const uint32_t w = tw[(t0 << mu) + bi];
 #pragma unroll
                 for (int i = 0; i < h; ++i) {
                     const int i0 = bi * 2 * h + i, i1 = i0 + h;
@@ -62,7 +61,7 @@ __device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE]
 // Inverse GS levels mu = ne-1..0 (the exact inverse of fwd_levels without the 1/2 factors).
 // lvl0 = global level of mu = 0.
 template <class M, int LOGE>
-__device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint2 *__restrict__ tw,
+__device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
                                            uint32_t t0, int lvl0, int ne) {
     constexpr int E = 1 << LOGE;
 #pragma unroll
@@ -72,7 +71,7 @@ __device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE]
             const uint32_t m3 = 3u << (lvl0 + mu);
 #pragma unroll
             for (int bi = 0; bi < (1 << mu); ++bi) {
-                const uint2 w = tw[m3 - 1u - ((t0 << mu) + bi)];
+                const uint32_t w = tw[m3 - 1u - ((t0 << mu) + bi)];
 #pragma unroll
                 for (int i = 0; i < h; ++i) {
                     const int i0 = bi * 2 * h + i, i1 = i0 + h;
@@ -97,7 +96,7 @@ struct Chunking {
 struct ColArgs {
     const int32_t *src;   // digits (candidate stride src_stride) -- forward only
     uint32_t *ws;         // residues, [cand][prime][N]
-    const uint2 *tw;      // [prime][N]
+    const uint32_t *tw;   // [prime][N], Montgomery form
     PrimeArgs pr;
     long long src_stride; // in int32
     long long ws_stride;  // in uint32, per candidate
@@ -118,7 +117,7 @@ __global__ void __launch_bounds__(1024) k_col_fwd(ColArgs A) {
     const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
     const size_t N = (size_t)1 << A.n;
     uint32_t *dst = A.ws + cand * A.ws_stride + (size_t)prime * N;
-    const uint2 *tw = A.tw + (size_t)prime * N;
+    const uint32_t *tw = A.tw + (size_t)prime * N;
     const Chunking ch(g, LOGE);
     uint32_t x[E];
     for (int ci = 0; ci < ch.nch; ++ci) {
@@ -173,7 +172,7 @@ __global__ void __launch_bounds__(1024) k_col_inv(ColArgs A) {
     const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
     const size_t N = (size_t)1 << A.n;
     uint32_t *dst = A.ws + cand * A.ws_stride + (size_t)prime * N;
-    const uint2 *tw = A.tw + (size_t)prime * N;
+    const uint32_t *tw = A.tw + (size_t)prime * N;
     const Chunking ch(g, LOGE);
     uint32_t x[E];
     for (int ci = ch.nch - 1; ci >= 0; --ci) {
@@ -216,7 +215,7 @@ __global__ void __launch_bounds__(1024) k_col_inv(ColArgs A) {
 struct RowArgs {
     uint32_t *ws;          // [cand][prime][N]
     const uint32_t *zp;    // multiplicand [cand][prime][N] (MODE 1)
-    const uint2 *tw;
+    const uint32_t *tw;
     PrimeArgs pr;
     long long ws_stride, zp_stride;
     int n, a, ng;
@@ -240,7 +239,7 @@ __global__ void __launch_bounds__(1024) k_row(RowArgs A) {
     const uint32_t row = blockIdx.x * A.ng + q;
     const size_t N = (size_t)1 << A.n;
     uint32_t *dst = A.ws + cand * A.ws_stride + (size_t)prime * N + (size_t)row * G;
-    const uint2 *tw = A.tw + (size_t)prime * N;
+    const uint32_t *tw = A.tw + (size_t)prime * N;
     uint32_t *L = lds + q * (G + spr);
     const Chunking ch(g, LOGE);
     uint32_t x[E];
@@ -317,7 +316,7 @@ __global__ void __launch_bounds__(1024) k_row(RowArgs A) {
 struct NormArgs {
     const uint32_t *ws;    // [cand][prime][N]
     int32_t *dst;          // digits [cand][N] (register 0 of each candidate)
-    int64_t *carry;        // [cand][N / S]
+    int64_t *carry;        // [cand][gridDim.x]: carry-out of each workgroup
     const CrtConst *crt;   // [cand] (or one shared entry when crt_stride == 0)
     const uint32_t *b;     // [cand]
     const uint8_t *dup;    // [cand] or null (then dup_all)
@@ -326,14 +325,77 @@ struct NormArgs {
     int n, dup_all;
 };
 
-// Scale, CRT and carry over S consecutive digits per thread; the carry-out goes to carry[seg].
+// Adds carry c into d[0..S-1] digit by digit while a carry remains; the last digit absorbs the rest
+// (digits stay bounded: |d| <= b/2 + 4 for carries of the sizes a squaring produces).
+template <int S>
+__device__ __forceinline__ void absorb_carry(int32_t (&d)[S], int64_t c, uint32_t b, double binv) {
+#pragma unroll
+    for (int j = 0; j < S - 1; ++j) {
+        if (c != 0) {
+            const int64_t f = (int64_t)d[j] + c;
+            d[j] = div_bal((uint64_t)f, (double)f, b, binv, &c);
+        }
+    }
+    d[S - 1] += (int32_t)c;
+}
+
+// Second half of a carry pass: thread t owns the S digits d (computed with carry-in 0; its carry-out
+// is cr) starting at digit i0 of register dc.  The carry-out of thread t is added into thread t + 1's
+// digits through LDS; the workgroup's carry-out is stored to wc[blockIdx.x] for k_carry_wg, which
+// adds it into the next workgroup's first digits after this kernel (cross-workgroup data goes
+// through a kernel boundary: gfx950's per-XCD L2s make an in-kernel device-scope fence an L2
+// write-back).  blockDim.x must be 256.
+template <int S>
+__device__ __forceinline__ void carry_tail(int32_t (&d)[S], int64_t cr, int32_t *dc, size_t i0, int64_t *wc,
+                                           uint32_t b, double binv) {
+    constexpr int T = 256;
+    __shared__ int64_t cs[T];
+    const int tid = threadIdx.x;
+    cs[tid] = cr;
+    __syncthreads();
+    if (tid > 0) absorb_carry<S>(d, cs[tid - 1], b, binv);
+    int32_t *o = dc + i0;
+#pragma unroll
+    for (int j = 0; j < S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);
+    if (tid == T - 1) wc[blockIdx.x] = cr;
+}
+
+struct CarryArgs {
+    int32_t *dst;
+    const int64_t *carry;
+    const uint32_t *b;
+    long long dst_stride, carry_stride;
+    unsigned nwg;   // workgroups of the carry pass (each 256 x 8 digits)
+};
+
+// Adds the carry-out of workgroup w - 1 (of the last one, negated, for w = 0: x^N = -1) into the
+// first digits of workgroup w's range.
+__global__ void __launch_bounds__(256) k_carry_wg(CarryArgs A) {
+    constexpr int S = 8;
+    const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= A.nwg) return;
+    const int cand = blockIdx.z;
+    const int64_t *wc = A.carry + cand * A.carry_stride;
+    const int64_t cin = (w == 0) ? -wc[A.nwg - 1] : wc[w - 1];
+    if (cin == 0) return;
+    const uint32_t b = A.b[cand];
+    int32_t *p = A.dst + cand * A.dst_stride + (size_t)w * 256 * S;
+    int32_t e[S];
+    const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);
+    e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w; e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
+    absorb_carry<S>(e, cin, b, 1.0 / (double)b);
+    *reinterpret_cast<int4 *>(p) = make_int4(e[0], e[1], e[2], e[3]);
+    *reinterpret_cast<int4 *>(p + 4) = make_int4(e[4], e[5], e[6], e[7]);
+}
+
+// Scale, Garner CRT and balanced carry.  Thread t owns S consecutive digits; carries cross threads
+// in carry_tail and workgroups in k_carry_wg.
 template <int RNS, class M, int S>
 __global__ void __launch_bounds__(256) k_norm(NormArgs A) {
-    const int cand = blockIdx.z;
+    constexpr int T = 256;
+    const int cand = blockIdx.z, tid = threadIdx.x;
     const size_t N = (size_t)1 << A.n;
-    const size_t seg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t i0 = seg * S;
-    if (i0 >= N) return;
+    const size_t i0 = ((size_t)blockIdx.x * T + tid) * S;
     const CrtConst &c = A.crt[cand * A.crt_stride];
     const uint32_t b = A.b[cand];
     const double binv = 1.0 / (double)b;
@@ -363,40 +425,7 @@ __global__ void __launch_bounds__(256) k_norm(NormArgs A) {
         xd += (double)cr;
         d[j] = div_bal(x64, xd, b, binv, &cr);
     }
-    int32_t *o = A.dst + cand * A.dst_stride + i0;
-#pragma unroll
-    for (int j = 0; j < S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);
-    A.carry[cand * A.carry_stride + seg] = cr;
-}
-
-struct CarryArgs {
-    int32_t *dst;
-    const int64_t *carry;
-    const uint32_t *b;
-    long long dst_stride, carry_stride;
-    int n, seg_log;   // segment = 2^seg_log digits
-};
-
-// Adds carry[seg - 1] (carry[last] negated for seg 0: x^N = -1) into segment seg, reducing
-// digit by digit while a carry remains; the segment's last digit absorbs what is left.
-__global__ void __launch_bounds__(256) k_carry(CarryArgs A) {
-    const int cand = blockIdx.z;
-    const size_t N = (size_t)1 << A.n;
-    const size_t nseg = N >> A.seg_log;
-    const size_t seg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (seg >= nseg) return;
-    const int64_t *cy = A.carry + cand * A.carry_stride;
-    int64_t c = (seg == 0) ? -cy[nseg - 1] : cy[seg - 1];
-    if (c == 0) return;
-    const uint32_t b = A.b[cand];
-    const double binv = 1.0 / (double)b;
-    int32_t *d = A.dst + cand * A.dst_stride + (seg << A.seg_log);
-    const int S = 1 << A.seg_log;
-    for (int j = 0; j < S - 1 && c != 0; ++j) {
-        const int64_t f = (int64_t)d[j] + c;
-        d[j] = div_bal((uint64_t)f, (double)f, b, binv, &c);
-    }
-    if (c != 0) d[S - 1] += (int32_t)c;
+    carry_tail<S>(d, cr, A.dst + cand * A.dst_stride, i0, A.carry + cand * A.carry_stride, b, binv);
 }
 
 struct MulScalarArgs {
@@ -404,28 +433,36 @@ struct MulScalarArgs {
     int64_t *carry;
     const uint32_t *b;
     long long dst_stride, carry_stride;
-    int n, seg_log;
+    int n;
     int32_t a;
 };
 
-// r = a * r on balanced digits (a small): per-segment carry, then k_carry (the reference's
-// mulscalar kernel, ocl/kernel.cl:2461-2492, applied after the normalisation).
+// r = a r on the balanced digits (the reference's mulscalar, ocl/kernel.cl:2461-2492, run after
+// the normalisation as baseModMul does, src/transformGPU.h:828-837), 8 digits per thread.
 __global__ void __launch_bounds__(256) k_mulscalar(MulScalarArgs A) {
-    const int cand = blockIdx.z;
-    const size_t N = (size_t)1 << A.n;
-    const size_t nseg = N >> A.seg_log;
-    const size_t seg = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (seg >= nseg) return;
+    constexpr int S = 8;
+    const int cand = blockIdx.z, tid = threadIdx.x;
+    const size_t i0 = ((size_t)blockIdx.x * 256 + tid) * S;
     const uint32_t b = A.b[cand];
     const double binv = 1.0 / (double)b;
-    int32_t *d = A.dst + cand * A.dst_stride + (seg << A.seg_log);
-    const int S = 1 << A.seg_log;
+    int32_t *dc = A.dst + cand * A.dst_stride;
+    int32_t d[S];
+    const int4 v0 = *reinterpret_cast<const int4 *>(dc + i0), v1 = *reinterpret_cast<const int4 *>(dc + i0 + 4);
+    d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w; d[4] = v1.x; d[5] = v1.y; d[6] = v1.z; d[7] = v1.w;
     int64_t c = 0;
+#pragma unroll
     for (int j = 0; j < S; ++j) {
         const int64_t f = (int64_t)d[j] * A.a + c;
         d[j] = div_bal((uint64_t)f, (double)f, b, binv, &c);
     }
-    A.carry[cand * A.carry_stride + seg] = c;
+    carry_tail<S>(d, c, dc, i0, A.carry + cand * A.carry_stride, b, binv);
+}
+
+// Register copy, 16 bytes per thread (the reference's copy kernel, ocl/kernel.cl:2501-2506).
+__global__ void __launch_bounds__(256) k_copy(int4 *__restrict__ dst, const int4 *__restrict__ src, size_t n4,
+                                               long long dst_stride, long long src_stride) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) dst[blockIdx.z * dst_stride + i] = src[blockIdx.z * src_stride + i];
 }
 
 // r = a (digit 0 = a, others 0).

@@ -78,8 +78,8 @@ const char *gfe_last_error(void);
 
 /* Times `count` squarings (dup pattern: dup_bits bit i for squaring i mod 64) with HIP events on
  * the engine's stream, after the queued work.  *ms = total; kernel_ms[0..4] = average duration of
- * one launch of each pass (column forward, row, column inverse, normalise, carry), measured in a
- * second, per-launch-timed run of the same count (kernel_ms may be null). */
+ * one launch of each pass (column forward, row, column inverse, normalise with carry; [4] = 0),
+ * measured in a second, per-launch-timed run of the same count (kernel_ms may be null). */
 int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double *ms, double *kernel_ms);
 
 /* Autotuner: times every admissible pass plan for this engine's n on the device and keeps the
