@@ -104,20 +104,24 @@ def bench_n22(G, steps: int, warmup: int):
 
 
 def bench_batched_n15(G, count: int, block: int):
-    """Candidates/hour at n = 15: `count` seeded candidates in one batch vs one per launch."""
-    if not hasattr(G, "Batch"):
-        return None
+    """Candidates/hour at n = 15: `count` seeded candidates in one batch vs one candidate per launch.
+
+    Both legs time squaring iterations with HIP events on resident state; candidates/hour =
+    3600 s / (seconds per squaring x mean esize) x candidates per iteration."""
     n = 15
     bases = seeded_bases(15, 100_000_000, 2_000_000_000, count)
     mean_es = sum(esize_of(b, n) for b in bases) / count
-    log(f"[bench] n=15 batched: {count} candidates, mean esize {mean_es:.0f}")
-    bt = G.Batch(n, bases)
-    bt.start_prp()
-    bt.step_squarings(block)  # warm-up
-    t_b = bt.time_squarings(4 * block)
-    per_iter_b = t_b / (4 * block)
+    log(f"[bench] n=15: {count} candidates, mean esize {mean_es:.0f}")
+    bt = G.Batch(n, count)
+    for i, b in enumerate(bases):
+        if bt.submit(b, 0, i) is None:
+            raise RuntimeError("batch refused a candidate")
+    bt.step()  # the first 64-iteration block of every candidate's program
+    plan_b = bt.tune(2)
+    log(f"[bench] batch plan: {plan_b}")
+    bt.time_rounds(block)
+    per_iter_b = bt.time_rounds(4 * block) / (4 * block)
     bt.close()
-    # plain: one candidate per launch (the transform engine), same seeds, sampled
     plain_ms = []
     for b in bases[:4]:
         t = G.Transform(b, n)
@@ -130,8 +134,8 @@ def bench_batched_n15(G, count: int, block: int):
     cand_h_batched = 3600.0 / (per_iter_b * 1e-3 * mean_es) * count
     cand_h_plain = 3600.0 / (per_sq_plain * 1e-3 * mean_es)
     return {
-        "candidates": count, "mean_esize": round(mean_es), "block": block,
-        "batched_ms_per_iteration": per_iter_b, "plain_ms_per_squaring": per_sq_plain,
+        "candidates": count, "mean_esize": round(mean_es), "block": block, "batch_plan": plan_b,
+        "batched_ms_per_iteration": round(per_iter_b, 4), "plain_ms_per_squaring": round(per_sq_plain, 5),
         "candidates_per_hour_batched": round(cand_h_batched, 1),
         "candidates_per_hour_plain": round(cand_h_plain, 1),
         "speedup": round(cand_h_batched / cand_h_plain, 2),

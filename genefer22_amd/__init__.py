@@ -63,7 +63,29 @@ ABI = [
     ("gfe_run_prp", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_int32, I32P, ctypes.POINTER(Result)]),
     ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),
     ("gfe_residue", ctypes.c_int, [I32P, ctypes.c_uint32, ctypes.c_uint32, I32P, ctypes.POINTER(Result)]),
+    ("gfe_batch_create", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("gfe_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_batch_slot_bytes", ctypes.c_size_t, [ctypes.c_uint32]),
+    ("gfe_batch_tune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gfe_batch_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
+    ("gfe_batch_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32)]),
+    ("gfe_batch_step", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    ("gfe_batch_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    ("gfe_batch_active", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_batch_get_digits", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, I32P]),
+    ("gfe_batch_time_rounds", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    ("gfe_batch_run", ctypes.c_int, [ctypes.c_uint32, U32P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]),
 ]
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("b", ctypes.c_uint32), ("n", ctypes.c_uint32),
+                ("res64", ctypes.c_uint64), ("old64", ctypes.c_uint64), ("hash64", ctypes.c_uint64),
+                ("esize", ctypes.c_uint64), ("iters", ctypes.c_uint64), ("is_prp", ctypes.c_int32),
+                ("gl_ok", ctypes.c_int32), ("t_submit", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 def lib():
@@ -224,3 +246,79 @@ def run_proof(b: int, n: int, depth: int, device: int = 0) -> dict:
     out = r.as_dict()
     out["mu_hash64"] = list(mu)
     return out
+
+
+class Batch:
+    """Many candidates of one n squared in shared launches (include/gfe_batch.h)."""
+
+    def __init__(self, n: int, capacity: int, device: int = 0):
+        self.n, self.capacity = n, capacity
+        h = ctypes.c_void_p()
+        _ck(lib().gfe_batch_create(n, capacity, device, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gfe_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def slot_bytes(n: int) -> int:
+        return lib().gfe_batch_slot_bytes(n)
+
+    def info(self) -> str:
+        buf = ctypes.create_string_buffer(512)
+        _ck(lib().gfe_batch_info(self._h, buf, len(buf)))
+        return buf.value.decode()
+
+    def tune(self, reps: int = 4) -> str:
+        _ck(lib().gfe_batch_tune(self._h, reps))
+        return self.info()
+
+    def submit(self, b: int, iters: int = 0, tag: int = 0) -> int | None:
+        """Slot of the new candidate, or None when the batch is full."""
+        slot = ctypes.c_int32(-1)
+        rc = lib().gfe_batch_submit(self._h, b, iters, tag, ctypes.byref(slot))
+        if rc == -6:
+            return None
+        _ck(rc)
+        return slot.value
+
+    def step(self) -> int:
+        a = ctypes.c_int32(0)
+        _ck(lib().gfe_batch_step(self._h, ctypes.byref(a)))
+        return a.value
+
+    def poll(self, max_results: int = 4096) -> list[dict]:
+        arr = (BatchResult * max_results)()
+        cnt = ctypes.c_uint32(0)
+        _ck(lib().gfe_batch_poll(self._h, arr, max_results, ctypes.byref(cnt)))
+        return [arr[i].as_dict() for i in range(cnt.value)]
+
+    def active(self) -> int:
+        return lib().gfe_batch_active(self._h)
+
+    def get_digits(self, slot: int, reg: int = 0) -> np.ndarray:
+        d = np.zeros(1 << self.n, dtype=np.int32)
+        _ck(lib().gfe_batch_get_digits(self._h, slot, reg, _i32(d)))
+        return d
+
+    def time_rounds(self, rounds: int) -> float:
+        """ms for `rounds` squarings of every busy slot (benchmark: squares r0 in place)."""
+        ms = ctypes.c_double()
+        _ck(lib().gfe_batch_time_rounds(self._h, rounds, ctypes.byref(ms)))
+        return ms.value
+
+
+def run_batch(n: int, bases, iters: int = 0, capacity: int = 0, device: int = 0) -> list[dict]:
+    """Quick tests (iters = 0) or truncated PRP runs of many b of one n in one batch."""
+    b = np.ascontiguousarray(bases, dtype=np.uint32)
+    res = (BatchResult * len(b))()
+    _ck(lib().gfe_batch_run(n, b.ctypes.data_as(U32P), len(b), iters, capacity or len(b), device, res))
+    return [res[i].as_dict() for i in range(len(b))]

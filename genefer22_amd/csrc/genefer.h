@@ -23,6 +23,26 @@ namespace gfe {
 
 inline int ilog2_32(uint32_t x) { return 31 - __builtin_clz(x); }
 
+// res = sum of the B-bit chunks of E (the exponent of the Gerbicz-Li check, src/genefer.h:580-588),
+// in one linear pass over E's words (the reference's repeated mpz_div_2exp is quadratic in |E|).
+inline void gl_chunk_sum(mpz_t res, mpz_srcptr E, unsigned long B) {
+    const size_t nbits = mpz_sizeinbase(E, 2);
+    std::vector<uint64_t> w((nbits + 63) / 64 + 1, 0);
+    size_t cnt = 0;
+    mpz_export(w.data(), &cnt, -1, 8, 0, 0, E);
+    mpz_set_ui(res, 0);
+    mpz_t t;
+    mpz_init(t);
+    for (size_t s0 = 0; s0 < nbits; s0 += B) {
+        const size_t wb = s0 / 64, we = std::min(w.size(), (s0 + B + 63) / 64 + 1);
+        mpz_import(t, we - wb, -1, 8, 0, 0, w.data() + wb);
+        mpz_tdiv_q_2exp(t, t, (mp_bitcnt_t)(s0 % 64));
+        mpz_tdiv_r_2exp(t, t, (mp_bitcnt_t)B);
+        mpz_add(res, res, t);
+    }
+    mpz_clear(t);
+}
+
 class genefer {
 public:
     genefer(uint32_t b, uint32_t n, int device, size_t num_regs)
@@ -101,18 +121,12 @@ public:
         T.copy(0, 1);
         for (int i = B_GL - 1; i >= 0; --i) T.squareDup(false);
         T.copy(1, 0);
-        mpz_t res, e, tt;
-        mpz_init_set_ui(res, 0);
-        mpz_init_set(e, _E);
-        mpz_init(tt);
-        while (mpz_sgn(e) != 0) {
-            mpz_mod_2exp(tt, e, (unsigned long)B_GL);
-            mpz_add(res, res, tt);
-            mpz_div_2exp(e, e, (unsigned long)B_GL);
-        }
+        mpz_t res;
+        mpz_init(res);
+        gl_chunk_sum(res, _E, (unsigned long)B_GL);
         T.set(1);
         for (int i = (int)mpz_sizeinbase(res, 2) - 1; i >= 0; --i) T.squareDup(mpz_tstbit(res, (mp_bitcnt_t)i) != 0);
-        mpz_clear(res); mpz_clear(e); mpz_clear(tt);
+        mpz_clear(res);
         T.mul(1);
         T.getInt(_g);
         const uint64_t h1 = _g.gethash64();

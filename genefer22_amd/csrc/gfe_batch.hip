@@ -1,0 +1,837 @@
+// gfe_batch.hip -- batched squaring engine: many candidates of one n per launch (include/gfe_batch.h).
+//
+// Device state of a batch of `cap` slots (per slot s, strides fixed by n):
+//   regs  : cap x 3 x N int32 digits (r0, r1 = Gerbicz-Li d(t), r2 = scratch)
+//   ws,zp : cap x 3 x N uint32 residues (transform work space, multiplicand)
+//   carry : cap x N/2048 int64 workgroup carries
+//   crt   : cap CrtConst, slots: cap SlotInfo (b, prime count, prime width)
+//   tw    : 2 x 3 x N uint32 twiddles (31-bit set, 32-bit set), shared by all slots
+// Every kernel runs over grid.z = slot and exits at once for a slot that is not active in the
+// current round (per-round activity and dup flags are uploaded per 64-round block).  Each slot
+// runs its own program: squaring runs (the PRP loop, the Gerbicz-Li check's squarings) and
+// register operations (copy, set, multiplicand, product, read-back), as src/genefer.h:479-615
+// sequences them for one candidate.
+
+#include <gmp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/gfe.h"
+#include "../../include/gfe_batch.h"
+#include "gfe_engine.h"
+#include "genefer.h"
+#include "gint.h"
+
+namespace gfe {
+
+struct SlotInfo {
+    uint32_t b;
+    int32_t rns, is32, pad;
+};
+
+struct BArgs {
+    int32_t *regs;
+    uint32_t *ws, *zp;
+    int64_t *carry;
+    const CrtConst *crt;
+    const SlotInfo *slots;
+    const uint32_t *tw;      // [2][3][N]
+    const uint8_t *active;   // [cap] or null (all)
+    const uint8_t *dup;      // [cap] or null
+    PrimeArgs prS, prU;
+    long long reg_stride;    // 3 N
+    int n, a, ng;
+    uint32_t src_reg, dst_reg;
+    int32_t value;
+};
+
+__device__ __forceinline__ int bslot(const BArgs &A) {
+    const int s = (int)blockIdx.z;
+    if (A.active && !A.active[s]) return -1;
+    return s;
+}
+
+template <int LOGE>
+__global__ void __launch_bounds__(1024) kb_col_fwd(BArgs A, int to_zp) {
+    extern __shared__ uint32_t lds[];
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const int prime = blockIdx.y;
+    if (prime >= si.rns) return;
+    const size_t N = (size_t)1 << A.n;
+    const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * N;
+    uint32_t *dst = (to_zp ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    if (si.is32) col_fwd_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    else col_fwd_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+template <int LOGE>
+__global__ void __launch_bounds__(1024) kb_col_inv(BArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const int prime = blockIdx.y;
+    if (prime >= si.rns) return;
+    const size_t N = (size_t)1 << A.n;
+    uint32_t *dst = A.ws + ((size_t)s * 3 + prime) * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    if (si.is32) col_inv_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    else col_inv_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+template <int LOGE, int MODE>
+__global__ void __launch_bounds__(1024) kb_row(BArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const int prime = blockIdx.y;
+    if (prime >= si.rns) return;
+    const size_t N = (size_t)1 << A.n;
+    uint32_t *data = ((MODE == ROW_FWD) ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
+    const uint32_t *zp = A.zp + ((size_t)s * 3 + prime) * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    if (si.is32) row_body<ModU, LOGE, MODE>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    else row_body<ModS, LOGE, MODE>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+__global__ void __launch_bounds__(256) kb_norm(BArgs A) {
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const size_t N = (size_t)1 << A.n;
+    const CrtConst &c = A.crt[s];
+    const bool dup = A.dup ? (A.dup[s] != 0) : false;
+    const uint32_t *ws = A.ws + (size_t)s * 3 * N;
+    int32_t *dst = A.regs + s * A.reg_stride;
+    int64_t *wc = A.carry + (size_t)s * (N >> 11);
+    if (si.rns == 3) {
+        if (si.is32) norm_body<3, ModU, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
+        else norm_body<3, ModS, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
+    } else {
+        if (si.is32) norm_body<2, ModU, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
+        else norm_body<2, ModS, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
+    }
+}
+
+__global__ void __launch_bounds__(256) kb_carry_wg(BArgs A) {
+    const int s = bslot(A);
+    if (s < 0) return;
+    const size_t N = (size_t)1 << A.n;
+    const unsigned nwg = (unsigned)(N >> 11);
+    const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nwg) carry_wg_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * nwg, nwg, A.slots[s].b, w);
+}
+
+// r_dst = r_src of each active slot.
+__global__ void __launch_bounds__(256) kb_copy(BArgs A) {
+    const int s = bslot(A);
+    if (s < 0) return;
+    const size_t N = (size_t)1 << A.n, n4 = N / 4;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    const int4 *src = reinterpret_cast<const int4 *>(A.regs + s * A.reg_stride + A.src_reg * N);
+    int4 *dst = reinterpret_cast<int4 *>(A.regs + s * A.reg_stride + A.dst_reg * N);
+    dst[i] = src[i];
+}
+
+// r_dst = value of each active slot.
+__global__ void __launch_bounds__(256) kb_set(BArgs A) {
+    const int s = bslot(A);
+    if (s < 0) return;
+    const size_t N = (size_t)1 << A.n;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) A.regs[s * A.reg_stride + A.dst_reg * N + i] = (i == 0) ? A.value : 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host side
+
+static double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+enum ItemKind { IT_RUN = 0, IT_COPY, IT_SET, IT_INITMUL, IT_MUL, IT_READ_RES, IT_READ_H1, IT_READ_H2 };
+enum RunKind { RUN_PRP = 0, RUN_ZERO = 1, RUN_RES = 2 };
+
+struct Item {
+    int kind;
+    uint32_t a = 0, b = 0;            // op arguments (dst, src) / (value)
+    int run = RUN_PRP;                // squaring run source
+    uint64_t start = 0, len = 0;      // PRP: index (from the top) of the first squaring; length
+};
+
+struct Slot {
+    bool busy = false, done = false;
+    uint32_t b = 0;
+    int rns = 0, is32 = 0;
+    uint64_t tag = 0, esize = 0, iters = 0;
+    double t_submit = 0, t_done = 0;
+    std::vector<uint64_t> ebits;      // E = b^(2^n), little-endian 64-bit words
+    std::vector<uint8_t> resbits;     // Gerbicz-Li exponent `res`, most significant bit first
+    std::deque<Item> q;
+    uint64_t pos = 0;                 // progress inside the head run
+    uint64_t res64 = 0, old64 = 0, hash64 = 0, h1 = 0;
+    int is_prp = 0, gl_ok = -1;
+
+    bool bit(uint64_t k) const {  // k-th squaring of the PRP loop (bit esize - 1 - k)
+        const uint64_t i = esize - 1 - k;
+        return (ebits[i >> 6] >> (i & 63)) & 1;
+    }
+};
+
+struct Batch {
+    uint32_t n = 0, cap = 0;
+    int device = 0;
+    size_t N = 0;
+    Plan plan;
+    hipStream_t stream = nullptr;
+    int32_t *regs = nullptr;
+    uint32_t *ws = nullptr, *zp = nullptr, *tw = nullptr;
+    int64_t *carry = nullptr;
+    CrtConst *crt = nullptr;
+    SlotInfo *slots_dev = nullptr;
+    uint8_t *act_dev = nullptr, *dup_dev = nullptr;   // [64][cap] each
+    uint8_t *stage[2] = {nullptr, nullptr};           // pinned, 2 x (2 x 64 x cap)
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int stage_i = 0;
+    int32_t *readback = nullptr;                      // pinned N int32
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    PrimeArgs prS, prU;
+    std::vector<Slot> slots;
+    std::vector<gfe_batch_result> finished;
+    double t0 = 0;
+    bool ok = false;
+
+    ~Batch();
+    int init(uint32_t n, uint32_t cap, int device);
+    BArgs args(const uint8_t *active, const uint8_t *dup) const;
+    size_t lds_col() const { return (size_t)plan.col_ng * ((1u << plan.a) + ((1u << plan.a) >> plan.col_loge)) * 4; }
+    size_t lds_row() const {
+        const uint32_t G = 1u << (n - plan.a);
+        return (size_t)plan.row_ng * (G + (G >> plan.row_loge)) * 4;
+    }
+    void launch_col_fwd(const BArgs &A, int to_zp);
+    void launch_row(const BArgs &A, int mode);
+    void launch_col_inv(const BArgs &A);
+    void launch_norm_carry(const BArgs &A);
+    void square(const uint8_t *active, const uint8_t *dup);
+    void op_copy(const uint8_t *active, uint32_t dst, uint32_t src);
+    void op_set(const uint8_t *active, uint32_t dst, int32_t value);
+    void op_initmul(const uint8_t *active, uint32_t src);
+    void op_mul(const uint8_t *active);
+    uint8_t *stage_acquire();
+    void stage_release();
+    int submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot);
+    int step(int32_t *active);
+    int read_residue(int s, uint32_t reg, gint &g);
+    int check(const char *what);
+};
+
+int check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) return set_error(GFE_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return GFE_OK;
+}
+
+int Batch::check(const char *what) { return check_hip(hipGetLastError(), what); }
+
+Batch::~Batch() {
+    if (!ok) return;
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(regs); (void)hipFree(ws); (void)hipFree(zp); (void)hipFree(tw); (void)hipFree(carry);
+    (void)hipFree(crt); (void)hipFree(slots_dev); (void)hipFree(act_dev); (void)hipFree(dup_dev);
+    for (int i = 0; i < 2; ++i) {
+        if (stage[i]) (void)hipHostFree(stage[i]);
+        if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
+    }
+    if (readback) (void)hipHostFree(readback);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void make_twiddles_set(int is32, int k, uint32_t n, uint32_t *out);  // gfe_engine.hip
+void set_lds_attr(const void *fn);
+
+int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
+    n = n_; cap = cap_; device = device_;
+    N = (size_t)1 << n;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(GFE_ENODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return set_error(GFE_ENODEV, "device index out of range");
+    if (hipSetDevice(device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    ok = true;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
+    plan = default_plan(n);
+    const size_t nwg = N >> 11;
+    if (hipMalloc(&regs, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)cap * 3 * N * 4) != hipSuccess ||
+        hipMalloc(&zp, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&tw, (size_t)6 * N * 4) != hipSuccess ||
+        hipMalloc(&carry, (size_t)cap * nwg * 8) != hipSuccess || hipMalloc(&crt, (size_t)cap * sizeof(CrtConst)) != hipSuccess ||
+        hipMalloc(&slots_dev, (size_t)cap * sizeof(SlotInfo)) != hipSuccess ||
+        hipMalloc(&act_dev, (size_t)64 * cap) != hipSuccess || hipMalloc(&dup_dev, (size_t)64 * cap) != hipSuccess)
+        return set_error(GFE_ENOMEM, "device allocation failed");
+    for (int i = 0; i < 2; ++i) {
+        if (hipHostMalloc((void **)&stage[i], (size_t)2 * 64 * cap) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
+        if (hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming) != hipSuccess) return set_error(GFE_EHIP, "event");
+    }
+    if (hipHostMalloc((void **)&readback, N * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
+    if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return set_error(GFE_EHIP, "event");
+    std::vector<uint32_t> h(N);
+    for (int set = 0; set < 2; ++set)
+        for (int k = 0; k < 3; ++k) {
+            make_twiddles_set(set, k, n, h.data());
+            if (hipMemcpy(tw + ((size_t)set * 3 + k) * N, h.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess)
+                return set_error(GFE_EHIP, "twiddle upload");
+        }
+    for (int k = 0; k < 3; ++k) {
+        prS.p[k] = prime_of(0, k); prU.p[k] = prime_of(1, k);
+        CrtConst c0 = make_crt(3, 0, n), c1 = make_crt(3, 1, n);
+        prS.q[k] = c0.q[k]; prU.q[k] = c1.q[k];
+    }
+    std::vector<SlotInfo> si(cap);
+    for (auto &x : si) { x.b = 1024; x.rns = 0; x.is32 = 0; x.pad = 0; }
+    if (hipMemcpy(slots_dev, si.data(), cap * sizeof(SlotInfo), hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(GFE_EHIP, "slot upload");
+    if (hipMemsetAsync(regs, 0, (size_t)cap * 3 * N * 4, stream) != hipSuccess ||
+        hipMemsetAsync(act_dev, 0, (size_t)64 * cap, stream) != hipSuccess ||
+        hipMemsetAsync(dup_dev, 0, (size_t)64 * cap, stream) != hipSuccess)
+        return set_error(GFE_EHIP, "memset");
+    set_lds_attr((const void *)kb_col_fwd<3>); set_lds_attr((const void *)kb_col_fwd<4>); set_lds_attr((const void *)kb_col_fwd<5>);
+    set_lds_attr((const void *)kb_col_inv<3>); set_lds_attr((const void *)kb_col_inv<4>); set_lds_attr((const void *)kb_col_inv<5>);
+    set_lds_attr((const void *)kb_row<3, ROW_SQUARE>); set_lds_attr((const void *)kb_row<4, ROW_SQUARE>); set_lds_attr((const void *)kb_row<5, ROW_SQUARE>);
+    set_lds_attr((const void *)kb_row<3, ROW_MUL>); set_lds_attr((const void *)kb_row<4, ROW_MUL>); set_lds_attr((const void *)kb_row<5, ROW_MUL>);
+    set_lds_attr((const void *)kb_row<3, ROW_FWD>); set_lds_attr((const void *)kb_row<4, ROW_FWD>); set_lds_attr((const void *)kb_row<5, ROW_FWD>);
+    slots.resize(cap);
+    t0 = now_s();
+    return check_hip(hipStreamSynchronize(stream), "init");
+}
+
+BArgs Batch::args(const uint8_t *active, const uint8_t *dup) const {
+    BArgs A;
+    memset(&A, 0, sizeof(A));
+    A.regs = regs; A.ws = ws; A.zp = zp; A.carry = carry; A.crt = crt; A.slots = slots_dev; A.tw = tw;
+    A.active = active; A.dup = dup; A.prS = prS; A.prU = prU; A.reg_stride = (long long)(3 * N); A.n = (int)n;
+    return A;
+}
+
+void Batch::launch_col_fwd(const BArgs &A0, int to_zp) {
+    BArgs A = A0;
+    A.a = plan.a; A.ng = plan.col_ng;
+    const uint32_t C = 1u << (n - plan.a);
+    const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+    const dim3 grid(C / plan.col_ng, 3, cap);
+    switch (plan.col_loge) {
+        case 3: hipLaunchKernelGGL(kb_col_fwd<3>, grid, dim3(T), lds_col(), stream, A, to_zp); break;
+        case 4: hipLaunchKernelGGL(kb_col_fwd<4>, grid, dim3(T), lds_col(), stream, A, to_zp); break;
+        default: hipLaunchKernelGGL(kb_col_fwd<5>, grid, dim3(T), lds_col(), stream, A, to_zp); break;
+    }
+}
+
+void Batch::launch_col_inv(const BArgs &A0) {
+    BArgs A = A0;
+    A.a = plan.a; A.ng = plan.col_ng;
+    const uint32_t C = 1u << (n - plan.a);
+    const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+    const dim3 grid(C / plan.col_ng, 3, cap);
+    switch (plan.col_loge) {
+        case 3: hipLaunchKernelGGL(kb_col_inv<3>, grid, dim3(T), lds_col(), stream, A); break;
+        case 4: hipLaunchKernelGGL(kb_col_inv<4>, grid, dim3(T), lds_col(), stream, A); break;
+        default: hipLaunchKernelGGL(kb_col_inv<5>, grid, dim3(T), lds_col(), stream, A); break;
+    }
+}
+
+template <int MODE>
+static void launch_row_mode(const Batch &B, const BArgs &A, dim3 grid, int T) {
+    switch (B.plan.row_loge) {
+        case 3: hipLaunchKernelGGL((kb_row<3, MODE>), grid, dim3(T), B.lds_row(), B.stream, A); break;
+        case 4: hipLaunchKernelGGL((kb_row<4, MODE>), grid, dim3(T), B.lds_row(), B.stream, A); break;
+        default: hipLaunchKernelGGL((kb_row<5, MODE>), grid, dim3(T), B.lds_row(), B.stream, A); break;
+    }
+}
+
+void Batch::launch_row(const BArgs &A0, int mode) {
+    BArgs A = A0;
+    A.a = plan.a; A.ng = plan.row_ng;
+    const uint32_t R = 1u << plan.a, G = 1u << (n - plan.a);
+    const int T = plan.row_ng * G / (1 << plan.row_loge);
+    const dim3 grid(R / plan.row_ng, 3, cap);
+    if (mode == ROW_SQUARE) launch_row_mode<ROW_SQUARE>(*this, A, grid, T);
+    else if (mode == ROW_MUL) launch_row_mode<ROW_MUL>(*this, A, grid, T);
+    else launch_row_mode<ROW_FWD>(*this, A, grid, T);
+}
+
+void Batch::launch_norm_carry(const BArgs &A) {
+    const unsigned nwg = (unsigned)(N >> 11);
+    hipLaunchKernelGGL(kb_norm, dim3(nwg, 1, cap), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(kb_carry_wg, dim3((nwg + 255) / 256, 1, cap), dim3(256), 0, stream, A);
+}
+
+void Batch::square(const uint8_t *active, const uint8_t *dup) {
+    BArgs A = args(active, dup);
+    A.src_reg = 0;
+    launch_col_fwd(A, 0);
+    launch_row(A, ROW_SQUARE);
+    launch_col_inv(A);
+    launch_norm_carry(A);
+}
+
+void Batch::op_copy(const uint8_t *active, uint32_t dst, uint32_t src) {
+    BArgs A = args(active, nullptr);
+    A.src_reg = src; A.dst_reg = dst;
+    hipLaunchKernelGGL(kb_copy, dim3((unsigned)((N / 4 + 255) / 256), 1, cap), dim3(256), 0, stream, A);
+}
+
+void Batch::op_set(const uint8_t *active, uint32_t dst, int32_t value) {
+    BArgs A = args(active, nullptr);
+    A.dst_reg = dst; A.value = value;
+    hipLaunchKernelGGL(kb_set, dim3((unsigned)((N + 255) / 256), 1, cap), dim3(256), 0, stream, A);
+}
+
+void Batch::op_initmul(const uint8_t *active, uint32_t src) {
+    BArgs A = args(active, nullptr);
+    A.src_reg = src;
+    launch_col_fwd(A, 1);
+    launch_row(A, ROW_FWD);
+}
+
+void Batch::op_mul(const uint8_t *active) {
+    BArgs A = args(active, nullptr);
+    A.src_reg = 0;
+    launch_col_fwd(A, 0);
+    launch_row(A, ROW_MUL);
+    launch_col_inv(A);
+    launch_norm_carry(A);
+}
+
+// Pinned staging for per-block flags: two buffers, each reused only after its last upload ended.
+uint8_t *Batch::stage_acquire() {
+    (void)hipEventSynchronize(stage_ev[stage_i]);
+    return stage[stage_i];
+}
+void Batch::stage_release() {
+    (void)hipEventRecord(stage_ev[stage_i], stream);
+    stage_i ^= 1;
+}
+
+
+int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
+    int s = -1;
+    for (uint32_t i = 0; i < cap; ++i)
+        if (!slots[i].busy) { s = (int)i; break; }
+    if (s < 0) return set_error(GFE_EFULL, "batch full");
+    Slot &S = slots[s];
+    S = Slot();
+    S.busy = true;
+    S.b = b;
+    S.tag = tag;
+    S.t_submit = now_s() - t0;
+    select_primes(b, n, &S.rns, &S.is32);
+    // E = b^(2^n) and its bits (src/genefer.h:1476)
+    mpz_t E;
+    mpz_init(E);
+    mpz_ui_pow_ui(E, b, 1ul << n);
+    S.esize = mpz_sizeinbase(E, 2);
+    S.ebits.assign((S.esize + 63) / 64, 0);
+    size_t cnt = 0;
+    mpz_export(S.ebits.data(), &cnt, -1, 8, 0, 0, E);
+    const bool full = (iters == 0 || iters >= S.esize);
+    S.iters = full ? S.esize : iters;
+    if (!full) {
+        Item r; r.kind = IT_RUN; r.run = RUN_PRP; r.start = 0; r.len = S.iters;
+        S.q.push_back(r);
+        Item rd; rd.kind = IT_READ_RES;
+        S.q.push_back(rd);
+    } else {
+        // B_GerbiczLi (src/genefer.h:467-471) rounded up to a multiple of 64
+        const uint64_t L = uint64_t(2) << (ilog2_32((uint32_t)S.esize) / 2);
+        uint64_t B = (S.esize - 1) / L + 1;
+        B = (B + 63) / 64 * 64;
+        // PRP loop: squaring k handles bit i = esize - 1 - k; after bit i with i % B == 0 and
+        // i / B != 0, the d(t) update r2 = r0; r0 *= r1; r1 = r0; r0 = r2 (src/genefer.h:529-535)
+        uint64_t k = 0;
+        while (k < S.esize) {
+            const uint64_t i = S.esize - 1 - k;  // first bit of this run
+            const uint64_t last = (i / B) * B;    // last bit of the run: next multiple of B below
+            const uint64_t len = i - last + 1;
+            Item r; r.kind = IT_RUN; r.run = RUN_PRP; r.start = k; r.len = len;
+            S.q.push_back(r);
+            k += len;
+            if (last != 0) {
+                Item c; c.kind = IT_COPY; c.a = 2; c.b = 0; S.q.push_back(c);
+                Item m; m.kind = IT_INITMUL; m.a = 1; S.q.push_back(m);
+                Item u; u.kind = IT_MUL; S.q.push_back(u);
+                Item c1; c1.kind = IT_COPY; c1.a = 1; c1.b = 0; S.q.push_back(c1);
+                Item c2; c2.kind = IT_COPY; c2.a = 0; c2.b = 2; S.q.push_back(c2);
+            }
+        }
+        Item rd; rd.kind = IT_READ_RES; S.q.push_back(rd);
+        // Gerbicz-Li check (src/genefer.h:557-615)
+        { Item m; m.kind = IT_INITMUL; m.a = 1; S.q.push_back(m); }
+        { Item u; u.kind = IT_MUL; S.q.push_back(u); }
+        { Item c; c.kind = IT_COPY; c.a = 2; c.b = 0; S.q.push_back(c); }
+        { Item c; c.kind = IT_COPY; c.a = 0; c.b = 1; S.q.push_back(c); }
+        { Item r; r.kind = IT_RUN; r.run = RUN_ZERO; r.len = B; S.q.push_back(r); }
+        { Item c; c.kind = IT_COPY; c.a = 1; c.b = 0; S.q.push_back(c); }
+        mpz_t res;
+        mpz_init(res);
+        gl_chunk_sum(res, E, (unsigned long)B);
+        const size_t rs = mpz_sizeinbase(res, 2);
+        for (long i = (long)rs - 1; i >= 0; --i) S.resbits.push_back((uint8_t)mpz_tstbit(res, (mp_bitcnt_t)i));
+        mpz_clear(res);
+        { Item st; st.kind = IT_SET; st.a = 0; st.b = 1; S.q.push_back(st); }
+        { Item r; r.kind = IT_RUN; r.run = RUN_RES; r.len = S.resbits.size(); S.q.push_back(r); }
+        { Item m; m.kind = IT_INITMUL; m.a = 1; S.q.push_back(m); }
+        { Item u; u.kind = IT_MUL; S.q.push_back(u); }
+        { Item h; h.kind = IT_READ_H1; S.q.push_back(h); }
+        { Item c; c.kind = IT_COPY; c.a = 0; c.b = 2; S.q.push_back(c); }
+        { Item h; h.kind = IT_READ_H2; S.q.push_back(h); }
+    }
+    mpz_clear(E);
+    // slot table and CRT constants; r0 = 1, r1 = 1 (prp: set(1); copy(1, 0))
+    SlotInfo si;
+    si.b = b; si.rns = S.rns; si.is32 = S.is32; si.pad = 0;
+    const CrtConst c = make_crt(S.rns, S.is32, n);
+    int rc = check_hip(hipMemcpyAsync(slots_dev + s, &si, sizeof(si), hipMemcpyHostToDevice, stream), "slot");
+    if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(crt + s, &c, sizeof(c), hipMemcpyHostToDevice, stream), "crt");
+    if (rc == GFE_OK) rc = check_hip(hipStreamSynchronize(stream), "submit");
+    if (rc != GFE_OK) return rc;
+    std::vector<uint8_t> one(cap, 0);
+    one[s] = 1;
+    uint8_t *st = stage_acquire();
+    memcpy(st, one.data(), cap);
+    rc = check_hip(hipMemcpyAsync(act_dev, st, cap, hipMemcpyHostToDevice, stream), "flags");
+    stage_release();
+    if (rc != GFE_OK) return rc;
+    op_set(act_dev, 0, 1);
+    op_set(act_dev, 1, 1);
+    if (slot_out) *slot_out = s;
+    return check("submit");
+}
+
+int Batch::read_residue(int s, uint32_t reg, gint &g) {
+    int rc = check_hip(hipMemcpyAsync(readback, regs + (size_t)s * 3 * N + reg * N, N * 4, hipMemcpyDeviceToHost, stream),
+                       "readback");
+    if (rc == GFE_OK) rc = check_hip(hipStreamSynchronize(stream), "readback sync");
+    if (rc != GFE_OK) return rc;
+    memcpy(g.data(), readback, N * 4);
+    g.reset();
+    return GFE_OK;
+}
+
+int Batch::step(int32_t *active_out) {
+    std::vector<uint8_t> flags(cap);
+    // 1. register operations, grouped by (kind, arguments) across slots
+    for (;;) {
+        std::map<std::tuple<int, uint32_t, uint32_t>, std::vector<int>> groups;
+        for (uint32_t s = 0; s < cap; ++s) {
+            Slot &S = slots[s];
+            if (!S.busy || S.done || S.q.empty()) continue;
+            const Item &it = S.q.front();
+            if (it.kind == IT_RUN) continue;
+            groups[std::make_tuple(it.kind, it.a, it.b)].push_back((int)s);
+        }
+        if (groups.empty()) break;
+        for (auto &kv : groups) {
+            const int kind = std::get<0>(kv.first);
+            const uint32_t a = std::get<1>(kv.first), b2 = std::get<2>(kv.first);
+            if (kind == IT_READ_RES || kind == IT_READ_H1 || kind == IT_READ_H2) {
+                for (int s : kv.second) {
+                    Slot &S = slots[s];
+                    gint gs(N, S.b);
+                    const int rc = read_residue(s, 0, gs);
+                    if (rc != GFE_OK) return rc;
+                    if (kind == IT_READ_RES) {
+                        uint64_t r = 0, o = 0;
+                        S.is_prp = gs.isOne(r, o) ? 1 : 0;
+                        S.res64 = r; S.old64 = o;
+                        S.hash64 = S.is_prp ? 0 : gs.gethash64();
+                    } else if (kind == IT_READ_H1) {
+                        S.h1 = gs.gethash64();
+                    } else {
+                        S.gl_ok = (gs.gethash64() == S.h1) ? 1 : 0;
+                    }
+                    S.q.pop_front();
+                }
+                continue;
+            }
+            std::fill(flags.begin(), flags.end(), 0);
+            for (int s : kv.second) flags[s] = 1;
+            uint8_t *st = stage_acquire();
+            memcpy(st, flags.data(), cap);
+            int rc = check_hip(hipMemcpyAsync(act_dev, st, cap, hipMemcpyHostToDevice, stream), "flags");
+            stage_release();
+            if (rc != GFE_OK) return rc;
+            switch (kind) {
+                case IT_COPY: op_copy(act_dev, a, b2); break;
+                case IT_SET: op_set(act_dev, a, (int32_t)b2); break;
+                case IT_INITMUL: op_initmul(act_dev, a); break;
+                case IT_MUL: op_mul(act_dev); break;
+            }
+            rc = check("op");
+            if (rc != GFE_OK) return rc;
+            for (int s : kv.second) slots[s].q.pop_front();
+        }
+    }
+    // 2. finished programs
+    for (uint32_t s = 0; s < cap; ++s) {
+        Slot &S = slots[s];
+        if (S.busy && !S.done && S.q.empty()) {
+            S.done = true;
+            S.t_done = now_s() - t0;
+            gfe_batch_result r;
+            memset(&r, 0, sizeof(r));
+            r.tag = S.tag; r.b = S.b; r.n = n; r.res64 = S.res64; r.old64 = S.old64; r.hash64 = S.hash64;
+            r.esize = S.esize; r.iters = S.iters; r.is_prp = S.is_prp; r.gl_ok = S.gl_ok;
+            r.t_submit = S.t_submit; r.t_done = S.t_done;
+            finished.push_back(r);
+            S.busy = false;
+            S.ebits.clear(); S.ebits.shrink_to_fit();
+        }
+    }
+    // 3. one block of up to 64 squaring rounds
+    int rounds = 0;
+    std::vector<uint64_t> len(cap, 0);
+    for (uint32_t s = 0; s < cap; ++s) {
+        Slot &S = slots[s];
+        if (!S.busy || S.done || S.q.empty() || S.q.front().kind != IT_RUN) continue;
+        len[s] = std::min<uint64_t>(64, S.q.front().len - S.pos);
+        rounds = std::max(rounds, (int)len[s]);
+    }
+    if (rounds > 0) {
+        uint8_t *st = stage_acquire();
+        uint8_t *act = st, *dup = st + (size_t)64 * cap;
+        memset(st, 0, (size_t)2 * 64 * cap);
+        for (uint32_t s = 0; s < cap; ++s) {
+            if (!len[s]) continue;
+            const Slot &S = slots[s];
+            const Item &it = S.q.front();
+            for (uint64_t r = 0; r < len[s]; ++r) {
+                act[r * cap + s] = 1;
+                const uint64_t k = S.pos + r;
+                uint8_t d = 0;
+                if (it.run == RUN_PRP) d = S.bit(it.start + k) ? 1 : 0;
+                else if (it.run == RUN_RES) d = S.resbits[k];
+                dup[r * cap + s] = d;
+            }
+        }
+        int rc = check_hip(hipMemcpyAsync(act_dev, act, (size_t)rounds * cap, hipMemcpyHostToDevice, stream), "act");
+        if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(dup_dev, dup, (size_t)rounds * cap, hipMemcpyHostToDevice, stream), "dup");
+        stage_release();
+        if (rc != GFE_OK) return rc;
+        for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
+        rc = check("square block");
+        if (rc != GFE_OK) return rc;
+        for (uint32_t s = 0; s < cap; ++s) {
+            if (!len[s]) continue;
+            Slot &S = slots[s];
+            S.pos += len[s];
+            if (S.pos == S.q.front().len) {
+                S.q.pop_front();
+                S.pos = 0;
+            }
+        }
+    }
+    if (active_out) {
+        int a = 0;
+        for (auto &S : slots) a += S.busy ? 1 : 0;
+        *active_out = a;
+    }
+    return GFE_OK;
+}
+
+}  // namespace gfe
+
+// ================================================================================================
+// C ABI
+
+using gfe::Batch;
+using gfe::set_error;
+
+struct gfe_batch {
+    Batch bt;
+};
+
+extern "C" {
+
+size_t gfe_batch_slot_bytes(uint32_t n) {
+    const size_t N = (size_t)1 << n;
+    return 9 * N * 4 + (N >> 11) * 8 + sizeof(gfe::CrtConst) + sizeof(gfe::SlotInfo);
+}
+
+int gfe_batch_create(uint32_t n, uint32_t capacity, int device, gfe_batch **out) {
+    if (!out) return set_error(GFE_EINVAL, "null output");
+    *out = nullptr;
+    if (n < 12 || n > 23) return set_error(GFE_EINVAL, "n must be in [12, 23]");
+    if (capacity < 1 || capacity > 65535) return set_error(GFE_EINVAL, "capacity must be in [1, 65535]");
+    gfe_batch *b = new gfe_batch;
+    const int rc = b->bt.init(n, capacity, device);
+    if (rc != GFE_OK) {
+        const std::string msg = gfe::g_last_error;
+        delete b;
+        return set_error(rc, msg);
+    }
+    *out = b;
+    return GFE_OK;
+}
+
+int gfe_batch_destroy(gfe_batch *b) {
+    delete b;
+    return GFE_OK;
+}
+
+int gfe_batch_info(const gfe_batch *b, char *buf, size_t len) {
+    if (!b || !buf || len == 0) return set_error(GFE_EINVAL, "info");
+    std::ostringstream ss;
+    ss << "batch n=" << b->bt.n << " capacity=" << b->bt.cap << "; " << gfe::plan_str(b->bt.plan, b->bt.n);
+    snprintf(buf, len, "%s", ss.str().c_str());
+    return GFE_OK;
+}
+
+int gfe_batch_submit(gfe_batch *b, uint32_t base, uint64_t iters, uint64_t tag, int32_t *slot) {
+    if (!b) return set_error(GFE_EINVAL, "null batch");
+    if (base < 1024 || base > 2000000000u) return set_error(GFE_EINVAL, "b must be in [1024, 2000000000]");
+    if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    return b->bt.submit(base, iters, tag, slot);
+}
+
+int gfe_batch_step(gfe_batch *b, int32_t *active) {
+    if (!b) return set_error(GFE_EINVAL, "null batch");
+    if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    return b->bt.step(active);
+}
+
+int gfe_batch_poll(gfe_batch *b, gfe_batch_result *out, uint32_t max, uint32_t *count) {
+    if (!b || !count) return set_error(GFE_EINVAL, "poll");
+    uint32_t k = 0;
+    auto &f = b->bt.finished;
+    while (k < max && !f.empty() && out) {
+        out[k++] = f.front();
+        f.erase(f.begin());
+    }
+    *count = k;
+    return GFE_OK;
+}
+
+int gfe_batch_active(const gfe_batch *b) {
+    if (!b) return 0;
+    int a = 0;
+    for (auto &S : b->bt.slots) a += S.busy ? 1 : 0;
+    return a;
+}
+
+int gfe_batch_get_digits(gfe_batch *b, int32_t slot, uint32_t reg, int32_t *out) {
+    if (!b || !out || slot < 0 || (uint32_t)slot >= b->bt.cap || reg >= 3) return set_error(GFE_EINVAL, "get_digits");
+    if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    gfe::gint g(b->bt.N, b->bt.slots[slot].busy ? b->bt.slots[slot].b : 2);
+    const int rc = b->bt.read_residue(slot, reg, g);
+    if (rc != GFE_OK) return rc;
+    memcpy(out, g.data(), b->bt.N * 4);
+    return GFE_OK;
+}
+
+int gfe_batch_time_rounds(gfe_batch *b, uint32_t rounds, double *ms) {
+    if (!b || !ms || rounds == 0) return set_error(GFE_EINVAL, "time_rounds");
+    Batch &B = b->bt;
+    if (hipSetDevice(B.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    std::vector<uint8_t> f(B.cap, 0);
+    for (uint32_t s = 0; s < B.cap; ++s) f[s] = B.slots[s].busy ? 1 : 0;
+    uint8_t *st = B.stage_acquire();
+    memcpy(st, f.data(), B.cap);
+    int rc = gfe::check_hip(hipMemcpyAsync(B.act_dev, st, B.cap, hipMemcpyHostToDevice, B.stream), "flags");
+    B.stage_release();
+    if (rc != GFE_OK) return rc;
+    (void)hipEventRecord(B.ev0, B.stream);
+    for (uint32_t r = 0; r < rounds; ++r) B.square(B.act_dev, nullptr);
+    (void)hipEventRecord(B.ev1, B.stream);
+    rc = gfe::check_hip(hipEventSynchronize(B.ev1), "time sync");
+    if (rc != GFE_OK) return rc;
+    float t = 0;
+    (void)hipEventElapsedTime(&t, B.ev0, B.ev1);
+    *ms = t;
+    return B.check("time_rounds");
+}
+
+int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
+    if (!b) return set_error(GFE_EINVAL, "null batch");
+    Batch &B = b->bt;
+    if (hipSetDevice(B.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    if (reps == 0) reps = 4;
+    bool any = false;
+    for (auto &S : B.slots) any |= S.busy;
+    if (!any) return set_error(GFE_EINVAL, "tune needs busy slots");
+    std::vector<gfe::Plan> plans;
+    for (int a = 4; a <= (int)B.n - 4; ++a)
+        for (int cl = 3; cl <= 5; ++cl)
+            for (int cng = 1; cng <= 64; cng *= 2)
+                for (int rl = 3; rl <= 5; ++rl)
+                    for (int rng = 1; rng <= 64; rng *= 2) {
+                        gfe::Plan P;
+                        P.a = a; P.col_loge = cl; P.col_ng = cng; P.row_loge = rl; P.row_ng = rng; P.norm_s = 8;
+                        const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (B.n - a)) / (1 << rl);
+                        if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 512) continue;
+                        if (!gfe::plan_valid(P, B.n)) continue;
+                        plans.push_back(P);
+                    }
+    // squarings on the busy slots' r0: save and restore register 0 of every slot
+    const size_t bytes = (size_t)B.cap * 3 * B.N * 4;
+    int32_t *save = nullptr;
+    if (hipMalloc(&save, bytes) != hipSuccess) return set_error(GFE_ENOMEM, "tune save");
+    (void)hipMemcpyAsync(save, B.regs, bytes, hipMemcpyDeviceToDevice, B.stream);
+    const gfe::Plan orig = B.plan;
+    gfe::Plan best = orig;
+    double best_ms = 1e30;
+    for (const gfe::Plan &P : plans) {
+        B.plan = P;
+        double ms = 0;
+        if (gfe_batch_time_rounds(b, 1, &ms) != GFE_OK) continue;
+        if (gfe_batch_time_rounds(b, reps, &ms) != GFE_OK) continue;
+        if (ms < best_ms) { best_ms = ms; best = P; }
+    }
+    B.plan = best;
+    (void)hipMemcpyAsync(B.regs, save, bytes, hipMemcpyDeviceToDevice, B.stream);
+    (void)hipStreamSynchronize(B.stream);
+    (void)hipFree(save);
+    return B.check("tune");
+}
+
+int gfe_batch_run(uint32_t n, const uint32_t *b, uint32_t count, uint64_t iters, uint32_t capacity, int device,
+                  gfe_batch_result *results) {
+    if (!b || !results || count == 0) return set_error(GFE_EINVAL, "batch_run");
+    gfe_batch *bt = nullptr;
+    int rc = gfe_batch_create(n, std::min(count, capacity ? capacity : count), device, &bt);
+    if (rc != GFE_OK) return rc;
+    uint32_t next = 0, done = 0;
+    std::vector<gfe_batch_result> buf(bt->bt.cap);
+    while (done < count) {
+        while (next < count) {
+            int32_t slot;
+            rc = gfe_batch_submit(bt, b[next], iters, next, &slot);
+            if (rc == GFE_EFULL) break;
+            if (rc != GFE_OK) { gfe_batch_destroy(bt); return rc; }
+            ++next;
+        }
+        int32_t active = 0;
+        rc = gfe_batch_step(bt, &active);
+        if (rc != GFE_OK) { gfe_batch_destroy(bt); return rc; }
+        uint32_t got = 0;
+        gfe_batch_poll(bt, buf.data(), (uint32_t)buf.size(), &got);
+        for (uint32_t k = 0; k < got; ++k) results[buf[k].tag] = buf[k];
+        done += got;
+    }
+    gfe_batch_destroy(bt);
+    return GFE_OK;
+}
+
+}  // extern "C"

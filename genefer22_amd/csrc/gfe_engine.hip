@@ -31,6 +31,43 @@ int set_error(int code, const std::string &msg) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Single-engine kernels of the small passes
+
+struct CarryArgs {
+    int32_t *dst;
+    const int64_t *carry;
+    uint32_t b;
+    unsigned nwg;   // workgroups of the carry pass (each 256 x 8 digits)
+};
+
+__global__ void __launch_bounds__(256) k_carry_wg(CarryArgs A) {
+    const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < A.nwg) carry_wg_body(A.dst, A.carry, A.nwg, A.b, w);
+}
+
+struct MulScalarArgs {
+    int32_t *dst;
+    int64_t *carry;
+    uint32_t b;
+    int32_t a;
+};
+
+__global__ void __launch_bounds__(256) k_mulscalar(MulScalarArgs A) { mulscalar_body(A.dst, A.carry, A.b, A.a, blockIdx.x); }
+
+// Register copy, 16 bytes per thread (the reference's copy kernel, ocl/kernel.cl:2501-2506).
+__global__ void __launch_bounds__(256) k_copy(int4 *__restrict__ dst, const int4 *__restrict__ src, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) dst[i] = src[i];
+}
+
+// r = a (digit 0 = a, others 0).
+__global__ void __launch_bounds__(256) k_set(int32_t *dst, size_t N, int32_t a) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) dst[i] = (i == 0) ? a : 0;
+}
+
+
+// ------------------------------------------------------------------------------------------------
 // Host modular arithmetic
 
 static const uint32_t k_primes[6] = {127u * (1u << 24) + 1u, 63u * (1u << 25) + 1u, 15u * (1u << 27) + 1u,
@@ -82,6 +119,12 @@ static void make_twiddles(int is32, int k, uint32_t n, uint32_t *out) {
         for (uint32_t bit = 0; bit < n; ++bit) r |= ((j >> bit) & 1) << (n - 1 - bit);
         out[j] = mulw_const(pw[r], p);
     }
+}
+
+void make_twiddles_set(int is32, int k, uint32_t n, uint32_t *out) { make_twiddles(is32, k, n, out); }
+
+void set_lds_attr(const void *fn) {
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
 CrtConst make_crt(int rns, int is32, uint32_t n) {
@@ -154,9 +197,9 @@ typedef void (*NormFn)(NormArgs);
 
 template <class M> static ColFn col_fwd_fn(int loge) {
     switch (loge) {
-        case 3: return k_col_fwd<M, 3, true>;
-        case 4: return k_col_fwd<M, 4, true>;
-        default: return k_col_fwd<M, 5, true>;
+        case 3: return k_col_fwd<M, 3>;
+        case 4: return k_col_fwd<M, 4>;
+        default: return k_col_fwd<M, 5>;
     }
 }
 template <class M> static ColFn col_inv_fn(int loge) {
@@ -288,7 +331,7 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
     const KernelSet K = kernels_for(rns, is32, plan);
     ColArgs A;
     A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
-    A.src_stride = 0; A.ws_stride = 0; A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng; A.rns = rns;
+    A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
     hipLaunchKernelGGL(K.col_fwd, dim3(C / plan.col_ng, rns, 1), dim3(T), lds_col(), stream, A);
@@ -298,7 +341,7 @@ void Engine::launch_col_inv() {
     const KernelSet K = kernels_for(rns, is32, plan);
     ColArgs A;
     A.src = nullptr; A.ws = ws; A.tw = tw; A.pr = pr;
-    A.src_stride = 0; A.ws_stride = 0; A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng; A.rns = rns;
+    A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
     hipLaunchKernelGGL(K.col_inv, dim3(C / plan.col_ng, rns, 1), dim3(T), lds_col(), stream, A);
@@ -307,7 +350,7 @@ void Engine::launch_col_inv() {
 void Engine::launch_row(int mode, uint32_t *data) {
     const KernelSet K = kernels_for(rns, is32, plan);
     RowArgs A;
-    A.ws = data; A.zp = zp; A.tw = tw; A.pr = pr; A.ws_stride = 0; A.zp_stride = 0;
+    A.ws = data; A.zp = zp; A.tw = tw; A.pr = pr;
     A.n = (int)n; A.a = plan.a; A.ng = plan.row_ng;
     const uint32_t R = 1u << plan.a, G = 1u << (n - plan.a);
     const int T = plan.row_ng * G / (1 << plan.row_loge);
@@ -318,8 +361,7 @@ void Engine::launch_row(int mode, uint32_t *data) {
 void Engine::launch_norm(int dup) {
     const KernelSet K = kernels_for(rns, is32, plan);
     NormArgs A;
-    A.ws = ws; A.dst = regs; A.carry = carry; A.crt = crt; A.b = bdev; A.dup = nullptr;
-    A.ws_stride = 0; A.dst_stride = 0; A.carry_stride = 0; A.crt_stride = 0; A.n = (int)n; A.dup_all = dup;
+    A.ws = ws; A.dst = regs; A.carry = carry; A.crt = crt; A.b = b; A.n = (int)n; A.dup = dup;
     const unsigned nwg = (unsigned)(N / (plan.norm_s * 256));
     hipLaunchKernelGGL(K.norm, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
     launch_carry();
@@ -327,7 +369,7 @@ void Engine::launch_norm(int dup) {
 
 void Engine::launch_carry() {
     CarryArgs A;
-    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0;
+    A.dst = regs; A.carry = carry; A.b = b;
     A.nwg = (unsigned)(N / (8 * 256));
     hipLaunchKernelGGL(k_carry_wg, dim3((A.nwg + 255) / 256, 1, 1), dim3(256), 0, stream, A);
 }
@@ -353,8 +395,7 @@ void Engine::init_multiplicand(uint32_t src) {
 
 void Engine::mul_scalar(int32_t a) {
     MulScalarArgs A;
-    A.dst = regs; A.carry = carry; A.b = bdev; A.dst_stride = 0; A.carry_stride = 0;
-    A.n = (int)n; A.a = a;
+    A.dst = regs; A.carry = carry; A.b = b; A.a = a;
     const unsigned nwg = (unsigned)(N / (8 * 256));
     hipLaunchKernelGGL(k_mulscalar, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
     launch_carry();
@@ -365,12 +406,12 @@ void Engine::copy(uint32_t dst, uint32_t src) {
     const int T = 256;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)((n4 + T - 1) / T), 1, 1), dim3(T), 0, stream,
                        reinterpret_cast<int4 *>(regs + (size_t)dst * N), reinterpret_cast<const int4 *>(regs + (size_t)src * N),
-                       n4, 0ll, 0ll);
+                       n4);
 }
 
 void Engine::set(uint32_t a) {
     const int T = 256;
-    hipLaunchKernelGGL(k_set, dim3((unsigned)((N + T - 1) / T), 1, 1), dim3(T), 0, stream, regs, 0ll, N, (int32_t)a);
+    hipLaunchKernelGGL(k_set, dim3((unsigned)((N + T - 1) / T), 1, 1), dim3(T), 0, stream, regs, N, (int32_t)a);
 }
 
 int Engine::check(const char *what) {

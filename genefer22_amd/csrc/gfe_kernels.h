@@ -1,5 +1,4 @@
-// gfe_kernels.h -- gfx950 kernels of the squaring engine (one candidate or a batch of candidates
-// of the same n and prime set; grid.z indexes the candidate).
+// gfe_kernels.h -- gfx950 kernels of the squaring engine.
 //
 // Transform (per prime p, length N = 2^n, psi a primitive 2N-th root of unity mod p):
 // negacyclic number-theoretic transform, Cooley-Tukey forward with roots
@@ -10,16 +9,19 @@
 // "z-transform"); here the last level is also split and the product is pointwise.
 //
 // Decomposition per squaring (levels 0..a-1 = "columns", a..n-1 = "rows", C = N / 2^a):
-//   k_col_fwd : digits -> residues, levels 0..a-1 on each column c (elements c + i C), NG columns
-//               per workgroup;
-//   k_row     : levels a..n-1 on each row (C contiguous elements), pointwise square / product,
-//               inverse levels n-1..a;
-//   k_col_inv : inverse levels a-1..0;
-//   k_norm    : scale, Garner CRT, balanced carry over S consecutive digits per thread; carries
-//               cross threads through LDS;
-//   k_carry_wg: adds each workgroup's carry-out into the next one (negated across x^N = -1).
+//   col_fwd  : digits -> residues, levels 0..a-1 on each column c (elements c + i C), NG columns
+//              per workgroup;
+//   row      : levels a..n-1 on each row (C contiguous elements), pointwise square / product,
+//              inverse levels n-1..a;
+//   col_inv  : inverse levels a-1..0;
+//   norm     : scale, Garner CRT, balanced carry over S consecutive digits per thread; carries
+//              cross threads through LDS;
+//   carry_wg : adds each workgroup's carry-out into the next one (negated across x^N = -1).
 // Inside a pass, a thread holds E = 2^LOGE elements of one sub-transform in registers and runs
 // LOGE levels per chunk; chunks exchange through LDS.
+//
+// Each pass is a __device__ body; the single-engine kernels below (grid.y = prime) and the batch
+// kernels of gfe_batch.hip (grid.z = candidate slot) are thin wrappers around the bodies.
 #pragma once
 
 #include "gfe_device.h"
@@ -93,31 +95,19 @@ struct Chunking {
     __device__ __forceinline__ int ne(int ci, int loge) const { return (r != 0 && ci == 0) ? r : loge; }
 };
 
-struct ColArgs {
-    const int32_t *src;   // digits (candidate stride src_stride) -- forward only
-    uint32_t *ws;         // residues, [cand][prime][N]
-    const uint32_t *tw;   // [prime][N], Montgomery form
-    PrimeArgs pr;
-    long long src_stride; // in int32
-    long long ws_stride;  // in uint32, per candidate
-    int n, a, ng, rns;
-};
+// ---------------------------------------------------------------------------------------------
+// Column passes.  Thread t -> column q = t % NG, sub-transform s = t / NG of the column block
+// starting at column blk * NG.  LDS element (q, k) at ((k + (k >> LOGE)) * NG + q).
 
-// Column pass, forward: levels 0..a-1.  Thread t -> column q = t % NG, sub-transform s = t / NG.
-// LDS element (q, k) at ((k + (k >> LOGE)) * NG + q).
-template <class M, int LOGE, bool FROM_DIGITS>
-__global__ void __launch_bounds__(1024) k_col_fwd(ColArgs A) {
+// Forward levels 0..a-1 from signed digits `src` (N int32) to residues `dst` (N uint32).
+template <class M, int LOGE>
+__device__ __forceinline__ void col_fwd_body(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                             const uint32_t *__restrict__ tw, int n, int g, int NG, unsigned blk,
+                                             uint32_t *lds) {
     constexpr int E = 1 << LOGE;
-    extern __shared__ uint32_t lds[];
-    const int prime = blockIdx.y, cand = blockIdx.z;
-    const M md{A.pr.p[prime], A.pr.q[prime]};
-    const int g = A.a, G = 1 << g, NG = A.ng;
-    const uint32_t C = 1u << (A.n - g);
-    const uint32_t c0 = blockIdx.x * NG;
+    const uint32_t C = 1u << (n - g);
+    const uint32_t c0 = blk * NG;
     const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *dst = A.ws + cand * A.ws_stride + (size_t)prime * N;
-    const uint32_t *tw = A.tw + (size_t)prime * N;
     const Chunking ch(g, LOGE);
     uint32_t x[E];
     for (int ci = 0; ci < ch.nch; ++ci) {
@@ -129,9 +119,7 @@ __global__ void __launch_bounds__(1024) k_col_fwd(ColArgs A) {
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) {
                 const uint32_t k = base + (rr << ylog);
-                const size_t gi = (size_t)k * C + c0 + q;
-                if (FROM_DIGITS) x[rr] = to_res(A.src[cand * A.src_stride + gi], md.p);
-                else x[rr] = dst[gi];
+                x[rr] = to_res(src[(size_t)k * C + c0 + q], md.p);
             }
         } else {
             __syncthreads();
@@ -159,20 +147,14 @@ __global__ void __launch_bounds__(1024) k_col_fwd(ColArgs A) {
     }
 }
 
-// Column pass, inverse: levels a-1..0, in place on ws.
+// Inverse levels a-1..0, in place on the residues `dst`.
 template <class M, int LOGE>
-__global__ void __launch_bounds__(1024) k_col_inv(ColArgs A) {
+__device__ __forceinline__ void col_inv_body(const M &md, uint32_t *__restrict__ dst, const uint32_t *__restrict__ tw,
+                                             int n, int g, int NG, unsigned blk, uint32_t *lds) {
     constexpr int E = 1 << LOGE;
-    extern __shared__ uint32_t lds[];
-    const int prime = blockIdx.y, cand = blockIdx.z;
-    const M md{A.pr.p[prime], A.pr.q[prime]};
-    const int g = A.a, NG = A.ng;
-    const uint32_t C = 1u << (A.n - g);
-    const uint32_t c0 = blockIdx.x * NG;
+    const uint32_t C = 1u << (n - g);
+    const uint32_t c0 = blk * NG;
     const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *dst = A.ws + cand * A.ws_stride + (size_t)prime * N;
-    const uint32_t *tw = A.tw + (size_t)prime * N;
     const Chunking ch(g, LOGE);
     uint32_t x[E];
     for (int ci = ch.nch - 1; ci >= 0; --ci) {
@@ -212,34 +194,25 @@ __global__ void __launch_bounds__(1024) k_col_inv(ColArgs A) {
     }
 }
 
-struct RowArgs {
-    uint32_t *ws;          // [cand][prime][N]
-    const uint32_t *zp;    // multiplicand [cand][prime][N] (MODE 1)
-    const uint32_t *tw;
-    PrimeArgs pr;
-    long long ws_stride, zp_stride;
-    int n, a, ng;
-};
+// ---------------------------------------------------------------------------------------------
+// Row pass: levels a..n-1 on rows of C = 2^(n-a) contiguous elements, the pointwise square or
+// product (Montgomery, factor 2^-32), then inverse levels n-1..a.  ROW_FWD stops after the
+// forward levels (multiplicand).  Thread t -> row q = t / (C/E) of the row block starting at row
+// blk * NG, sub-transform s = t % (C/E).  LDS element (q, k) at (q * (C + C/E) + k + (k >> LOGE)).
 
 enum RowMode { ROW_SQUARE = 0, ROW_MUL = 1, ROW_FWD = 2 };
 
-// Row pass: levels a..n-1 on rows of C = 2^(n-a) contiguous elements, the pointwise square or
-// product (Montgomery, factor 2^-32), then inverse levels n-1..a.  ROW_FWD stops after the
-// forward levels (multiplicand).  Thread t -> row q = t / (C/E), sub-transform s = t % (C/E).
-// LDS element (q, k) at (q * (C + C/E) + k + (k >> LOGE)).
+// `data` and `zp` point at the prime's N residues.
 template <class M, int LOGE, int MODE>
-__global__ void __launch_bounds__(1024) k_row(RowArgs A) {
+__device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ zp,
+                                         const uint32_t *__restrict__ tw, int n, int a, int NG, unsigned blk,
+                                         uint32_t *lds) {
     constexpr int E = 1 << LOGE;
-    extern __shared__ uint32_t lds[];
-    const int prime = blockIdx.y, cand = blockIdx.z;
-    const M md{A.pr.p[prime], A.pr.q[prime]};
-    const int g = A.n - A.a, G = 1 << g;
+    const int g = n - a, G = 1 << g;
     const int spr = G >> LOGE;  // sub-transforms per row
     const int t = threadIdx.x, q = t / spr, s = t % spr;
-    const uint32_t row = blockIdx.x * A.ng + q;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *dst = A.ws + cand * A.ws_stride + (size_t)prime * N + (size_t)row * G;
-    const uint32_t *tw = A.tw + (size_t)prime * N;
+    const uint32_t row = blk * NG + q;
+    uint32_t *dst = data + (size_t)row * G;
     uint32_t *L = lds + q * (G + spr);
     const Chunking ch(g, LOGE);
     uint32_t x[E];
@@ -261,7 +234,7 @@ __global__ void __launch_bounds__(1024) k_row(RowArgs A) {
                 x[rr] = L[k + (k >> LOGE)];
             }
         }
-        fwd_levels<M, LOGE>(md, x, tw, ((1u << (A.a + lam0)) + (row << lam0) + hi), ne);
+        fwd_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), ne);
         if (ci < ch.nch - 1) {
             __syncthreads();
 #pragma unroll
@@ -280,9 +253,9 @@ __global__ void __launch_bounds__(1024) k_row(RowArgs A) {
 #pragma unroll
         for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], x[rr]);
     } else {
-        const uint32_t *zp = A.zp + cand * A.zp_stride + (size_t)prime * N + (size_t)row * G;
+        const uint32_t *z = zp + (size_t)row * G;
 #pragma unroll
-        for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], zp[base + (rr << ylog)]);
+        for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], z[base + (rr << ylog)]);
     }
     for (int ci = ch.nch - 1; ci >= 0; --ci) {
         const int lam0 = ch.lam0(ci, LOGE), ne = ch.ne(ci, LOGE);
@@ -298,7 +271,7 @@ __global__ void __launch_bounds__(1024) k_row(RowArgs A) {
                 x[rr] = L[k + (k >> LOGE)];
             }
         }
-        inv_levels<M, LOGE>(md, x, tw, ((1u << (A.a + lam0)) + (row << lam0) + hi), A.a + lam0, ne);
+        inv_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), a + lam0, ne);
         if (ci == 0) {
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) dst[base + (rr << ylog)] = x[rr];
@@ -313,17 +286,8 @@ __global__ void __launch_bounds__(1024) k_row(RowArgs A) {
     }
 }
 
-struct NormArgs {
-    const uint32_t *ws;    // [cand][prime][N]
-    int32_t *dst;          // digits [cand][N] (register 0 of each candidate)
-    int64_t *carry;        // [cand][gridDim.x]: carry-out of each workgroup
-    const CrtConst *crt;   // [cand] (or one shared entry when crt_stride == 0)
-    const uint32_t *b;     // [cand]
-    const uint8_t *dup;    // [cand] or null (then dup_all)
-    long long ws_stride, dst_stride, carry_stride;
-    int crt_stride;
-    int n, dup_all;
-};
+// ---------------------------------------------------------------------------------------------
+// Carries
 
 // Adds carry c into d[0..S-1] digit by digit while a carry remains; the last digit absorbs the rest
 // (digits stay bounded: |d| <= b/2 + 4 for carries of the sizes a squaring produces).
@@ -341,13 +305,13 @@ __device__ __forceinline__ void absorb_carry(int32_t (&d)[S], int64_t c, uint32_
 
 // Second half of a carry pass: thread t owns the S digits d (computed with carry-in 0; its carry-out
 // is cr) starting at digit i0 of register dc.  The carry-out of thread t is added into thread t + 1's
-// digits through LDS; the workgroup's carry-out is stored to wc[blockIdx.x] for k_carry_wg, which
-// adds it into the next workgroup's first digits after this kernel (cross-workgroup data goes
-// through a kernel boundary: gfx950's per-XCD L2s make an in-kernel device-scope fence an L2
-// write-back).  blockDim.x must be 256.
+// digits through LDS; the workgroup's carry-out is stored to wc[blk] for carry_wg_body, which adds
+// it into the next workgroup's first digits after this kernel (cross-workgroup data goes through a
+// kernel boundary: gfx950's per-XCD L2s make an in-kernel device-scope fence an L2 write-back).
+// blockDim.x must be 256.
 template <int S>
 __device__ __forceinline__ void carry_tail(int32_t (&d)[S], int64_t cr, int32_t *dc, size_t i0, int64_t *wc,
-                                           uint32_t b, double binv) {
+                                           unsigned blk, uint32_t b, double binv) {
     constexpr int T = 256;
     __shared__ int64_t cs[T];
     const int tid = threadIdx.x;
@@ -357,51 +321,19 @@ __device__ __forceinline__ void carry_tail(int32_t (&d)[S], int64_t cr, int32_t 
     int32_t *o = dc + i0;
 #pragma unroll
     for (int j = 0; j < S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);
-    if (tid == T - 1) wc[blockIdx.x] = cr;
+    if (tid == T - 1) wc[blk] = cr;
 }
 
-struct CarryArgs {
-    int32_t *dst;
-    const int64_t *carry;
-    const uint32_t *b;
-    long long dst_stride, carry_stride;
-    unsigned nwg;   // workgroups of the carry pass (each 256 x 8 digits)
-};
-
-// Adds the carry-out of workgroup w - 1 (of the last one, negated, for w = 0: x^N = -1) into the
-// first digits of workgroup w's range.
-__global__ void __launch_bounds__(256) k_carry_wg(CarryArgs A) {
-    constexpr int S = 8;
-    const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= A.nwg) return;
-    const int cand = blockIdx.z;
-    const int64_t *wc = A.carry + cand * A.carry_stride;
-    const int64_t cin = (w == 0) ? -wc[A.nwg - 1] : wc[w - 1];
-    if (cin == 0) return;
-    const uint32_t b = A.b[cand];
-    int32_t *p = A.dst + cand * A.dst_stride + (size_t)w * 256 * S;
-    int32_t e[S];
-    const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);
-    e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w; e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
-    absorb_carry<S>(e, cin, b, 1.0 / (double)b);
-    *reinterpret_cast<int4 *>(p) = make_int4(e[0], e[1], e[2], e[3]);
-    *reinterpret_cast<int4 *>(p + 4) = make_int4(e[4], e[5], e[6], e[7]);
-}
-
-// Scale, Garner CRT and balanced carry.  Thread t owns S consecutive digits; carries cross threads
-// in carry_tail and workgroups in k_carry_wg.
+// Scale, Garner CRT and balanced carry of one candidate: `ws` = its rns x N residues, `dst` = its
+// register 0, `wc` = its workgroup carries; workgroup blk covers digits [blk 256 S, (blk+1) 256 S).
 template <int RNS, class M, int S>
-__global__ void __launch_bounds__(256) k_norm(NormArgs A) {
-    constexpr int T = 256;
-    const int cand = blockIdx.z, tid = threadIdx.x;
-    const size_t N = (size_t)1 << A.n;
-    const size_t i0 = ((size_t)blockIdx.x * T + tid) * S;
-    const CrtConst &c = A.crt[cand * A.crt_stride];
-    const uint32_t b = A.b[cand];
+__device__ __forceinline__ void norm_body(const CrtConst &c, const uint32_t *__restrict__ ws, int32_t *dst, int64_t *wc,
+                                          uint32_t b, bool dup, int n, unsigned blk) {
+    const size_t N = (size_t)1 << n;
+    const size_t i0 = ((size_t)blk * 256 + threadIdx.x) * S;
     const double binv = 1.0 / (double)b;
-    const bool dup = A.dup ? (A.dup[cand] != 0) : (A.dup_all != 0);
     const M m1{c.p[0], c.q[0]}, m2{c.p[1], c.q[1]}, m3{c.p[2], c.q[2]};
-    const uint32_t *w0 = A.ws + cand * A.ws_stride + i0;
+    const uint32_t *w0 = ws + i0;
     uint32_t a[MAX_RNS][S];
 #pragma unroll
     for (int k = 0; k < RNS; ++k) {
@@ -425,50 +357,100 @@ __global__ void __launch_bounds__(256) k_norm(NormArgs A) {
         xd += (double)cr;
         d[j] = div_bal(x64, xd, b, binv, &cr);
     }
-    carry_tail<S>(d, cr, A.dst + cand * A.dst_stride, i0, A.carry + cand * A.carry_stride, b, binv);
+    carry_tail<S>(d, cr, dst, i0, wc, blk, b, binv);
 }
-
-struct MulScalarArgs {
-    int32_t *dst;
-    int64_t *carry;
-    const uint32_t *b;
-    long long dst_stride, carry_stride;
-    int n;
-    int32_t a;
-};
 
 // r = a r on the balanced digits (the reference's mulscalar, ocl/kernel.cl:2461-2492, run after
 // the normalisation as baseModMul does, src/transformGPU.h:828-837), 8 digits per thread.
-__global__ void __launch_bounds__(256) k_mulscalar(MulScalarArgs A) {
+__device__ __forceinline__ void mulscalar_body(int32_t *dc, int64_t *wc, uint32_t b, int32_t a, unsigned blk) {
     constexpr int S = 8;
-    const int cand = blockIdx.z, tid = threadIdx.x;
-    const size_t i0 = ((size_t)blockIdx.x * 256 + tid) * S;
-    const uint32_t b = A.b[cand];
+    const size_t i0 = ((size_t)blk * 256 + threadIdx.x) * S;
     const double binv = 1.0 / (double)b;
-    int32_t *dc = A.dst + cand * A.dst_stride;
     int32_t d[S];
     const int4 v0 = *reinterpret_cast<const int4 *>(dc + i0), v1 = *reinterpret_cast<const int4 *>(dc + i0 + 4);
     d[0] = v0.x; d[1] = v0.y; d[2] = v0.z; d[3] = v0.w; d[4] = v1.x; d[5] = v1.y; d[6] = v1.z; d[7] = v1.w;
     int64_t c = 0;
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-        const int64_t f = (int64_t)d[j] * A.a + c;
+        const int64_t f = (int64_t)d[j] * a + c;
         d[j] = div_bal((uint64_t)f, (double)f, b, binv, &c);
     }
-    carry_tail<S>(d, c, dc, i0, A.carry + cand * A.carry_stride, b, binv);
+    carry_tail<S>(d, c, dc, i0, wc, blk, b, binv);
 }
 
-// Register copy, 16 bytes per thread (the reference's copy kernel, ocl/kernel.cl:2501-2506).
-__global__ void __launch_bounds__(256) k_copy(int4 *__restrict__ dst, const int4 *__restrict__ src, size_t n4,
-                                               long long dst_stride, long long src_stride) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n4) dst[blockIdx.z * dst_stride + i] = src[blockIdx.z * src_stride + i];
+// Adds the carry-out of workgroup w - 1 (of the last one, negated, for w = 0: x^N = -1) into the
+// first 8 digits of workgroup w's range (256 x 8 digits per workgroup).
+__device__ __forceinline__ void carry_wg_body(int32_t *dc, const int64_t *wc, unsigned nwg, uint32_t b, unsigned w) {
+    constexpr int S = 8;
+    const int64_t cin = (w == 0) ? -wc[nwg - 1] : wc[w - 1];
+    if (cin == 0) return;
+    int32_t *p = dc + (size_t)w * 256 * S;
+    int32_t e[S];
+    const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);
+    e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w; e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
+    absorb_carry<S>(e, cin, b, 1.0 / (double)b);
+    *reinterpret_cast<int4 *>(p) = make_int4(e[0], e[1], e[2], e[3]);
+    *reinterpret_cast<int4 *>(p + 4) = make_int4(e[4], e[5], e[6], e[7]);
 }
 
-// r = a (digit 0 = a, others 0).
-__global__ void __launch_bounds__(256) k_set(int32_t *dst, long long stride, size_t N, int32_t a) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N) dst[blockIdx.z * stride + i] = (i == 0) ? a : 0;
+// ---------------------------------------------------------------------------------------------
+// Single-engine kernels (grid.y = prime)
+
+struct ColArgs {
+    const int32_t *src;   // digits (forward)
+    uint32_t *ws;         // residues [prime][N]
+    const uint32_t *tw;   // [prime][N], Montgomery form
+    PrimeArgs pr;
+    int n, a, ng;
+};
+
+template <class M, int LOGE>
+__global__ void __launch_bounds__(1024) k_col_fwd(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    col_fwd_body<M, LOGE>(md, A.src, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+template <class M, int LOGE>
+__global__ void __launch_bounds__(1024) k_col_inv(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    col_inv_body<M, LOGE>(md, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+struct RowArgs {
+    uint32_t *ws;          // [prime][N]
+    const uint32_t *zp;    // multiplicand [prime][N] (ROW_MUL)
+    const uint32_t *tw;
+    PrimeArgs pr;
+    int n, a, ng;
+};
+
+template <class M, int LOGE, int MODE>
+__global__ void __launch_bounds__(1024) k_row(RowArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    row_body<M, LOGE, MODE>(md, A.ws + prime * N, A.zp + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+struct NormArgs {
+    const uint32_t *ws;    // [prime][N]
+    int32_t *dst;          // register 0
+    int64_t *carry;        // [gridDim.x]
+    const CrtConst *crt;
+    uint32_t b;
+    int n, dup;
+};
+
+template <int RNS, class M, int S>
+__global__ void __launch_bounds__(256) k_norm(NormArgs A) {
+    norm_body<RNS, M, S>(*A.crt, A.ws, A.dst, A.carry, A.b, A.dup != 0, A.n, blockIdx.x);
 }
 
 }  // namespace gfe

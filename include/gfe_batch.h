@@ -1,0 +1,80 @@
+/* gfe_batch.h -- C ABI of the batched squaring engine: many candidates b of the same n share one
+ * set of launches (grid.z = candidate slot), with every candidate's registers resident in HBM.
+ *
+ * The reference runs one candidate per process and per device (src/genefer.h:1363-1573 with
+ * transformGPUs, src/transformGPU.h:931-1195).  A batch runs the same per-candidate work -- the
+ * Fermat PRP loop with Gerbicz-Li updates and the final Gerbicz-Li check (src/genefer.h:479-615,
+ * 736-748) -- for up to `capacity` candidates at once; each candidate's residues are bit-exact with
+ * its own single-candidate run.  Prime sets (2/3 primes, 31/32-bit) may differ between slots.
+ *
+ * Gerbicz-Li cadence: the d(t) product is updated every B iterations with B = the reference's
+ * B_GerbiczLi(esize) rounded up to a multiple of 64, so that updates fall on the 64-iteration
+ * block boundaries of the batch; r0 (and so RES64) does not depend on B, and the check is the
+ * reference's with that B.
+ *
+ * Conventions of gfe.h: GFE_OK or a negative code, gfe_last_error() for the message; one batch
+ * must not be used from two threads at once.
+ */
+#ifndef GFE_BATCH_H
+#define GFE_BATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFE_EFULL (-6) /* no free slot */
+
+typedef struct gfe_batch gfe_batch;
+
+typedef struct gfe_batch_result {
+    uint64_t tag;      /* caller's tag of the request */
+    uint32_t b, n;
+    uint64_t res64, old64, hash64;
+    uint64_t esize;    /* bits of E = b^(2^n) */
+    uint64_t iters;    /* PRP squarings run (esize for a full test) */
+    int32_t is_prp;    /* residue == 1 (full tests) */
+    int32_t gl_ok;     /* Gerbicz-Li check: 1 pass, 0 fail, -1 not run (truncated request) */
+    double t_submit, t_done;  /* seconds on the batch's clock (steady clock since creation) */
+} gfe_batch_result;
+
+/* A batch of `capacity` slots for candidates b^(2^n) + 1, 12 <= n <= 23. */
+int gfe_batch_create(uint32_t n, uint32_t capacity, int device, gfe_batch **out);
+int gfe_batch_destroy(gfe_batch *bt);
+/* Device bytes one slot holds (registers, work space, multiplicand, carries): the admission unit. */
+size_t gfe_batch_slot_bytes(uint32_t n);
+/* Chooses the pass plan by timing the admissible ones on `reps` rounds of the batch's current
+ * slots (or of synthetic full slots when empty); the reference's tune, src/transformGPU.h:851-900. */
+int gfe_batch_tune(gfe_batch *bt, uint32_t reps);
+int gfe_batch_info(const gfe_batch *bt, char *buf, size_t len);
+
+/* Submits the quick test of b^(2^n) + 1 (iters == 0: the whole PRP loop, then the Gerbicz-Li
+ * check) or the first `iters` squarings of its PRP loop (no check).  *slot receives the slot; returns
+ * GFE_EFULL when every slot is busy. */
+int gfe_batch_submit(gfe_batch *bt, uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot);
+/* Runs the pending register operations of every slot, then one block of up to 64 squaring rounds
+ * for every slot in a squaring run.  *active receives the number of busy slots after the step. */
+int gfe_batch_step(gfe_batch *bt, int32_t *active);
+/* Moves up to `max` finished results to out (their slots become free); *count receives how many. */
+int gfe_batch_poll(gfe_batch *bt, gfe_batch_result *out, uint32_t max, uint32_t *count);
+/* Number of busy slots. */
+int gfe_batch_active(const gfe_batch *bt);
+/* Canonical digits of register `reg` of a busy slot (for checks; waits for the queued work). */
+int gfe_batch_get_digits(gfe_batch *bt, int32_t slot, uint32_t reg, int32_t *out);
+
+/* Benchmark: times `rounds` squaring rounds (one squaring of every busy slot per round) with HIP
+ * events, without advancing the slots' programs (their register 0 is squared in place). */
+int gfe_batch_time_rounds(gfe_batch *bt, uint32_t rounds, double *ms);
+
+/* Runs quick tests (or their first `iters` squarings) of `count` candidates of one n to completion
+ * in one batch of min(count, capacity) slots, refilling slots as candidates finish.
+ * results[k] is the result of b[k]. */
+int gfe_batch_run(uint32_t n, const uint32_t *b, uint32_t count, uint64_t iters, uint32_t capacity, int device,
+                  gfe_batch_result *results);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

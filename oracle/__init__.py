@@ -405,6 +405,8 @@ class RefEngine:
             L.gfref_get_error.argtypes = [ctypes.c_void_p]
             L.gfref_get_digits.argtypes = [ctypes.c_void_p, I32P, ctypes.c_int]
             L.gfref_set_digits.argtypes = [ctypes.c_void_p, I32P]
+            L.gfref_prp_trace.restype = ctypes.c_long
+            L.gfref_prp_trace.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_long, U64P, U64P]
             L.gfref_prp.restype = ctypes.c_long
             L.gfref_prp.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_int, ctypes.c_int, U64P, U64P,
                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -450,6 +452,14 @@ class RefEngine:
 
     def residue(self, reg: int = 0) -> Residue:
         return residue(self.get_digits(reg), self.n, self.b)
+
+    def prp_trace(self, iters: int, every: int):
+        """RES64 / old64 after every `every`-th squaring of the reference PRP loop, up to iters."""
+        cnt = iters // every
+        r = (ctypes.c_uint64 * max(cnt, 1))()
+        o = (ctypes.c_uint64 * max(cnt, 1))()
+        got = self._lib().gfref_prp_trace(self._h, iters, every, r, o)
+        return [(r[i], o[i]) for i in range(got)]
 
     def prp(self, iters: int = 0, b_gl: int = 0, do_gl: bool = False):
         """Reference PRP loop in native code (ref_shim.cpp gfref_prp).  Returns

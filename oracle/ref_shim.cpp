@@ -156,4 +156,37 @@ long gfref_prp(void *h, long iters, int b_gl, int do_gl, uint64_t *res64, uint64
     return esize;
 }
 
+// The PRP loop as gfref_prp, recording RES64 / old64 of register 0 after every `every`-th
+// squaring (squarings counted from 1) up to `iters`; returns the number of records.
+long gfref_prp_trace(void *h, long iters, long every, uint64_t *res64, uint64_t *old64) {
+    auto *e = static_cast<ref_engine *>(h);
+    transform *t = e->t;
+    mpz_t exponent; mpz_init(exponent);
+    mpz_ui_pow_ui(exponent, e->b, 1ul << e->n);
+    const long esize = (long)mpz_sizeinbase(exponent, 2);
+    const int B_GL = (int)((esize - 1) / (2l << (ilog2_32((uint32_t)esize) / 2)) + 1);
+    t->set(1);
+    t->copy(1, 0);
+    const long i0 = esize - 1;
+    const long i_end = (iters <= 0 || iters >= esize) ? 0 : esize - iters;
+    long rec = 0;
+    for (long i = i0; i >= i_end; --i) {
+        t->squareDup(mpz_tstbit(exponent, (mp_bitcnt_t)i) != 0);
+        if ((i % B_GL == 0) && (i / B_GL != 0)) {
+            t->copy(2, 0);
+            t->mul(1);
+            t->copy(1, 0);
+            t->copy(0, 2);
+        }
+        const long k = i0 - i + 1;
+        if (every > 0 && k % every == 0) {
+            uint64_t hh = 0;
+            gfref_res64(h, &res64[rec], &old64[rec], &hh);
+            ++rec;
+        }
+    }
+    mpz_clear(exponent);
+    return rec;
+}
+
 }  // extern "C"

@@ -1,0 +1,64 @@
+"""GPU parity of the batched engine: every candidate's result in a batch equals its own
+single-candidate run, the reference's KATs and the oracle, with mixed prime sets in one batch."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batch_quick_n12_kats(gfe):
+    """Full quick tests of the reference's n=12 KATs (src/main.cpp:416-425) in one batch, all four
+    prime sets mixed; RES64 equal to the KATs and the Gerbicz-Li check passes."""
+    kats = {23445612: None, 45686464: None, 999999618: None, 1999999266: None,
+            23448336: 0x81CFF004ACE85CAA, 45687570: 0x6C3EA0FBFFEE2C34,
+            1000000000: 0x4E43A5B93273C649, 2000000000: 0xE5F3EA34F6C68EBD}
+    bases = list(kats)
+    out = gfe.run_batch(12, bases)
+    for b, r in zip(bases, out):
+        assert r["b"] == b and r["gl_ok"] == 1, r
+        if kats[b] is not None:
+            assert r["res64"] == kats[b], (b, hex(r["res64"]))
+    # same residues as one candidate per engine
+    for b, r in zip(bases[:4], out[:4]):
+        single = gfe.run_quick(b, 12)
+        assert (single["res64"], single["old64"]) == (r["res64"], r["old64"])
+
+
+@pytest.mark.parametrize("n", [13, 15])
+def test_batch_truncated_matches_oracle(gfe, oracle, n):
+    """First k PRP squarings of several candidates in one batch (slot refill: capacity < count)
+    against the oracle's restatement."""
+    rs = np.random.RandomState(n)
+    bases = [int(x) * 2 for x in rs.randint(50_000_000, 1_000_000_000, 6)] + [1999999266, 20000]
+    iters = 48 if n == 13 else 24
+    out = gfe.run_batch(n, bases, iters=iters, capacity=5)
+    for b, r in zip(bases, out):
+        e = oracle.Engine(b, n)
+        oracle.prp(e, oracle.exponent(b, n), iters=iters, B_GL=-1)
+        want = e.residue(0)
+        assert r["iters"] == iters
+        assert r["res64"] == want.res64 and r["old64"] == want.old64, (b, hex(r["res64"]), hex(want.res64))
+
+
+def test_batch_alone_equals_mixed(gfe):
+    """A candidate's result does not depend on what else is in the batch."""
+    mix = [1000000000, 1534, 30000000, 1999999266, 777777778, 4096 + 2]
+    alone = gfe.run_batch(14, [mix[0]], iters=300)[0]
+    mixed = gfe.run_batch(14, mix, iters=300)[0]
+    assert (alone["res64"], alone["old64"], alone["hash64"]) == (mixed["res64"], mixed["old64"], mixed["hash64"])
+
+
+def test_batch_api_submit_step_poll(gfe):
+    bt = gfe.Batch(12, 3)
+    slots = [bt.submit(b, iters=100, tag=i) for i, b in enumerate([1000000000, 1534, 2000000000])]
+    assert sorted(slots) == [0, 1, 2]
+    assert bt.submit(1026, iters=10) is None  # full
+    done = []
+    while len(done) < 3:
+        bt.step()
+        done += bt.poll()
+    assert sorted(r["tag"] for r in done) == [0, 1, 2]
+    for r in done:
+        single = gfe.run_prp(r["b"], 12, iters=100, b_gl=-1)
+        assert single["res64"] == r["res64"]
+    bt.close()
