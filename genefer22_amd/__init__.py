@@ -61,6 +61,7 @@ ABI = [
     ("gfe_tune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("gfe_run_quick", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(Result)]),
     ("gfe_run_prp", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_int32, I32P, ctypes.POINTER(Result)]),
+    ("gfe_run_prp_trace", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, U64P, U64P, ctypes.POINTER(Result)]),
     ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),
     ("gfe_residue", ctypes.c_int, [I32P, ctypes.c_uint32, ctypes.c_uint32, I32P, ctypes.POINTER(Result)]),
     ("gfe_batch_create", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -236,6 +237,16 @@ def run_prp(b: int, n: int, iters: int = 0, b_gl: int = 0, device: int = 0, digi
     if digits:
         out["digits"] = d
     return out
+
+
+def run_prp_trace(b: int, n: int, iters: int, every: int, device: int = 0) -> list[tuple[int, int]]:
+    """(RES64, old64) after every `every`-th squaring of the PRP loop, up to `iters`."""
+    cnt = iters // every
+    r = (ctypes.c_uint64 * max(cnt, 1))()
+    o = (ctypes.c_uint64 * max(cnt, 1))()
+    res = Result()
+    _ck(lib().gfe_run_prp_trace(b, n, device, iters, every, r, o, ctypes.byref(res)))
+    return [(r[i], o[i]) for i in range(cnt)]
 
 
 def run_proof(b: int, n: int, depth: int, device: int = 0) -> dict:

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <map>
 #include <memory>
 #include <vector>
@@ -89,6 +90,9 @@ public:
     // The PRP loop, src/genefer.h:479-552.  Runs exponent bits esize-1 .. i_end (i_end = 0: all).
     // Pietrzak-Li checkpoints go to registers 3 + i/B_PL when ckpt_in_regs (the reference's
     // fast_checkpoints), else to host digit vectors.
+    // Called after each squaring k = 1, 2, ... of prp() when set.
+    std::function<void(long)> on_square;
+
     void prp(int B_GL, int B_PL, bool ckpt_in_regs, long i_end = 0) {
         transformHIP &T = *_t;
         T.set(1);
@@ -110,6 +114,7 @@ public:
                     _ckpts[size_t(i / B_PL)].assign(_g.data(), _g.data() + _g.getSize());
                 }
             }
+            if (on_square) on_square(i0 - i + 1);
         }
     }
 

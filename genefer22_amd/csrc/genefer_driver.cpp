@@ -75,6 +75,38 @@ int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl
     }
 }
 
+int gfe_run_prp_trace(uint32_t b, uint32_t n, int device, uint64_t iters, uint64_t every, uint64_t *res64,
+                      uint64_t *old64, gfe_result *out) {
+    if (!out || !res64 || !old64 || every == 0) return gfe::set_error(GFE_EINVAL, "trace arguments");
+    memset(out, 0, sizeof(*out));
+    try {
+        const double t0 = now_s();
+        gfe::genefer G(b, n, device, 3);
+        const size_t es = G.esize();
+        const int B_GL = gfe::genefer::B_GerbiczLi(es);
+        const long i_end = (iters == 0 || iters >= es) ? 0 : (long)(es - iters);
+        size_t rec = 0;
+        G.on_square = [&](long k) {
+            if ((uint64_t)k % every == 0) {
+                G.t().getInt(G.gi());
+                G.gi().isOne(res64[rec], old64[rec]);
+                ++rec;
+            }
+        };
+        G.prp(B_GL, 0, false, i_end);
+        G.on_square = nullptr;
+        G.t().getInt(G.gi());
+        fill_residue(G.gi(), out);
+        out->esize = es;
+        out->iters = es - (size_t)i_end;
+        out->gl_ok = -1;
+        out->seconds = now_s() - t0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EHIP, ex.what());
+    }
+}
+
 int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_hash64, gfe_result *out) {
     if (!out || depth < 1 || depth > 12) return gfe::set_error(GFE_EINVAL, "proof arguments");
     memset(out, 0, sizeof(*out));

@@ -34,6 +34,11 @@ int gfe_run_quick(uint32_t b, uint32_t n, int device, gfe_result *out);
 int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl, int32_t *digits_out,
                 gfe_result *out);
 
+/* As gfe_run_prp, recording RES64 / old64 of r0 after every `every`-th squaring (counted from 1)
+ * into res64[] / old64[] (iters / every entries); the d(t) update runs at the reference's cadence. */
+int gfe_run_prp_trace(uint32_t b, uint32_t n, int device, uint64_t iters, uint64_t every, uint64_t *res64,
+                      uint64_t *old64, gfe_result *out);
+
 /* proof(), src/genefer.h:750-766: PRP with Pietrzak-Li checkpoints (in registers for n <= 17 as the
  * reference's fast_checkpoints, host copies otherwise), the Gerbicz-Li check, then the proof.
  * mu_hash64 (may be null) receives gethash64 of mu_0..mu_depth (depth + 1 values). */

@@ -385,6 +385,17 @@ class RefEngine:
         return os.path.exists(cls.PATH)
 
     @classmethod
+    def gint(cls, digits: np.ndarray, n: int, b: int):
+        """The reference's gint on a digit vector: (canonical digits, res64, old64, hash64, hash32, is_one)."""
+        d = np.ascontiguousarray(digits, dtype=np.int32)
+        c = np.zeros(1 << n, dtype=np.int32)
+        r, o, h = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        h32 = ctypes.c_uint32()
+        one = cls._lib().gfref_gint(_i32(d), n, b, _i32(c), ctypes.byref(r), ctypes.byref(o), ctypes.byref(h),
+                                    ctypes.byref(h32))
+        return c, r.value, o.value, h.value, h32.value, bool(one)
+
+    @classmethod
     def _lib(cls):
         global _REF
         if _REF is None:
@@ -405,6 +416,8 @@ class RefEngine:
             L.gfref_get_error.argtypes = [ctypes.c_void_p]
             L.gfref_get_digits.argtypes = [ctypes.c_void_p, I32P, ctypes.c_int]
             L.gfref_set_digits.argtypes = [ctypes.c_void_p, I32P]
+            L.gfref_gint.restype = ctypes.c_int
+            L.gfref_gint.argtypes = [I32P, ctypes.c_uint32, ctypes.c_uint32, I32P, U64P, U64P, U64P, U32P]
             L.gfref_prp_trace.restype = ctypes.c_long
             L.gfref_prp_trace.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_long, U64P, U64P]
             L.gfref_prp.restype = ctypes.c_long

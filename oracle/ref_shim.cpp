@@ -156,6 +156,20 @@ long gfref_prp(void *h, long iters, int b_gl, int do_gl, uint64_t *res64, uint64
     return esize;
 }
 
+// The reference's gint on a digit vector: canonical digits (into canon, may be null), RES64 /
+// old64 / isOne (src/gint.h:169-202), hash64 / hash32 (204-225; 0 when the value is 1).
+int gfref_gint(const int32_t *digits, uint32_t n, uint32_t b, int32_t *canon, uint64_t *res64, uint64_t *old64,
+               uint64_t *hash64, uint32_t *hash32) {
+    gint g(size_t(1) << n, b);
+    std::memcpy(g.data(), digits, (size_t(1) << n) * sizeof(int32_t));
+    g.reset();
+    const bool one = g.isOne(*res64, *old64);
+    *hash64 = one ? 0 : g.gethash64();
+    *hash32 = one ? 0 : g.gethash32();
+    if (canon) std::memcpy(canon, g.data(), (size_t(1) << n) * sizeof(int32_t));
+    return one ? 1 : 0;
+}
+
 // The PRP loop as gfref_prp, recording RES64 / old64 of register 0 after every `every`-th
 // squaring (squarings counted from 1) up to `iters`; returns the number of records.
 long gfref_prp_trace(void *h, long iters, long every, uint64_t *res64, uint64_t *old64) {
