@@ -26,6 +26,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from genefer22_amd import workloads as W  # noqa: E402
+
 METRIC = "ns per squaring at n=22; candidates/hour at n=15 batched vs plain, bit-exact"
 HBM_PEAK_GBS = 8000.0
 
@@ -34,33 +36,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def splitmix64(state: int):
-    while True:
-        state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
-        z = state
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
-        yield z ^ (z >> 31)
-
-
-def seeded_bases(seed: int, lo: int, hi: int, count: int) -> list[int]:
-    """Even b uniform in [lo, hi), b >= 1024, not a power of two (src/main.cpp:249-253)."""
-    out, g = [], splitmix64(seed)
-    while len(out) < count:
-        b = lo + 2 * ((next(g) % ((hi - lo) // 2)))
-        if b >= 1024 and (b & (b - 1)) != 0:
-            out.append(b)
-    return out
-
-
-def esize_of(b: int, n: int) -> int:
-    """Bits of E = b^(2^n) (the number of squarings of a full test)."""
-    return (b ** (1 << n)).bit_length()
-
-
 def bench_n22(G, steps: int, warmup: int):
-    b = seeded_bases(22, 10_000_000, 35_000_000, 1)[0]
     n = 22
+    b = W.config_bases(W.C4, 1)[0]
     t = G.Transform(b, n)
     log(f"[bench] n=22 b={b}: tuning")
     t.tune(6)
@@ -109,8 +87,8 @@ def bench_batched_n15(G, count: int, block: int):
     Both legs time squaring iterations with HIP events on resident state; candidates/hour =
     3600 s / (seconds per squaring x mean esize) x candidates per iteration."""
     n = 15
-    bases = seeded_bases(15, 100_000_000, 2_000_000_000, count)
-    mean_es = sum(esize_of(b, n) for b in bases) / count
+    bases = W.config_bases(W.C3, count)
+    mean_es = sum(W.esize(b, n) for b in bases) / count
     log(f"[bench] n=15: {count} candidates, mean esize {mean_es:.0f}")
     bt = G.Batch(n, count)
     for i, b in enumerate(bases):

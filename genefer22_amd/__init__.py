@@ -35,6 +35,28 @@ class Result(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServeResult(ctypes.Structure):
+    _fields_ = [("tag", ctypes.c_uint64), ("n", ctypes.c_uint32), ("b", ctypes.c_uint32),
+                ("res64", ctypes.c_uint64), ("old64", ctypes.c_uint64), ("hash64", ctypes.c_uint64),
+                ("esize", ctypes.c_uint64), ("iters", ctypes.c_uint64), ("is_prp", ctypes.c_int32),
+                ("gl_ok", ctypes.c_int32), ("t_arrival", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("admitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("queued", ctypes.c_uint64), ("bytes_budget", ctypes.c_uint64), ("bytes_reserved", ctypes.c_uint64),
+                ("bytes_peak", ctypes.c_uint64), ("pools", ctypes.c_uint32), ("pools_created", ctypes.c_uint32),
+                ("pools_released", ctypes.c_uint32), ("busy_slots", ctypes.c_uint32), ("pumps", ctypes.c_uint64),
+                ("blocks", ctypes.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every exported entry point; tests check each is present.
 ABI = [
     ("gfe_create", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
@@ -76,6 +98,18 @@ ABI = [
     ("gfe_batch_get_digits", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint32, I32P]),
     ("gfe_batch_time_rounds", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     ("gfe_batch_run", ctypes.c_int, [ctypes.c_uint32, U32P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]),
+    ("gfe_batch_backlog", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_batch_device_bytes", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32]),
+    ("gfe_server_create", ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("gfe_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_server_clock", ctypes.c_double, [ctypes.c_void_p]),
+    ("gfe_server_prepare", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    ("gfe_server_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]),
+    ("gfe_server_pump", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
+    ("gfe_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    ("gfe_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    ("gfe_serve_replay", ctypes.c_int, [U32P, U32P, U64P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_int,
+                                        ctypes.c_uint64, U32P, ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
 ]
 
 
@@ -315,6 +349,11 @@ class Batch:
     def active(self) -> int:
         return lib().gfe_batch_active(self._h)
 
+    def backlog(self) -> int:
+        r = lib().gfe_batch_backlog(self._h)
+        _ck(min(r, 0))
+        return r
+
     def get_digits(self, slot: int, reg: int = 0) -> np.ndarray:
         d = np.zeros(1 << self.n, dtype=np.int32)
         _ck(lib().gfe_batch_get_digits(self._h, slot, reg, _i32(d)))
@@ -333,3 +372,65 @@ def run_batch(n: int, bases, iters: int = 0, capacity: int = 0, device: int = 0)
     res = (BatchResult * len(b))()
     _ck(lib().gfe_batch_run(n, b.ctypes.data_as(U32P), len(b), iters, capacity or len(b), device, res))
     return [res[i].as_dict() for i in range(len(b))]
+
+
+class Server:
+    """The serving loop (include/gfe_serve.h): requests of mixed n, continuous batching."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = ctypes.c_void_p()
+        _ck(lib().gfe_server_create(device, hbm_budget, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gfe_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self) -> float:
+        return lib().gfe_server_clock(self._h)
+
+    def prepare(self, n: int, capacity: int = 0):
+        _ck(lib().gfe_server_prepare(self._h, n, capacity))
+
+    def submit(self, n: int, b: int, iters: int = 0, tag: int = 0):
+        _ck(lib().gfe_server_submit(self._h, n, b, iters, tag))
+
+    def pump(self) -> int:
+        busy = ctypes.c_int32(0)
+        _ck(lib().gfe_server_pump(self._h, ctypes.byref(busy)))
+        return busy.value
+
+    def poll(self, max_results: int = 4096) -> list[dict]:
+        arr = (ServeResult * max_results)()
+        cnt = ctypes.c_uint32(0)
+        _ck(lib().gfe_server_poll(self._h, arr, max_results, ctypes.byref(cnt)))
+        return [arr[i].as_dict() for i in range(cnt.value)]
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _ck(lib().gfe_server_get_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+
+def serve_replay(requests, device: int = 0, hbm_budget: int = 0, prepare=()) -> tuple[list[dict], dict]:
+    """Open-loop replay of (n, b, iters, arrival_seconds) requests through the serving loop.
+    Returns (results in request order, server statistics)."""
+    k = len(requests)
+    n = np.ascontiguousarray([r[0] for r in requests], dtype=np.uint32)
+    b = np.ascontiguousarray([r[1] for r in requests], dtype=np.uint32)
+    it = np.ascontiguousarray([r[2] for r in requests], dtype=np.uint64)
+    t = np.ascontiguousarray([r[3] for r in requests], dtype=np.float64)
+    pn = np.ascontiguousarray(list(prepare) or [0], dtype=np.uint32)
+    res = (ServeResult * k)()
+    st = ServerStats()
+    _ck(lib().gfe_serve_replay(n.ctypes.data_as(U32P), b.ctypes.data_as(U32P), it.ctypes.data_as(U64P),
+                               t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k, device, hbm_budget,
+                               pn.ctypes.data_as(U32P), len(prepare), res, ctypes.byref(st)))
+    return [res[i].as_dict() for i in range(k)], st.as_dict()

@@ -184,6 +184,7 @@ struct Slot {
     uint64_t pos = 0;                 // progress inside the head run
     uint64_t res64 = 0, old64 = 0, hash64 = 0, h1 = 0;
     int is_prp = 0, gl_ok = -1;
+    int reads_pending = 0;            // result read-backs enqueued and not yet processed
 
     bool bit(uint64_t k) const {  // k-th squaring of the PRP loop (bit esize - 1 - k)
         const uint64_t i = esize - 1 - k;
@@ -203,14 +204,35 @@ struct Batch {
     CrtConst *crt = nullptr;
     SlotInfo *slots_dev = nullptr;
     uint8_t *act_dev = nullptr, *dup_dev = nullptr;   // [64][cap] each
-    uint8_t *stage[2] = {nullptr, nullptr};           // pinned, 2 x (2 x 64 x cap)
-    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // Pinned ring of flag uploads: a buffer is rewritten only after its last upload has ended.
+    static constexpr int NSTAGE = 8;
+    uint8_t *stage[NSTAGE] = {};                      // each 2 x 64 x cap
+    hipEvent_t stage_ev[NSTAGE] = {};
     int stage_i = 0;
-    int32_t *readback = nullptr;                      // pinned N int32
+    // Pinned mirrors of the per-slot tables.  Entry s is written only by submit() into a free
+    // slot; the previous occupant's program ended with a read-back, whose completion the host
+    // waited for, so no upload of entry s is still pending then.
+    SlotInfo *slot_host = nullptr;
+    CrtConst *crt_host = nullptr;
+    int32_t *readback = nullptr;                      // pinned N int32 (synchronous reads)
+    // Asynchronous result read-backs: pinned N-digit buffers, each with a completion event,
+    // processed in stream order.
+    static constexpr int NREAD = 8;
+    int32_t *rbuf[NREAD] = {};
+    hipEvent_t rev[NREAD] = {};
+    struct Read { int slot, kind, buf; };
+    std::deque<Read> reads;
+    std::vector<int> rfree;
+    // One event after each enqueued squaring block: the device backlog of this batch.
+    static constexpr int NBLK = 8;
+    hipEvent_t blk_ev[NBLK] = {};
+    std::deque<int> blk_q;
+    int blk_next = 0;
+    uint32_t zl = 0;                                  // 1 + highest busy slot: grid.z of launches
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     PrimeArgs prS, prU;
     std::vector<Slot> slots;
-    std::vector<gfe_batch_result> finished;
+    std::deque<gfe_batch_result> finished;
     double t0 = 0;
     bool ok = false;
 
@@ -236,6 +258,12 @@ struct Batch {
     int submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot);
     int step(int32_t *active);
     int read_residue(int s, uint32_t reg, gint &g);
+    int read_issue(int s, int kind);
+    int read_complete(bool wait);
+    int harvest();
+    int backlog();
+    void update_zl();
+    unsigned zdim() const { return zl ? zl : 1u; }
     int check(const char *what);
 };
 
@@ -252,10 +280,18 @@ Batch::~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     (void)hipFree(regs); (void)hipFree(ws); (void)hipFree(zp); (void)hipFree(tw); (void)hipFree(carry);
     (void)hipFree(crt); (void)hipFree(slots_dev); (void)hipFree(act_dev); (void)hipFree(dup_dev);
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NSTAGE; ++i) {
         if (stage[i]) (void)hipHostFree(stage[i]);
         if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
     }
+    for (int i = 0; i < NREAD; ++i) {
+        if (rbuf[i]) (void)hipHostFree(rbuf[i]);
+        if (rev[i]) (void)hipEventDestroy(rev[i]);
+    }
+    for (int i = 0; i < NBLK; ++i)
+        if (blk_ev[i]) (void)hipEventDestroy(blk_ev[i]);
+    if (slot_host) (void)hipHostFree(slot_host);
+    if (crt_host) (void)hipHostFree(crt_host);
     if (readback) (void)hipHostFree(readback);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -282,10 +318,20 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
         hipMalloc(&slots_dev, (size_t)cap * sizeof(SlotInfo)) != hipSuccess ||
         hipMalloc(&act_dev, (size_t)64 * cap) != hipSuccess || hipMalloc(&dup_dev, (size_t)64 * cap) != hipSuccess)
         return set_error(GFE_ENOMEM, "device allocation failed");
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NSTAGE; ++i) {
         if (hipHostMalloc((void **)&stage[i], (size_t)2 * 64 * cap) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
         if (hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming) != hipSuccess) return set_error(GFE_EHIP, "event");
     }
+    for (int i = 0; i < NREAD; ++i) {
+        if (hipHostMalloc((void **)&rbuf[i], N * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
+        if (hipEventCreateWithFlags(&rev[i], hipEventDisableTiming) != hipSuccess) return set_error(GFE_EHIP, "event");
+        rfree.push_back(i);
+    }
+    for (int i = 0; i < NBLK; ++i)
+        if (hipEventCreateWithFlags(&blk_ev[i], hipEventDisableTiming) != hipSuccess) return set_error(GFE_EHIP, "event");
+    if (hipHostMalloc((void **)&slot_host, (size_t)cap * sizeof(SlotInfo)) != hipSuccess ||
+        hipHostMalloc((void **)&crt_host, (size_t)cap * sizeof(CrtConst)) != hipSuccess)
+        return set_error(GFE_ENOMEM, "pinned");
     if (hipHostMalloc((void **)&readback, N * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
     if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return set_error(GFE_EHIP, "event");
     std::vector<uint32_t> h(N);
@@ -331,7 +377,7 @@ void Batch::launch_col_fwd(const BArgs &A0, int to_zp) {
     A.a = plan.a; A.ng = plan.col_ng;
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
-    const dim3 grid(C / plan.col_ng, 3, cap);
+    const dim3 grid(C / plan.col_ng, 3, zdim());
     switch (plan.col_loge) {
         case 3: hipLaunchKernelGGL(kb_col_fwd<3>, grid, dim3(T), lds_col(), stream, A, to_zp); break;
         case 4: hipLaunchKernelGGL(kb_col_fwd<4>, grid, dim3(T), lds_col(), stream, A, to_zp); break;
@@ -344,7 +390,7 @@ void Batch::launch_col_inv(const BArgs &A0) {
     A.a = plan.a; A.ng = plan.col_ng;
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
-    const dim3 grid(C / plan.col_ng, 3, cap);
+    const dim3 grid(C / plan.col_ng, 3, zdim());
     switch (plan.col_loge) {
         case 3: hipLaunchKernelGGL(kb_col_inv<3>, grid, dim3(T), lds_col(), stream, A); break;
         case 4: hipLaunchKernelGGL(kb_col_inv<4>, grid, dim3(T), lds_col(), stream, A); break;
@@ -366,7 +412,7 @@ void Batch::launch_row(const BArgs &A0, int mode) {
     A.a = plan.a; A.ng = plan.row_ng;
     const uint32_t R = 1u << plan.a, G = 1u << (n - plan.a);
     const int T = plan.row_ng * G / (1 << plan.row_loge);
-    const dim3 grid(R / plan.row_ng, 3, cap);
+    const dim3 grid(R / plan.row_ng, 3, zdim());
     if (mode == ROW_SQUARE) launch_row_mode<ROW_SQUARE>(*this, A, grid, T);
     else if (mode == ROW_MUL) launch_row_mode<ROW_MUL>(*this, A, grid, T);
     else launch_row_mode<ROW_FWD>(*this, A, grid, T);
@@ -374,8 +420,8 @@ void Batch::launch_row(const BArgs &A0, int mode) {
 
 void Batch::launch_norm_carry(const BArgs &A) {
     const unsigned nwg = (unsigned)(N >> 11);
-    hipLaunchKernelGGL(kb_norm, dim3(nwg, 1, cap), dim3(256), 0, stream, A);
-    hipLaunchKernelGGL(kb_carry_wg, dim3((nwg + 255) / 256, 1, cap), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(kb_norm, dim3(nwg, 1, zdim()), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(kb_carry_wg, dim3((nwg + 255) / 256, 1, zdim()), dim3(256), 0, stream, A);
 }
 
 void Batch::square(const uint8_t *active, const uint8_t *dup) {
@@ -390,13 +436,13 @@ void Batch::square(const uint8_t *active, const uint8_t *dup) {
 void Batch::op_copy(const uint8_t *active, uint32_t dst, uint32_t src) {
     BArgs A = args(active, nullptr);
     A.src_reg = src; A.dst_reg = dst;
-    hipLaunchKernelGGL(kb_copy, dim3((unsigned)((N / 4 + 255) / 256), 1, cap), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(kb_copy, dim3((unsigned)((N / 4 + 255) / 256), 1, zdim()), dim3(256), 0, stream, A);
 }
 
 void Batch::op_set(const uint8_t *active, uint32_t dst, int32_t value) {
     BArgs A = args(active, nullptr);
     A.dst_reg = dst; A.value = value;
-    hipLaunchKernelGGL(kb_set, dim3((unsigned)((N + 255) / 256), 1, cap), dim3(256), 0, stream, A);
+    hipLaunchKernelGGL(kb_set, dim3((unsigned)((N + 255) / 256), 1, zdim()), dim3(256), 0, stream, A);
 }
 
 void Batch::op_initmul(const uint8_t *active, uint32_t src) {
@@ -422,7 +468,24 @@ uint8_t *Batch::stage_acquire() {
 }
 void Batch::stage_release() {
     (void)hipEventRecord(stage_ev[stage_i], stream);
-    stage_i ^= 1;
+    stage_i = (stage_i + 1) % NSTAGE;
+}
+
+void Batch::update_zl() {
+    zl = 0;
+    for (uint32_t s = 0; s < cap; ++s)
+        if (slots[s].busy) zl = s + 1;
+}
+
+// Squaring blocks enqueued and not yet finished on the device.
+int Batch::backlog() {
+    while (!blk_q.empty()) {
+        const hipError_t e = hipEventQuery(blk_ev[blk_q.front()]);
+        if (e == hipErrorNotReady) { (void)hipGetLastError(); break; }
+        if (e != hipSuccess) return check_hip(e, "block event");
+        blk_q.pop_front();
+    }
+    return (int)blk_q.size();
 }
 
 
@@ -500,12 +563,12 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
     }
     mpz_clear(E);
     // slot table and CRT constants; r0 = 1, r1 = 1 (prp: set(1); copy(1, 0))
-    SlotInfo si;
+    SlotInfo &si = slot_host[s];
     si.b = b; si.rns = S.rns; si.is32 = S.is32; si.pad = 0;
-    const CrtConst c = make_crt(S.rns, S.is32, n);
+    crt_host[s] = make_crt(S.rns, S.is32, n);
+    zl = std::max(zl, (uint32_t)s + 1);
     int rc = check_hip(hipMemcpyAsync(slots_dev + s, &si, sizeof(si), hipMemcpyHostToDevice, stream), "slot");
-    if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(crt + s, &c, sizeof(c), hipMemcpyHostToDevice, stream), "crt");
-    if (rc == GFE_OK) rc = check_hip(hipStreamSynchronize(stream), "submit");
+    if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(crt + s, crt_host + s, sizeof(CrtConst), hipMemcpyHostToDevice, stream), "crt");
     if (rc != GFE_OK) return rc;
     std::vector<uint8_t> one(cap, 0);
     one[s] = 1;
@@ -530,8 +593,90 @@ int Batch::read_residue(int s, uint32_t reg, gint &g) {
     return GFE_OK;
 }
 
+// Enqueues the read-back of register 0 of slot s for result item `kind`.
+int Batch::read_issue(int s, int kind) {
+    if (rfree.empty()) {
+        const int rc = read_complete(true);
+        if (rc != GFE_OK) return rc;
+    }
+    const int k = rfree.back();
+    rfree.pop_back();
+    int rc = check_hip(hipMemcpyAsync(rbuf[k], regs + (size_t)s * 3 * N, N * 4, hipMemcpyDeviceToHost, stream), "readback");
+    if (rc == GFE_OK) rc = check_hip(hipEventRecord(rev[k], stream), "readback event");
+    if (rc != GFE_OK) return rc;
+    reads.push_back(Read{s, kind, k});
+    slots[s].reads_pending++;
+    return GFE_OK;
+}
+
+// Processes the oldest pending read-back: RES64 / old64 / hash64 (src/gint.h:169-219), or the
+// Gerbicz-Li hashes.  Returns 1 when there is none or (wait == false) it has not completed.
+int Batch::read_complete(bool wait) {
+    if (reads.empty()) return 1;
+    const Read r = reads.front();
+    if (wait) {
+        const int rc = check_hip(hipEventSynchronize(rev[r.buf]), "readback sync");
+        if (rc != GFE_OK) return rc;
+    } else {
+        const hipError_t e = hipEventQuery(rev[r.buf]);
+        if (e == hipErrorNotReady) { (void)hipGetLastError(); return 1; }
+        if (e != hipSuccess) return check_hip(e, "readback event");
+    }
+    reads.pop_front();
+    Slot &S = slots[r.slot];
+    gint gs(N, S.b);
+    memcpy(gs.data(), rbuf[r.buf], N * 4);
+    gs.reset();
+    rfree.push_back(r.buf);
+    if (r.kind == IT_READ_RES) {
+        uint64_t res = 0, old = 0;
+        S.is_prp = gs.isOne(res, old) ? 1 : 0;
+        S.res64 = res; S.old64 = old;
+        S.hash64 = S.is_prp ? 0 : gs.gethash64();
+    } else if (r.kind == IT_READ_H1) {
+        S.h1 = gs.gethash64();
+    } else {
+        S.gl_ok = (gs.gethash64() == S.h1) ? 1 : 0;
+    }
+    S.reads_pending--;
+    return GFE_OK;
+}
+
+// Completed read-backs are processed; slots whose program and read-backs are done move to
+// `finished` and become free.
+int Batch::harvest() {
+    for (;;) {
+        const int rc = read_complete(false);
+        if (rc == 1) break;
+        if (rc != GFE_OK) return rc;
+    }
+    bool freed = false;
+    for (uint32_t s = 0; s < cap; ++s) {
+        Slot &S = slots[s];
+        if (S.busy && !S.done && S.q.empty() && S.reads_pending == 0) {
+            S.done = true;
+            S.t_done = now_s() - t0;
+            gfe_batch_result r;
+            memset(&r, 0, sizeof(r));
+            r.tag = S.tag; r.b = S.b; r.n = n; r.res64 = S.res64; r.old64 = S.old64; r.hash64 = S.hash64;
+            r.esize = S.esize; r.iters = S.iters; r.is_prp = S.is_prp; r.gl_ok = S.gl_ok;
+            r.t_submit = S.t_submit; r.t_done = S.t_done;
+            finished.push_back(r);
+            S.busy = false;
+            S.ebits.clear(); S.ebits.shrink_to_fit();
+            S.resbits.clear(); S.resbits.shrink_to_fit();
+            freed = true;
+        }
+    }
+    if (freed) update_zl();
+    return GFE_OK;
+}
+
 int Batch::step(int32_t *active_out) {
     std::vector<uint8_t> flags(cap);
+    int rc0 = harvest();
+    if (rc0 != GFE_OK) return rc0;
+    bool issued = false;
     // 1. register operations, grouped by (kind, arguments) across slots
     for (;;) {
         std::map<std::tuple<int, uint32_t, uint32_t>, std::vector<int>> groups;
@@ -546,23 +691,12 @@ int Batch::step(int32_t *active_out) {
         for (auto &kv : groups) {
             const int kind = std::get<0>(kv.first);
             const uint32_t a = std::get<1>(kv.first), b2 = std::get<2>(kv.first);
+            issued = true;
             if (kind == IT_READ_RES || kind == IT_READ_H1 || kind == IT_READ_H2) {
                 for (int s : kv.second) {
-                    Slot &S = slots[s];
-                    gint gs(N, S.b);
-                    const int rc = read_residue(s, 0, gs);
+                    const int rc = read_issue(s, kind);
                     if (rc != GFE_OK) return rc;
-                    if (kind == IT_READ_RES) {
-                        uint64_t r = 0, o = 0;
-                        S.is_prp = gs.isOne(r, o) ? 1 : 0;
-                        S.res64 = r; S.old64 = o;
-                        S.hash64 = S.is_prp ? 0 : gs.gethash64();
-                    } else if (kind == IT_READ_H1) {
-                        S.h1 = gs.gethash64();
-                    } else {
-                        S.gl_ok = (gs.gethash64() == S.h1) ? 1 : 0;
-                    }
-                    S.q.pop_front();
+                    slots[s].q.pop_front();
                 }
                 continue;
             }
@@ -584,23 +718,7 @@ int Batch::step(int32_t *active_out) {
             for (int s : kv.second) slots[s].q.pop_front();
         }
     }
-    // 2. finished programs
-    for (uint32_t s = 0; s < cap; ++s) {
-        Slot &S = slots[s];
-        if (S.busy && !S.done && S.q.empty()) {
-            S.done = true;
-            S.t_done = now_s() - t0;
-            gfe_batch_result r;
-            memset(&r, 0, sizeof(r));
-            r.tag = S.tag; r.b = S.b; r.n = n; r.res64 = S.res64; r.old64 = S.old64; r.hash64 = S.hash64;
-            r.esize = S.esize; r.iters = S.iters; r.is_prp = S.is_prp; r.gl_ok = S.gl_ok;
-            r.t_submit = S.t_submit; r.t_done = S.t_done;
-            finished.push_back(r);
-            S.busy = false;
-            S.ebits.clear(); S.ebits.shrink_to_fit();
-        }
-    }
-    // 3. one block of up to 64 squaring rounds
+    // 2. one block of up to 64 squaring rounds
     int rounds = 0;
     std::vector<uint64_t> len(cap, 0);
     for (uint32_t s = 0; s < cap; ++s) {
@@ -633,6 +751,16 @@ int Batch::step(int32_t *active_out) {
         for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
         rc = check("square block");
         if (rc != GFE_OK) return rc;
+        if ((int)blk_q.size() == NBLK) {
+            rc = check_hip(hipEventSynchronize(blk_ev[blk_q.front()]), "block sync");
+            if (rc != GFE_OK) return rc;
+            blk_q.pop_front();
+        }
+        rc = check_hip(hipEventRecord(blk_ev[blk_next], stream), "block event");
+        if (rc != GFE_OK) return rc;
+        blk_q.push_back(blk_next);
+        blk_next = (blk_next + 1) % NBLK;
+        issued = true;
         for (uint32_t s = 0; s < cap; ++s) {
             if (!len[s]) continue;
             Slot &S = slots[s];
@@ -643,6 +771,14 @@ int Batch::step(int32_t *active_out) {
             }
         }
     }
+    // 3. nothing to enqueue but read-backs outstanding: wait for the oldest, so that repeated
+    // steps always make progress.
+    if (!issued && !reads.empty()) {
+        const int rc = read_complete(true);
+        if (rc != GFE_OK) return rc;
+    }
+    const int rc = harvest();
+    if (rc != GFE_OK) return rc;
     if (active_out) {
         int a = 0;
         for (auto &S : slots) a += S.busy ? 1 : 0;
@@ -714,11 +850,14 @@ int gfe_batch_step(gfe_batch *b, int32_t *active) {
 
 int gfe_batch_poll(gfe_batch *b, gfe_batch_result *out, uint32_t max, uint32_t *count) {
     if (!b || !count) return set_error(GFE_EINVAL, "poll");
+    if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    const int rc = b->bt.harvest();
+    if (rc != GFE_OK) return rc;
     uint32_t k = 0;
     auto &f = b->bt.finished;
     while (k < max && !f.empty() && out) {
         out[k++] = f.front();
-        f.erase(f.begin());
+        f.pop_front();
     }
     *count = k;
     return GFE_OK;
@@ -729,6 +868,17 @@ int gfe_batch_active(const gfe_batch *b) {
     int a = 0;
     for (auto &S : b->bt.slots) a += S.busy ? 1 : 0;
     return a;
+}
+
+int gfe_batch_backlog(gfe_batch *b) {
+    if (!b) return set_error(GFE_EINVAL, "null batch");
+    if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    return b->bt.backlog();
+}
+
+size_t gfe_batch_device_bytes(uint32_t n, uint32_t capacity) {
+    const size_t N = (size_t)1 << n;
+    return (size_t)capacity * (gfe_batch_slot_bytes(n) + 2 * 64) + 6 * N * 4;
 }
 
 int gfe_batch_get_digits(gfe_batch *b, int32_t slot, uint32_t reg, int32_t *out) {

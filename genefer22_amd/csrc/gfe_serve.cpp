@@ -1,0 +1,310 @@
+// gfe_serve.cpp -- the serving loop (include/gfe_serve.h): continuous batching of candidate
+// requests of mixed n over pools of gfe_batch, admission by device-memory footprint.
+//
+// Host code only: it drives the batch engine through its C ABI and owns no device memory itself
+// (hipMemGetInfo for the default budget is its only HIP call).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/gfe.h"
+#include "../../include/gfe_batch.h"
+#include "../../include/gfe_serve.h"
+
+namespace gfe {
+int set_error(int code, const std::string &msg);
+}
+
+namespace {
+
+double steady_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Default slots of a pool of n: about 2^26 digits per pool (enough candidates to fill the device
+// with one pool), between 16 and 1024 slots.
+uint32_t default_slots(uint32_t n) {
+    const uint64_t s = (n >= 26) ? 1 : (uint64_t(1) << (26 - n));
+    return (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(16, s));
+}
+
+struct Request {
+    uint64_t tag;
+    uint32_t n, b;
+    uint64_t iters;
+    double t_arrival;
+};
+
+struct Admitted {
+    Request rq;
+    double t_admit;
+};
+
+struct Pool {
+    gfe_batch *bt = nullptr;
+    uint32_t n = 0, cap = 0;
+    uint64_t bytes = 0;
+    uint32_t busy = 0;
+};
+
+}  // namespace
+
+struct gfe_server {
+    int device = 0;
+    double t0 = 0;
+    gfe_server_stats st;
+    std::deque<Request> queue;
+    std::vector<Pool> pools;
+    uint64_t seq = 0;                                  // the batch tag of each admitted request
+    std::unordered_map<uint64_t, Admitted> inflight;  // by batch tag
+    std::deque<gfe_serve_result> done;
+    std::vector<gfe_batch_result> buf;
+
+    double clock() const { return steady_s() - t0; }
+
+    void release(size_t i) {
+        gfe_batch_destroy(pools[i].bt);
+        st.bytes_reserved -= pools[i].bytes;
+        pools.erase(pools.begin() + (long)i);
+        st.pools_released++;
+        st.pools = (uint32_t)pools.size();
+    }
+
+    // Creates a pool of n with up to `want` slots within the budget, releasing idle pools of
+    // other n when needed.  *idx receives its index, or -1 when the budget has no room.
+    int create_pool(uint32_t n, uint32_t want, long *idx) {
+        *idx = -1;
+        auto fits = [&](uint32_t c) { return st.bytes_reserved + gfe_batch_device_bytes(n, c) <= st.bytes_budget; };
+        if (!fits(1)) {
+            for (size_t i = pools.size(); i-- > 0;)
+                if (pools[i].busy == 0 && pools[i].n != n) {
+                    release(i);
+                    if (fits(1)) break;
+                }
+        }
+        if (!fits(1)) return GFE_OK;
+        uint32_t cap = want;
+        while (cap > 1 && !fits(cap)) cap = std::max<uint32_t>(1, cap / 2);
+        Pool P;
+        P.n = n;
+        P.cap = cap;
+        P.bytes = gfe_batch_device_bytes(n, cap);
+        const int rc = gfe_batch_create(n, cap, device, &P.bt);
+        if (rc != GFE_OK) return rc;
+        pools.push_back(P);
+        st.bytes_reserved += P.bytes;
+        st.bytes_peak = std::max(st.bytes_peak, st.bytes_reserved);
+        st.pools_created++;
+        st.pools = (uint32_t)pools.size();
+        *idx = (long)pools.size() - 1;
+        return GFE_OK;
+    }
+
+    int admit() {
+        for (auto it = queue.begin(); it != queue.end();) {
+            const Request &rq = *it;
+            long pi = -1;
+            for (size_t i = 0; i < pools.size(); ++i)
+                if (pools[i].n == rq.n && pools[i].busy < pools[i].cap) { pi = (long)i; break; }
+            if (pi < 0) {
+                const int rc = create_pool(rq.n, default_slots(rq.n), &pi);
+                if (rc != GFE_OK) return rc;
+            }
+            if (pi < 0) { ++it; continue; }  // waits for memory
+            Pool &P = pools[(size_t)pi];
+            int32_t slot = -1;
+            const uint64_t id = seq++;
+            const int rc = gfe_batch_submit(P.bt, rq.b, rq.iters, id, &slot);
+            if (rc == GFE_EFULL) { ++it; continue; }
+            if (rc != GFE_OK) return rc;
+            inflight[id] = Admitted{rq, clock()};
+            P.busy++;
+            st.admitted++;
+            st.queued--;
+            it = queue.erase(it);
+        }
+        return GFE_OK;
+    }
+
+    int collect() {
+        for (Pool &P : pools) {
+            if (P.busy == 0) continue;
+            if (buf.size() < P.cap) buf.resize(P.cap);
+            uint32_t got = 0;
+            const int rc = gfe_batch_poll(P.bt, buf.data(), (uint32_t)buf.size(), &got);
+            if (rc != GFE_OK) return rc;
+            const double t = clock();
+            for (uint32_t k = 0; k < got; ++k) {
+                const gfe_batch_result &r = buf[k];
+                const auto f = inflight.find(r.tag);
+                if (f == inflight.end()) return gfe::set_error(GFE_EINVAL, "serve: result of an unknown request");
+                const Admitted a = f->second;
+                inflight.erase(f);
+                gfe_serve_result o;
+                memset(&o, 0, sizeof(o));
+                o.tag = a.rq.tag; o.n = r.n; o.b = r.b; o.res64 = r.res64; o.old64 = r.old64; o.hash64 = r.hash64;
+                o.esize = r.esize; o.iters = r.iters; o.is_prp = r.is_prp; o.gl_ok = r.gl_ok;
+                o.t_arrival = a.rq.t_arrival; o.t_admit = a.t_admit; o.t_done = t;
+                done.push_back(o);
+                P.busy--;
+                st.completed++;
+            }
+        }
+        return GFE_OK;
+    }
+
+    int pump(int32_t *busy_out) {
+        st.pumps++;
+        int rc = collect();
+        if (rc == GFE_OK) rc = admit();
+        if (rc != GFE_OK) return rc;
+        bool stepped = false;
+        for (Pool &P : pools) {
+            if (P.busy == 0) continue;
+            const int bl = gfe_batch_backlog(P.bt);
+            if (bl < 0) return bl;
+            if (bl >= 2) continue;
+            int32_t active = 0;
+            rc = gfe_batch_step(P.bt, &active);
+            if (rc != GFE_OK) return rc;
+            st.blocks++;
+            stepped = true;
+        }
+        rc = collect();
+        if (rc != GFE_OK) return rc;
+        uint32_t busy = 0;
+        for (const Pool &P : pools) busy += P.busy;
+        st.busy_slots = busy;
+        if (!stepped && busy > 0) std::this_thread::sleep_for(std::chrono::microseconds(50));
+        if (busy_out) *busy_out = (int32_t)busy;
+        return GFE_OK;
+    }
+};
+
+extern "C" {
+
+int gfe_server_create(int device, uint64_t hbm_budget, gfe_server **out) {
+    if (!out) return gfe::set_error(GFE_EINVAL, "null output");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return gfe::set_error(GFE_ENODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return gfe::set_error(GFE_ENODEV, "device index out of range");
+    if (hbm_budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+            return gfe::set_error(GFE_EHIP, "hipMemGetInfo");
+        hbm_budget = (uint64_t)free_b / 4 * 3;
+    }
+    gfe_server *s = new gfe_server;
+    s->device = device;
+    s->t0 = steady_s();
+    memset(&s->st, 0, sizeof(s->st));
+    s->st.bytes_budget = hbm_budget;
+    *out = s;
+    return GFE_OK;
+}
+
+int gfe_server_destroy(gfe_server *s) {
+    if (!s) return GFE_OK;
+    for (Pool &P : s->pools) gfe_batch_destroy(P.bt);
+    delete s;
+    return GFE_OK;
+}
+
+double gfe_server_clock(const gfe_server *s) { return s ? s->clock() : 0.0; }
+
+int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity) {
+    if (!s) return gfe::set_error(GFE_EINVAL, "null server");
+    if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
+    long pi = -1;
+    const int rc = s->create_pool(n, capacity ? capacity : default_slots(n), &pi);
+    if (rc != GFE_OK) return rc;
+    if (pi < 0) return gfe::set_error(GFE_ENOMEM, "serve: pool does not fit in the budget");
+    return GFE_OK;
+}
+
+int gfe_server_submit(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint64_t tag) {
+    if (!s) return gfe::set_error(GFE_EINVAL, "null server");
+    if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
+    if (b < 1024 || b > 2000000000u) return gfe::set_error(GFE_EINVAL, "b must be in [1024, 2000000000]");
+    if (gfe_batch_device_bytes(n, 1) > s->st.bytes_budget)
+        return gfe::set_error(GFE_ENOMEM, "serve: one candidate of this n exceeds the budget");
+    s->queue.push_back(Request{tag, n, b, iters, s->clock()});
+    s->st.submitted++;
+    s->st.queued++;
+    return GFE_OK;
+}
+
+int gfe_server_pump(gfe_server *s, int32_t *busy) {
+    if (!s) return gfe::set_error(GFE_EINVAL, "null server");
+    return s->pump(busy);
+}
+
+int gfe_server_poll(gfe_server *s, gfe_serve_result *out, uint32_t max, uint32_t *count) {
+    if (!s || !count) return gfe::set_error(GFE_EINVAL, "poll");
+    const int rc = s->collect();
+    if (rc != GFE_OK) return rc;
+    uint32_t k = 0;
+    while (k < max && out && !s->done.empty()) {
+        out[k++] = s->done.front();
+        s->done.pop_front();
+    }
+    *count = k;
+    return GFE_OK;
+}
+
+int gfe_server_get_stats(const gfe_server *s, gfe_server_stats *out) {
+    if (!s || !out) return gfe::set_error(GFE_EINVAL, "stats");
+    *out = s->st;
+    return GFE_OK;
+}
+
+int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters, const double *arrival,
+                     uint32_t count, int device, uint64_t hbm_budget, const uint32_t *prepare_n,
+                     uint32_t prepare_count, gfe_serve_result *results, gfe_server_stats *stats) {
+    if (!n || !b || !iters || !arrival || !results || count == 0) return gfe::set_error(GFE_EINVAL, "replay arguments");
+    for (uint32_t k = 1; k < count; ++k)
+        if (arrival[k] < arrival[k - 1]) return gfe::set_error(GFE_EINVAL, "replay: arrivals must be non-decreasing");
+    gfe_server *s = nullptr;
+    int rc = gfe_server_create(device, hbm_budget, &s);
+    if (rc != GFE_OK) return rc;
+    for (uint32_t i = 0; i < prepare_count && rc == GFE_OK; ++i) rc = gfe_server_prepare(s, prepare_n[i], 0);
+    const double start = s->clock();
+    uint32_t next = 0, finished = 0;
+    std::vector<gfe_serve_result> got(256);
+    while (rc == GFE_OK && finished < count) {
+        const double now = s->clock() - start;
+        while (next < count && arrival[next] <= now && rc == GFE_OK) {
+            rc = gfe_server_submit(s, n[next], b[next], iters[next], next);
+            ++next;
+        }
+        if (rc != GFE_OK) break;
+        int32_t busy = 0;
+        rc = gfe_server_pump(s, &busy);
+        if (rc != GFE_OK) break;
+        uint32_t c = 0;
+        rc = gfe_server_poll(s, got.data(), (uint32_t)got.size(), &c);
+        for (uint32_t k = 0; k < c; ++k) {
+            gfe_serve_result r = got[k];
+            r.t_arrival -= start; r.t_admit -= start; r.t_done -= start;
+            results[r.tag] = r;
+        }
+        finished += c;
+        if (busy == 0 && s->queue.empty() && next < count) {
+            const double wait = arrival[next] - (s->clock() - start);
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.01)));
+        }
+    }
+    if (stats) *stats = s->st;
+    gfe_server_destroy(s);
+    return rc;
+}
+
+}  // extern "C"

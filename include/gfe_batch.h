@@ -61,6 +61,10 @@ int gfe_batch_step(gfe_batch *bt, int32_t *active);
 int gfe_batch_poll(gfe_batch *bt, gfe_batch_result *out, uint32_t max, uint32_t *count);
 /* Number of busy slots. */
 int gfe_batch_active(const gfe_batch *bt);
+/* Squaring blocks enqueued on the device and not finished yet (>= 0), or a negative error code. */
+int gfe_batch_backlog(gfe_batch *bt);
+/* Device bytes a batch of `capacity` slots allocates (slots, shared twiddles, flags). */
+size_t gfe_batch_device_bytes(uint32_t n, uint32_t capacity);
 /* Canonical digits of register `reg` of a busy slot (for checks; waits for the queued work). */
 int gfe_batch_get_digits(gfe_batch *bt, int32_t slot, uint32_t reg, int32_t *out);
 

@@ -1,0 +1,69 @@
+"""Serving loop (include/gfe_serve.h): results in the mix equal results alone, bit-exact with the
+reference's fixtures; admission by device-memory footprint."""
+import json
+import os
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_serve_replay_matches_reference_fixtures(gfe):
+    """The first 48 seeded serving requests (n in 12..20, Poisson arrivals at 100/s) replayed on the
+    wall clock: every RES64/old64 equals the reference CPU transform's (tests/golden/ref_c5.json)."""
+    from genefer22_amd import serve, workloads
+    path = os.path.join(HERE, "golden", "ref_c5.json")
+    if not os.path.exists(path):
+        pytest.skip("ref_c5.json not generated")
+    ref = json.load(open(path))["results"]
+    reqs = workloads.serving_requests(48, 100.0)
+    assert [(q.n, q.b) for q in reqs] == [(r["n"], r["b"]) for r in ref]
+    results, summary, stats = serve.run(reqs)
+    for q, r, g in zip(reqs, ref, results):
+        assert (g["n"], g["b"], g["iters"]) == (q.n, q.b, q.iters)
+        assert (f"{g['res64']:016X}", f"{g['old64']:016X}") == (r["res64"], r["old64"]), (q.rid, q.n, q.b)
+        assert g["t_arrival"] <= g["t_admit"] <= g["t_done"]
+    assert stats["completed"] == 48 and stats["queued"] == 0
+    assert summary["p50_s"] <= summary["p99_s"]
+
+
+def test_serve_alone_equals_mix_under_a_small_budget(gfe):
+    """With a budget that holds one small pool at a time, requests wait for memory, idle pools are
+    released for other n, and every result equals the candidate's run alone."""
+    mix = [(12, 1000000000), (13, 1534), (12, 30000002), (14, 1999999266), (13, 777777778), (12, 4098),
+           (14, 100000002), (12, 1999999990)]
+    iters = 700
+    budget = gfe.lib().gfe_batch_device_bytes(14, 2) + 4096
+    reqs = [(n, b, iters, 0.002 * k) for k, (n, b) in enumerate(mix)]
+    results, stats = gfe.serve_replay(reqs, hbm_budget=budget)
+    assert stats["bytes_peak"] <= budget
+    assert stats["pools_released"] >= 2
+    for (n, b), g in zip(mix, results):
+        alone = gfe.run_prp(b, n, iters=iters, b_gl=-1)
+        assert (g["res64"], g["old64"], g["hash64"]) == (alone["res64"], alone["old64"], alone["hash64"]), (n, b)
+
+
+def test_server_api_full_quick_test(gfe):
+    """submit / pump / poll: a full quick test (PRP + Gerbicz-Li) through the server equals the
+    reference's known answer (src/main.cpp:424) and the plain driver's run."""
+    s = gfe.Server()
+    s.submit(12, 1000000000, 0, tag=7)
+    s.submit(13, 2000000000, 500, tag=8)
+    got = []
+    for _ in range(200000):
+        s.pump()
+        got += s.poll()
+        if len(got) == 2:
+            break
+    assert sorted(r["tag"] for r in got) == [7, 8]
+    r7 = next(r for r in got if r["tag"] == 7)
+    single = gfe.run_quick(1000000000, 12)
+    assert (r7["res64"], r7["old64"]) == (single["res64"], single["old64"])
+    assert r7["gl_ok"] == 1
+    r8 = next(r for r in got if r["tag"] == 8)
+    assert r8["res64"] == gfe.run_prp(2000000000, 13, iters=500, b_gl=-1)["res64"]
+    st = s.stats()
+    assert st["completed"] == 2 and st["pools"] >= 1
+    s.close()
