@@ -99,6 +99,10 @@ ABI = [
     ("gfe_batch_time_rounds", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     ("gfe_batch_run", ctypes.c_int, [ctypes.c_uint32, U32P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]),
     ("gfe_batch_backlog", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_get_plan", ctypes.c_int, [ctypes.c_void_p, I32P]),
+    ("gfe_set_plan", ctypes.c_int, [ctypes.c_void_p, I32P]),
+    ("gfe_batch_get_plan", ctypes.c_int, [ctypes.c_void_p, I32P]),
+    ("gfe_batch_set_plan", ctypes.c_int, [ctypes.c_void_p, I32P]),
     ("gfe_batch_device_bytes", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32]),
     ("gfe_server_create", ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     ("gfe_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
@@ -239,6 +243,16 @@ class Transform:
         _ck(lib().gfe_tune(self._h, reps))
         return self.info()
 
+    def get_plan(self) -> tuple:
+        """(a, column E log2, columns/workgroup, row E log2, rows/workgroup, factored row twiddles)."""
+        a = np.zeros(6, dtype=np.int32)
+        _ck(lib().gfe_get_plan(self._h, _i32(a)))
+        return tuple(int(v) for v in a)
+
+    def set_plan(self, plan) -> None:
+        a = np.ascontiguousarray(plan, dtype=np.int32)
+        _ck(lib().gfe_set_plan(self._h, _i32(a)))
+
     def residue(self, reg: int = 0) -> dict:
         return residue(self.get_digits(reg), self.n, self.b)
 
@@ -325,6 +339,15 @@ class Batch:
     def tune(self, reps: int = 4) -> str:
         _ck(lib().gfe_batch_tune(self._h, reps))
         return self.info()
+
+    def get_plan(self) -> tuple:
+        a = np.zeros(6, dtype=np.int32)
+        _ck(lib().gfe_batch_get_plan(self._h, _i32(a)))
+        return tuple(int(v) for v in a)
+
+    def set_plan(self, plan) -> None:
+        a = np.ascontiguousarray(plan, dtype=np.int32)
+        _ck(lib().gfe_batch_set_plan(self._h, _i32(a)))
 
     def submit(self, b: int, iters: int = 0, tag: int = 0) -> int | None:
         """Slot of the new candidate, or None when the batch is full."""

@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(1024) kb_col_inv(BArgs A) {
     else col_inv_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
 }
 
-template <int LOGE, int MODE>
+template <int LOGE, int MODE, int TWF>
 __global__ void __launch_bounds__(1024) kb_row(BArgs A) {
     extern __shared__ uint32_t lds[];
     const int s = bslot(A);
@@ -102,8 +102,8 @@ __global__ void __launch_bounds__(1024) kb_row(BArgs A) {
     uint32_t *data = ((MODE == ROW_FWD) ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
     const uint32_t *zp = A.zp + ((size_t)s * 3 + prime) * N;
     const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
-    if (si.is32) row_body<ModU, LOGE, MODE>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
-    else row_body<ModS, LOGE, MODE>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    if (si.is32) row_body<ModU, LOGE, MODE, TWF>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    else row_body<ModS, LOGE, MODE, TWF>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
 }
 
 __global__ void __launch_bounds__(256) kb_norm(BArgs A) {
@@ -301,6 +301,15 @@ Batch::~Batch() {
 void make_twiddles_set(int is32, int k, uint32_t n, uint32_t *out);  // gfe_engine.hip
 void set_lds_attr(const void *fn);
 
+template <int TWF> void set_row_attrs() {
+    set_lds_attr((const void *)kb_row<3, ROW_SQUARE, TWF>); set_lds_attr((const void *)kb_row<4, ROW_SQUARE, TWF>);
+    set_lds_attr((const void *)kb_row<5, ROW_SQUARE, TWF>);
+    set_lds_attr((const void *)kb_row<3, ROW_MUL, TWF>); set_lds_attr((const void *)kb_row<4, ROW_MUL, TWF>);
+    set_lds_attr((const void *)kb_row<5, ROW_MUL, TWF>);
+    set_lds_attr((const void *)kb_row<3, ROW_FWD, TWF>); set_lds_attr((const void *)kb_row<4, ROW_FWD, TWF>);
+    set_lds_attr((const void *)kb_row<5, ROW_FWD, TWF>);
+}
+
 int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     n = n_; cap = cap_; device = device_;
     N = (size_t)1 << n;
@@ -356,13 +365,13 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
         return set_error(GFE_EHIP, "memset");
     set_lds_attr((const void *)kb_col_fwd<3>); set_lds_attr((const void *)kb_col_fwd<4>); set_lds_attr((const void *)kb_col_fwd<5>);
     set_lds_attr((const void *)kb_col_inv<3>); set_lds_attr((const void *)kb_col_inv<4>); set_lds_attr((const void *)kb_col_inv<5>);
-    set_lds_attr((const void *)kb_row<3, ROW_SQUARE>); set_lds_attr((const void *)kb_row<4, ROW_SQUARE>); set_lds_attr((const void *)kb_row<5, ROW_SQUARE>);
-    set_lds_attr((const void *)kb_row<3, ROW_MUL>); set_lds_attr((const void *)kb_row<4, ROW_MUL>); set_lds_attr((const void *)kb_row<5, ROW_MUL>);
-    set_lds_attr((const void *)kb_row<3, ROW_FWD>); set_lds_attr((const void *)kb_row<4, ROW_FWD>); set_lds_attr((const void *)kb_row<5, ROW_FWD>);
+    set_row_attrs<0>();
+    set_row_attrs<1>();
     slots.resize(cap);
     t0 = now_s();
     return check_hip(hipStreamSynchronize(stream), "init");
 }
+
 
 BArgs Batch::args(const uint8_t *active, const uint8_t *dup) const {
     BArgs A;
@@ -398,13 +407,18 @@ void Batch::launch_col_inv(const BArgs &A0) {
     }
 }
 
+template <int MODE, int TWF>
+static void launch_row_mode_t(const Batch &B, const BArgs &A, dim3 grid, int T) {
+    switch (B.plan.row_loge) {
+        case 3: hipLaunchKernelGGL((kb_row<3, MODE, TWF>), grid, dim3(T), B.lds_row(), B.stream, A); break;
+        case 4: hipLaunchKernelGGL((kb_row<4, MODE, TWF>), grid, dim3(T), B.lds_row(), B.stream, A); break;
+        default: hipLaunchKernelGGL((kb_row<5, MODE, TWF>), grid, dim3(T), B.lds_row(), B.stream, A); break;
+    }
+}
 template <int MODE>
 static void launch_row_mode(const Batch &B, const BArgs &A, dim3 grid, int T) {
-    switch (B.plan.row_loge) {
-        case 3: hipLaunchKernelGGL((kb_row<3, MODE>), grid, dim3(T), B.lds_row(), B.stream, A); break;
-        case 4: hipLaunchKernelGGL((kb_row<4, MODE>), grid, dim3(T), B.lds_row(), B.stream, A); break;
-        default: hipLaunchKernelGGL((kb_row<5, MODE>), grid, dim3(T), B.lds_row(), B.stream, A); break;
-    }
+    if (B.plan.row_tw) launch_row_mode_t<MODE, 1>(B, A, grid, T);
+    else launch_row_mode_t<MODE, 0>(B, A, grid, T);
 }
 
 void Batch::launch_row(const BArgs &A0, int mode) {
@@ -870,6 +884,22 @@ int gfe_batch_active(const gfe_batch *b) {
     return a;
 }
 
+int gfe_batch_get_plan(const gfe_batch *b, int32_t plan[6]) {
+    if (!b || !plan) return set_error(GFE_EINVAL, "get_plan");
+    gfe::plan_to_array(b->bt.plan, plan);
+    return GFE_OK;
+}
+
+int gfe_batch_set_plan(gfe_batch *b, const int32_t plan[6]) {
+    if (!b || !plan) return set_error(GFE_EINVAL, "set_plan");
+    gfe::Plan P;
+    if (!gfe::plan_from_array(plan, b->bt.n, &P)) return set_error(GFE_EINVAL, "plan not admissible for this n");
+    if (hipSetDevice(b->bt.device) != hipSuccess || hipStreamSynchronize(b->bt.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "set_plan sync");
+    b->bt.plan = P;
+    return GFE_OK;
+}
+
 int gfe_batch_backlog(gfe_batch *b) {
     if (!b) return set_error(GFE_EINVAL, "null batch");
     if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
@@ -932,6 +962,8 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
                         const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (B.n - a)) / (1 << rl);
                         if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 512) continue;
                         if (!gfe::plan_valid(P, B.n)) continue;
+                        plans.push_back(P);
+                        P.row_tw = 1;
                         plans.push_back(P);
                     }
     // squarings on the busy slots' r0: save and restore register 0 of every slot

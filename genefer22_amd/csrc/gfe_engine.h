@@ -16,11 +16,14 @@ namespace gfe {
 // 2^row_loge elements per thread); norm_s digits per thread in the normalisation.
 struct Plan {
     int a = 0, col_loge = 4, col_ng = 16, row_loge = 4, row_ng = 1, norm_s = 8;
+    int row_tw = 0;  // 1: row twiddles formed from per-row factors (row_body TWF)
 };
 
 Plan default_plan(uint32_t n);
 bool plan_valid(const Plan &P, uint32_t n);
 std::string plan_str(const Plan &P, uint32_t n);
+void plan_to_array(const Plan &P, int32_t out[6]);
+bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out);
 void select_primes(uint32_t b, uint32_t n, int *rns, int *is32);
 uint32_t prime_of(int is32, int k);
 uint32_t mulw_const(uint32_t w, uint32_t p);

@@ -113,7 +113,7 @@ static void make_twiddles(int is32, int k, uint32_t n, uint32_t *out) {
     std::vector<uint32_t> pw(N);
     pw[0] = 1;
     for (size_t e = 1; e < N; ++e) pw[e] = hmul(pw[e - 1], psi, p);
-    out[0] = 0;
+    out[0] = mulw_const(1, p);  // Psi[0] = 1: the j = 0 factor of factored row twiddles
     for (size_t j = 1; j < N; ++j) {
         size_t r = 0;
         for (uint32_t bit = 0; bit < n; ++bit) r |= ((j >> bit) & 1) << (n - 1 - bit);
@@ -181,10 +181,27 @@ bool plan_valid(const Plan &P, uint32_t n) {
     return true;
 }
 
+void plan_to_array(const Plan &P, int32_t out[6]) {
+    out[0] = P.a; out[1] = P.col_loge; out[2] = P.col_ng; out[3] = P.row_loge; out[4] = P.row_ng; out[5] = P.row_tw;
+}
+
+bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out) {
+    Plan P;
+    P.a = in[0]; P.col_loge = in[1]; P.col_ng = in[2]; P.row_loge = in[3]; P.row_ng = in[4]; P.row_tw = in[5];
+    P.norm_s = 8;
+    auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
+    if (P.a < 1 || P.a >= (int)n || P.col_loge < 3 || P.col_loge > 5 || P.row_loge < 3 || P.row_loge > 5) return false;
+    if (!pow2(P.col_ng) || !pow2(P.row_ng) || (P.row_tw != 0 && P.row_tw != 1)) return false;
+    if (!plan_valid(P, n)) return false;
+    *out = P;
+    return true;
+}
+
 std::string plan_str(const Plan &P, uint32_t n) {
     std::ostringstream ss;
     ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (n - P.a)
-       << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg), norm " << P.norm_s << " digits/thread";
+       << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg" << (P.row_tw ? ", factored twiddles" : "")
+       << "), norm " << P.norm_s << " digits/thread";
     return ss.str();
 }
 
@@ -209,12 +226,15 @@ template <class M> static ColFn col_inv_fn(int loge) {
         default: return k_col_inv<M, 5>;
     }
 }
-template <class M, int MODE> static RowFn row_fn(int loge) {
+template <class M, int MODE, int TWF> static RowFn row_fn_t(int loge) {
     switch (loge) {
-        case 3: return k_row<M, 3, MODE>;
-        case 4: return k_row<M, 4, MODE>;
-        default: return k_row<M, 5, MODE>;
+        case 3: return k_row<M, 3, MODE, TWF>;
+        case 4: return k_row<M, 4, MODE, TWF>;
+        default: return k_row<M, 5, MODE, TWF>;
     }
+}
+template <class M, int MODE> static RowFn row_fn(int loge, int twf) {
+    return twf ? row_fn_t<M, MODE, 1>(loge) : row_fn_t<M, MODE, 0>(loge);
 }
 
 struct KernelSet {
@@ -228,16 +248,16 @@ static KernelSet kernels_for(int rns, int is32, const Plan &P) {
     if (is32) {
         K.col_fwd = col_fwd_fn<ModU>(P.col_loge);
         K.col_inv = col_inv_fn<ModU>(P.col_loge);
-        K.row_sq = row_fn<ModU, ROW_SQUARE>(P.row_loge);
-        K.row_mul = row_fn<ModU, ROW_MUL>(P.row_loge);
-        K.row_fwd = row_fn<ModU, ROW_FWD>(P.row_loge);
+        K.row_sq = row_fn<ModU, ROW_SQUARE>(P.row_loge, P.row_tw);
+        K.row_mul = row_fn<ModU, ROW_MUL>(P.row_loge, P.row_tw);
+        K.row_fwd = row_fn<ModU, ROW_FWD>(P.row_loge, P.row_tw);
         K.norm = (rns == 2) ? k_norm<2, ModU, 8> : k_norm<3, ModU, 8>;
     } else {
         K.col_fwd = col_fwd_fn<ModS>(P.col_loge);
         K.col_inv = col_inv_fn<ModS>(P.col_loge);
-        K.row_sq = row_fn<ModS, ROW_SQUARE>(P.row_loge);
-        K.row_mul = row_fn<ModS, ROW_MUL>(P.row_loge);
-        K.row_fwd = row_fn<ModS, ROW_FWD>(P.row_loge);
+        K.row_sq = row_fn<ModS, ROW_SQUARE>(P.row_loge, P.row_tw);
+        K.row_mul = row_fn<ModS, ROW_MUL>(P.row_loge, P.row_tw);
+        K.row_fwd = row_fn<ModS, ROW_FWD>(P.row_loge, P.row_tw);
         K.norm = (rns == 2) ? k_norm<2, ModS, 8> : k_norm<3, ModS, 8>;
     }
     return K;
@@ -250,10 +270,11 @@ static void set_lds_attributes() {
     std::call_once(g_attr_once, []() {
         const int lds = 160 * 1024;
         const int loges[3] = {3, 4, 5};
-        for (int L : loges) {
+        for (int L : loges) for (int twf = 0; twf < 2; ++twf) {
             Plan P;
             P.col_loge = L;
             P.row_loge = L;
+            P.row_tw = twf;
             for (int is32 = 0; is32 < 2; ++is32) {
                 KernelSet K = kernels_for(3, is32, P);
                 (void)hipFuncSetAttribute((const void *)K.col_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -679,6 +700,8 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
                         if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 512) continue;
                         if (!gfe::plan_valid(P, g.n)) continue;
                         plans.push_back(P);
+                        P.row_tw = 1;
+                        plans.push_back(P);
                     }
     const gfe::Plan orig = g.plan;
     gfe::Plan best = orig;
@@ -705,6 +728,22 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
         hipStreamSynchronize(g.stream) != hipSuccess)
         return set_error(GFE_EHIP, "tune restore");
     return g.check("tune");
+}
+
+int gfe_get_plan(const gfe_engine *e, int32_t plan[6]) {
+    if (!e || !plan) return set_error(GFE_EINVAL, "get_plan");
+    gfe::plan_to_array(e->eng.plan, plan);
+    return GFE_OK;
+}
+
+int gfe_set_plan(gfe_engine *e, const int32_t plan[6]) {
+    GFE_GUARD(e);
+    if (!plan) return set_error(GFE_EINVAL, "set_plan");
+    gfe::Plan P;
+    if (!gfe::plan_from_array(plan, e->eng.n, &P)) return set_error(GFE_EINVAL, "plan not admissible for this n");
+    if (hipStreamSynchronize(e->eng.stream) != hipSuccess) return set_error(GFE_EHIP, "set_plan sync");
+    e->eng.plan = P;
+    return GFE_OK;
 }
 
 }  // extern "C"

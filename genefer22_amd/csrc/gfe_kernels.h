@@ -35,11 +35,10 @@ struct PrimeArgs {
     uint32_t q[MAX_RNS];
 };
 
-// Forward CT levels mu = 0..ne-1 of a 2^LOGE-point sub-transform held in x[].  The twiddle of
-// level mu, register block bi is tw[(t0 << mu) + bi].
-template <class M, int LOGE>
-__device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
-                                           uint32_t t0, int ne) {
+// Forward CT levels mu = 0..ne-1 of a 2^LOGE-point sub-transform held in x[].  twf(mu, bi) is
+// the twiddle (Montgomery form) of level mu, register block bi.
+template <class M, int LOGE, class TWF>
+__device__ __forceinline__ void fwd_levels_t(const M &md, uint32_t (&x)[1 << LOGE], const TWF &twf, int ne) {
     constexpr int E = 1 << LOGE;
 #pragma unroll
     for (int mu = 0; mu < LOGE; ++mu) {
@@ -47,7 +46,7 @@ __device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE]
             const int h = E >> (mu + 1);
 #pragma unroll
             for (int bi = 0; bi < (1 << mu); ++bi) {
-                const uint32_t w = tw[(t0 << mu) + bi];
+                const uint32_t w = twf(mu, bi);
 #pragma unroll
                 for (int i = 0; i < h; ++i) {
                     const int i0 = bi * 2 * h + i, i1 = i0 + h;
@@ -60,20 +59,26 @@ __device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE]
     }
 }
 
-// Inverse GS levels mu = ne-1..0 (the exact inverse of fwd_levels without the 1/2 factors).
-// lvl0 = global level of mu = 0.
+// Table twiddles: level mu, block bi of a sub-transform whose first-level block index is t0
+// uses Psi[(t0 << mu) + bi].
 template <class M, int LOGE>
-__device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
-                                           uint32_t t0, int lvl0, int ne) {
+__device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
+                                           uint32_t t0, int ne) {
+    fwd_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return tw[(t0 << mu) + bi]; }, ne);
+}
+
+// Inverse GS levels mu = ne-1..0 (the exact inverse of fwd_levels_t without the 1/2 factors);
+// twf(mu, bi) is the inverse twiddle of level mu, block bi.
+template <class M, int LOGE, class TWF>
+__device__ __forceinline__ void inv_levels_t(const M &md, uint32_t (&x)[1 << LOGE], const TWF &twf, int ne) {
     constexpr int E = 1 << LOGE;
 #pragma unroll
     for (int mu = LOGE - 1; mu >= 0; --mu) {
         if (mu < ne) {
             const int h = E >> (mu + 1);
-            const uint32_t m3 = 3u << (lvl0 + mu);
 #pragma unroll
             for (int bi = 0; bi < (1 << mu); ++bi) {
-                const uint32_t w = tw[m3 - 1u - ((t0 << mu) + bi)];
+                const uint32_t w = twf(mu, bi);
 #pragma unroll
                 for (int i = 0; i < h; ++i) {
                     const int i0 = bi * 2 * h + i, i1 = i0 + h;
@@ -84,6 +89,13 @@ __device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE]
             }
         }
     }
+}
+
+// Table twiddles of the inverse: Psi[m + j]^-1 = -Psi[2m - 1 - j] (lvl0 = global level of mu = 0).
+template <class M, int LOGE>
+__device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
+                                           uint32_t t0, int lvl0, int ne) {
+    inv_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return tw[(3u << (lvl0 + mu)) - 1u - ((t0 << mu) + bi)]; }, ne);
 }
 
 // Chunk schedule of a g-level pass: a partial first chunk of r = g mod LOGE levels (if r > 0),
@@ -202,8 +214,15 @@ __device__ __forceinline__ void col_inv_body(const M &md, uint32_t *__restrict__
 
 enum RowMode { ROW_SQUARE = 0, ROW_MUL = 1, ROW_FWD = 2 };
 
+// Row twiddles.  The bit fields of m = 2^l + row 2^(l-a) + j (j < 2^(l-a)) are disjoint, so
+// bitrev_n(m) is the sum of their bit-reversals and Psi[m] = Psi[2^l + row 2^(l-a)] Psi[j]
+// (Psi[0] = 1).  With TWF = 1 a row's twiddles are formed from one per-row factor per level and
+// the table's first 2^(n-a-1) entries, which every row shares (cache-resident), instead of
+// reading a distinct 2^(n-a)-entry slice of the table per row.  The inverse's index
+// 3 2^l - 1 - m = 2^l + (R-1-row) 2^(l-a) + (2^(l-a) - 1 - j) factors the same way.
+//
 // `data` and `zp` point at the prime's N residues.
-template <class M, int LOGE, int MODE>
+template <class M, int LOGE, int MODE, int TWF>
 __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ zp,
                                          const uint32_t *__restrict__ tw, int n, int a, int NG, unsigned blk,
                                          uint32_t *lds) {
@@ -234,7 +253,14 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
                 x[rr] = L[k + (k >> LOGE)];
             }
         }
-        fwd_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), ne);
+        if (TWF) {
+            uint32_t rb[LOGE];
+#pragma unroll
+            for (int mu = 0; mu < LOGE; ++mu) rb[mu] = (mu < ne) ? tw[(1u << (a + lam0 + mu)) + (row << (lam0 + mu))] : 0u;
+            fwd_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return md.mont(rb[mu], tw[((uint32_t)hi << mu) + bi]); }, ne);
+        } else {
+            fwd_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), ne);
+        }
         if (ci < ch.nch - 1) {
             __syncthreads();
 #pragma unroll
@@ -271,7 +297,17 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
                 x[rr] = L[k + (k >> LOGE)];
             }
         }
-        inv_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), a + lam0, ne);
+        if (TWF) {
+            const uint32_t rrow = (1u << a) - 1u - row;
+            uint32_t rb[LOGE];
+#pragma unroll
+            for (int mu = 0; mu < LOGE; ++mu) rb[mu] = (mu < ne) ? tw[(1u << (a + lam0 + mu)) + (rrow << (lam0 + mu))] : 0u;
+            inv_levels_t<M, LOGE>(md, x, [&](int mu, int bi) {
+                return md.mont(rb[mu], tw[(1u << (lam0 + mu)) - 1u - (((uint32_t)hi << mu) + bi)]);
+            }, ne);
+        } else {
+            inv_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), a + lam0, ne);
+        }
         if (ci == 0) {
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) dst[base + (rr << ylog)] = x[rr];
@@ -430,13 +466,13 @@ struct RowArgs {
     int n, a, ng;
 };
 
-template <class M, int LOGE, int MODE>
+template <class M, int LOGE, int MODE, int TWF>
 __global__ void __launch_bounds__(1024) k_row(RowArgs A) {
     extern __shared__ uint32_t lds[];
     const int prime = blockIdx.y;
     const M md{A.pr.p[prime], A.pr.q[prime]};
     const size_t N = (size_t)1 << A.n;
-    row_body<M, LOGE, MODE>(md, A.ws + prime * N, A.zp + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
+    row_body<M, LOGE, MODE, TWF>(md, A.ws + prime * N, A.zp + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
 }
 
 struct NormArgs {

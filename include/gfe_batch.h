@@ -49,6 +49,9 @@ size_t gfe_batch_slot_bytes(uint32_t n);
  * slots (or of synthetic full slots when empty); the reference's tune, src/transformGPU.h:851-900. */
 int gfe_batch_tune(gfe_batch *bt, uint32_t reps);
 int gfe_batch_info(const gfe_batch *bt, char *buf, size_t len);
+/* The batch's pass plan, as gfe_get_plan / gfe_set_plan of gfe.h. */
+int gfe_batch_get_plan(const gfe_batch *bt, int32_t plan[6]);
+int gfe_batch_set_plan(gfe_batch *bt, const int32_t plan[6]);
 
 /* Submits the quick test of b^(2^n) + 1 (iters == 0: the whole PRP loop, then the Gerbicz-Li
  * check) or the first `iters` squarings of its PRP loop (no check).  *slot receives the slot; returns

@@ -62,3 +62,21 @@ def test_batch_api_submit_step_poll(gfe):
         single = gfe.run_prp(r["b"], 12, iters=100, b_gl=-1)
         assert single["res64"] == r["res64"]
     bt.close()
+
+
+@pytest.mark.parametrize("plan", [(7, 4, 8, 4, 1, 1), (6, 3, 16, 5, 2, 0), (4, 4, 32, 3, 4, 1)])
+def test_batch_plan_variants(gfe, plan):
+    """Batched squarings under fixed plans (incl. factored row twiddles) equal single runs."""
+    bases = [1000000000, 1534, 30000000, 1999999266]
+    bt = gfe.Batch(13, len(bases))
+    bt.set_plan(plan)
+    assert bt.get_plan() == plan
+    for i, b in enumerate(bases):
+        bt.submit(b, iters=200, tag=i)
+    done = []
+    while len(done) < len(bases):
+        bt.step()
+        done += bt.poll()
+    bt.close()
+    for r in done:
+        assert r["res64"] == gfe.run_prp(r["b"], 13, iters=200, b_gl=-1)["res64"], (plan, r["b"])

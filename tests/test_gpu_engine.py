@@ -84,3 +84,57 @@ def test_known_answer_quick_n12(gfe):
         r = gfe.run_quick(b, 12)
         assert r["res64"] == want, hex(r["res64"])
         assert r["gl_ok"] == 1
+
+
+def _plans(n):
+    for a in sorted({4, n // 2, n - 4}):
+        for cl in (3, 4, 5):
+            for rl in (3, 4, 5):
+                for cng in (1, 8, 32):
+                    for rng in (1, 4):
+                        for tw in (0, 1):
+                            yield (a, cl, cng, rl, rng, tw)
+
+
+@pytest.mark.parametrize("b,n", [(1534, 12), (1999999266, 12), (999999618, 14), (30000000, 17)])
+def test_every_plan_variant(gfe, oracle, b, n):
+    """Every admissible pass plan (column/row split, points per thread, sub-transforms per
+    workgroup, table or factored row twiddles) gives the oracle's residue."""
+    iters = 12
+    bits = prp_bits(b, n, iters)
+    o = oracle.Engine(b, n)
+    o.set(1)
+    for bit in bits:
+        o.square_dup(bit)
+    want = canon(oracle, o.get_digits(0), n, b)
+    t = gfe.Transform(b, n)
+    tried = 0
+    for plan in _plans(n):
+        try:
+            t.set_plan(plan)
+        except gfe.GfeError:
+            continue
+        assert t.get_plan() == plan
+        t.set(1)
+        for bit in bits:
+            t.square_dup(bit)
+        assert np.array_equal(canon(oracle, t.get_digits(0), n, b), want), plan
+        tried += 1
+    assert tried >= 20
+    with pytest.raises(gfe.GfeError):
+        t.set_plan((n, 4, 1, 4, 1, 0))  # no row levels left
+
+
+def test_multiplicand_with_factored_twiddles(gfe, oracle):
+    """initMultiplicand + mul (row pass forward-only and product modes) with factored twiddles."""
+    b, n = 1000000000, 14
+    t = gfe.Transform(b, n)
+    t.set_plan((7, 4, 8, 4, 1, 1))
+    o = oracle.Engine(b, n)
+    t.set(3); o.set(3)
+    for _ in range(5):
+        t.square_dup(False); o.square_dup(False)
+    t.copy(1, 0); o.copy(1, 0)
+    t.square_dup(True); o.square_dup(True)
+    t.mul(1); o.mul(1)
+    assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b))
