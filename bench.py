@@ -36,35 +36,50 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bench_n22(G, steps: int, warmup: int):
+def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     n = 22
     b = W.config_bases(W.C4, 1)[0]
     t = G.Transform(b, n)
-    log(f"[bench] n=22 b={b}: tuning")
-    t.tune(6)
+    if plan_arg:
+        t.set_plan([int(v) for v in plan_arg.split(",")])
+    elif tune:
+        log(f"[bench] n=22 b={b}: tuning")
+        t.tune(6)
     plan = t.info()
-    log(f"[bench] plan: {plan}")
+    log(f"[bench] plan: {plan} {t.get_plan()}")
+    # HBM calibration: register copies of N int32 (k_copy, 2 x 16 MiB per launch) for the
+    # FETCH_SIZE / WRITE_SIZE scale of the profiles
+    for _ in range(8):
+        t.copy(1, 2)
+        t.copy(2, 1)
+    t.sync()
     t.set(1)
     dup_pattern = 0x9B4D_63A1_5C27_E8F0
     t.time_squarings(max(warmup, 1), dup_pattern)
     ms = t.time_squarings(steps, dup_pattern)
     kreps = min(steps, 200)
     _, km = t.time_squarings(kreps, dup_pattern, per_kernel=True)
-    names = ["col_fwd", "row", "col_inv", "norm"]
+    fused = bool(t.get_plan()[5] & 2)
+    names = ["col_fwd_all", "row", "col_inv_norm", "fold"] if fused else ["col_fwd", "row", "col_inv", "norm"]
     rns = 3 if "NTT3" in plan else 2
     N = 1 << n
     dom = max(range(4), key=lambda i: km[i])
-    alg_bytes = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 8 * rns * N, 3: 4 * rns * N + 4 * N}[dom]
+    if fused:  # digits in, residues out | row | residues in, digits out | segment carries
+        ng = t.get_plan()[2]
+        alg_bytes = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 4 * rns * N + 4 * N + 8 * N // ng,
+                     3: 16 * N // ng}[dom]
+    else:
+        alg_bytes = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 8 * rns * N, 3: 4 * rns * N + 4 * N}[dom]
     achieved = alg_bytes / (km[dom] * 1e-3) / 1e9
     # butterflies of the dominant pass (row: (n - a) levels fwd + inv, N/2 per level per prime)
-    a = int(plan.split("cols 2^")[1].split(" ")[0])
+    a = t.get_plan()[0]
     levels = {0: a, 1: 2 * (n - a), 2: a}.get(dom, 0)
     bfly = rns * levels * (N // 2)
     bfly_rate = bfly / (km[dom] * 1e-3) if levels else 0.0
     slots = 14.0  # VALU issue slots of one butterfly (2 half-rate mul_lo/mul_hi + 1 mad_u64 + 8 adds/mins)
     valu_peak_slots = 256 * 128 * 2.4e9
     res = {
-        "b": b, "plan": plan, "ms_per_step": ms / steps, "ns_per_squaring": ms / steps * 1e6,
+        "b": b, "plan": plan, "plan_tuple": list(t.get_plan()), "ms_per_step": ms / steps, "ns_per_squaring": ms / steps * 1e6,
         "kernel_us": {names[i]: round(km[i] * 1e3, 2) for i in range(4)},
         "roofline": {
             "bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -157,6 +172,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the n=15 batched leg")
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--plan", default=None, help="fixed n=22 pass plan a,cl,cng,rl,rng,flags (default: the library's)")
+    ap.add_argument("--tune", action="store_true", help="autotune the n=22 plan before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): candidates are independent and shard "
@@ -165,7 +182,7 @@ def main():
 
     import genefer22_amd as G
     G.lib()
-    res = bench_n22(G, args.steps, args.warmup)
+    res = bench_n22(G, args.steps, args.warmup, args.plan, args.tune)
     log(f"[bench] n=22: {res['ns_per_squaring']:.0f} ns/squaring; kernels {res['kernel_us']}")
 
     batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64)

@@ -3,7 +3,7 @@
 // Device state of a batch of `cap` slots (per slot s, strides fixed by n):
 //   regs  : cap x 3 x N int32 digits (r0, r1 = Gerbicz-Li d(t), r2 = scratch)
 //   ws,zp : cap x 3 x N uint32 residues (transform work space, multiplicand)
-//   carry : cap x N/2048 int64 workgroup carries
+//   carry : cap x N/8 int64 workgroup carries (or segment carries of fused plans)
 //   crt   : cap CrtConst, slots: cap SlotInfo (b, prime count, prime width)
 //   tw    : 2 x 3 x N uint32 twiddles (31-bit set, 32-bit set), shared by all slots
 // Every kernel runs over grid.z = slot and exits at once for a slot that is not active in the
@@ -34,7 +34,7 @@ namespace gfe {
 
 struct SlotInfo {
     uint32_t b;
-    int32_t rns, is32, pad;
+    int32_t rns, is32, rounds;   // rounds: carry rounds of the fused inverse column pass
 };
 
 struct BArgs {
@@ -60,7 +60,7 @@ __device__ __forceinline__ int bslot(const BArgs &A) {
 }
 
 template <int LOGE>
-__global__ void __launch_bounds__(1024) kb_col_fwd(BArgs A, int to_zp) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_fwd(BArgs A, int to_zp) {
     extern __shared__ uint32_t lds[];
     const int s = bslot(A);
     if (s < 0) return;
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(1024) kb_col_fwd(BArgs A, int to_zp) {
 }
 
 template <int LOGE>
-__global__ void __launch_bounds__(1024) kb_col_inv(BArgs A) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_inv(BArgs A) {
     extern __shared__ uint32_t lds[];
     const int s = bslot(A);
     if (s < 0) return;
@@ -106,6 +106,108 @@ __global__ void __launch_bounds__(1024) kb_row(BArgs A) {
     else row_body<ModS, LOGE, MODE, TWF>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
 }
 
+// Pipelined plans: persistent column and normalisation kernels over the blocks of each
+// (slot, prime), prefetching the next block (see col_fwd_loop / col_inv_loop / norm_loop).
+template <int LOGE>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_fwd_p(BArgs A, int to_zp) {
+    extern __shared__ uint32_t lds[];
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const int prime = blockIdx.y;
+    if (prime >= si.rns) return;
+    const size_t N = (size_t)1 << A.n;
+    const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * N;
+    uint32_t *dst = (to_zp ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    if (si.is32) col_fwd_loop<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
+    else col_fwd_loop<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
+}
+
+template <int LOGE>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_inv_p(BArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const int prime = blockIdx.y;
+    if (prime >= si.rns) return;
+    const size_t N = (size_t)1 << A.n;
+    uint32_t *data = A.ws + ((size_t)s * 3 + prime) * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    if (si.is32) col_inv_loop<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
+    else col_inv_loop<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
+}
+
+__global__ void __launch_bounds__(256) kb_norm_p(BArgs A) {
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const size_t N = (size_t)1 << A.n;
+    const CrtConst &c = A.crt[s];
+    const bool dup = A.dup ? (A.dup[s] != 0) : false;
+    const uint32_t *ws = A.ws + (size_t)s * 3 * N;
+    int32_t *dst = A.regs + s * A.reg_stride;
+    int64_t *wc = A.carry + (size_t)s * (N >> 3);
+    if (si.rns == 3) {
+        if (si.is32) norm_loop<3, ModU, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x, gridDim.x);
+        else norm_loop<3, ModS, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x, gridDim.x);
+    } else {
+        if (si.is32) norm_loop<2, ModU, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x, gridDim.x);
+        else norm_loop<2, ModS, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x, gridDim.x);
+    }
+}
+
+// Fused plans: all primes of a slot per workgroup (see col_fwd_all_body / col_inv_norm_body).
+template <int LOGE>
+__global__ void __launch_bounds__(512) kb_col_fwd_all(BArgs A, int to_zp) {
+    extern __shared__ uint32_t lds[];
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * ((size_t)1 << A.n);
+    uint32_t *dst = (to_zp ? A.zp : A.ws) + (size_t)s * 3 * ((size_t)1 << A.n);
+    const uint32_t *tw = A.tw + (size_t)si.is32 * 3 * ((size_t)1 << A.n);
+    if (si.rns == 3) {
+        if (si.is32) col_fwd_all_body<3, ModU, LOGE>(A.prU, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+        else col_fwd_all_body<3, ModS, LOGE>(A.prS, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    } else {
+        if (si.is32) col_fwd_all_body<2, ModU, LOGE>(A.prU, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+        else col_fwd_all_body<2, ModS, LOGE>(A.prS, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    }
+}
+
+template <int LOGE>
+__global__ void __launch_bounds__(512) kb_col_inv_norm(BArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int s = bslot(A);
+    if (s < 0) return;
+    const SlotInfo si = A.slots[s];
+    const size_t N = (size_t)1 << A.n;
+    const CrtConst &c = A.crt[s];
+    const bool dup = A.dup ? (A.dup[s] != 0) : false;
+    const uint32_t *ws = A.ws + (size_t)s * 3 * N;
+    const uint32_t *tw = A.tw + (size_t)si.is32 * 3 * N;
+    int32_t *dst = A.regs + s * A.reg_stride;
+    int64_t *seg = A.carry + (size_t)s * (N >> 3);
+    if (si.rns == 3) {
+        if (si.is32) col_inv_norm_body<3, ModU, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
+        else col_inv_norm_body<3, ModS, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
+    } else {
+        if (si.is32) col_inv_norm_body<2, ModU, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
+        else col_inv_norm_body<2, ModS, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
+    }
+}
+
+__global__ void __launch_bounds__(256) kb_fold(BArgs A) {
+    const int s = bslot(A);
+    if (s < 0) return;
+    const size_t N = (size_t)1 << A.n;
+    const size_t nseg = N / (size_t)A.ng;
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nseg) fold_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * (N >> 3), nseg, A.ng, A.slots[s].b, j);
+}
+
 __global__ void __launch_bounds__(256) kb_norm(BArgs A) {
     const int s = bslot(A);
     if (s < 0) return;
@@ -115,7 +217,7 @@ __global__ void __launch_bounds__(256) kb_norm(BArgs A) {
     const bool dup = A.dup ? (A.dup[s] != 0) : false;
     const uint32_t *ws = A.ws + (size_t)s * 3 * N;
     int32_t *dst = A.regs + s * A.reg_stride;
-    int64_t *wc = A.carry + (size_t)s * (N >> 11);
+    int64_t *wc = A.carry + (size_t)s * (N >> 3);
     if (si.rns == 3) {
         if (si.is32) norm_body<3, ModU, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
         else norm_body<3, ModS, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
@@ -131,7 +233,7 @@ __global__ void __launch_bounds__(256) kb_carry_wg(BArgs A) {
     const size_t N = (size_t)1 << A.n;
     const unsigned nwg = (unsigned)(N >> 11);
     const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w < nwg) carry_wg_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * nwg, nwg, A.slots[s].b, w);
+    if (w < nwg) carry_wg_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * (N >> 3), nwg, A.slots[s].b, w);
 }
 
 // r_dst = r_src of each active slot.
@@ -229,6 +331,9 @@ struct Batch {
     std::deque<int> blk_q;
     int blk_next = 0;
     uint32_t zl = 0;                                  // 1 + highest busy slot: grid.z of launches
+    int cus = 0;
+    std::map<std::tuple<const void *, int, size_t>, unsigned> resident;
+    unsigned resident_blocks(const void *fn, int threads, size_t lds);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     PrimeArgs prS, prU;
     std::vector<Slot> slots;
@@ -319,11 +424,11 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     if (hipSetDevice(device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
     ok = true;
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
     plan = default_plan(n);
-    const size_t nwg = N >> 11;
     if (hipMalloc(&regs, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)cap * 3 * N * 4) != hipSuccess ||
         hipMalloc(&zp, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&tw, (size_t)6 * N * 4) != hipSuccess ||
-        hipMalloc(&carry, (size_t)cap * nwg * 8) != hipSuccess || hipMalloc(&crt, (size_t)cap * sizeof(CrtConst)) != hipSuccess ||
+        hipMalloc(&carry, (size_t)cap * (N >> 3) * 8) != hipSuccess || hipMalloc(&crt, (size_t)cap * sizeof(CrtConst)) != hipSuccess ||
         hipMalloc(&slots_dev, (size_t)cap * sizeof(SlotInfo)) != hipSuccess ||
         hipMalloc(&act_dev, (size_t)64 * cap) != hipSuccess || hipMalloc(&dup_dev, (size_t)64 * cap) != hipSuccess)
         return set_error(GFE_ENOMEM, "device allocation failed");
@@ -356,7 +461,7 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
         prS.q[k] = c0.q[k]; prU.q[k] = c1.q[k];
     }
     std::vector<SlotInfo> si(cap);
-    for (auto &x : si) { x.b = 1024; x.rns = 0; x.is32 = 0; x.pad = 0; }
+    for (auto &x : si) { x.b = 1024; x.rns = 0; x.is32 = 0; x.rounds = 1; }
     if (hipMemcpy(slots_dev, si.data(), cap * sizeof(SlotInfo), hipMemcpyHostToDevice) != hipSuccess)
         return set_error(GFE_EHIP, "slot upload");
     if (hipMemsetAsync(regs, 0, (size_t)cap * 3 * N * 4, stream) != hipSuccess ||
@@ -367,6 +472,10 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     set_lds_attr((const void *)kb_col_inv<3>); set_lds_attr((const void *)kb_col_inv<4>); set_lds_attr((const void *)kb_col_inv<5>);
     set_row_attrs<0>();
     set_row_attrs<1>();
+    set_lds_attr((const void *)kb_col_fwd_p<3>); set_lds_attr((const void *)kb_col_fwd_p<4>); set_lds_attr((const void *)kb_col_fwd_p<5>);
+    set_lds_attr((const void *)kb_col_inv_p<3>); set_lds_attr((const void *)kb_col_inv_p<4>); set_lds_attr((const void *)kb_col_inv_p<5>);
+    set_lds_attr((const void *)kb_col_fwd_all<3>); set_lds_attr((const void *)kb_col_fwd_all<4>);
+    set_lds_attr((const void *)kb_col_inv_norm<3>); set_lds_attr((const void *)kb_col_inv_norm<4>);
     slots.resize(cap);
     t0 = now_s();
     return check_hip(hipStreamSynchronize(stream), "init");
@@ -386,6 +495,24 @@ void Batch::launch_col_fwd(const BArgs &A0, int to_zp) {
     A.a = plan.a; A.ng = plan.col_ng;
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+    if (plan.fused) {
+        const dim3 g1(C / plan.col_ng, 1, zdim());
+        if (plan.col_loge == 3) hipLaunchKernelGGL(kb_col_fwd_all<3>, g1, dim3(T), lds_col(), stream, A, to_zp);
+        else hipLaunchKernelGGL(kb_col_fwd_all<4>, g1, dim3(T), lds_col(), stream, A, to_zp);
+        return;
+    }
+    if (plan.pipe) {
+        const size_t lds = lds_col() + ((size_t)4 << plan.a);
+        const void *fn = plan.col_loge == 3 ? (const void *)kb_col_fwd_p<3> : plan.col_loge == 4 ? (const void *)kb_col_fwd_p<4>
+                                                                                                  : (const void *)kb_col_fwd_p<5>;
+        const dim3 g1(persistent_grid(C / plan.col_ng, 3 * zdim(), resident_blocks(fn, T, lds)), 3, zdim());
+        switch (plan.col_loge) {
+            case 3: hipLaunchKernelGGL(kb_col_fwd_p<3>, g1, dim3(T), lds, stream, A, to_zp); break;
+            case 4: hipLaunchKernelGGL(kb_col_fwd_p<4>, g1, dim3(T), lds, stream, A, to_zp); break;
+            default: hipLaunchKernelGGL(kb_col_fwd_p<5>, g1, dim3(T), lds, stream, A, to_zp); break;
+        }
+        return;
+    }
     const dim3 grid(C / plan.col_ng, 3, zdim());
     switch (plan.col_loge) {
         case 3: hipLaunchKernelGGL(kb_col_fwd<3>, grid, dim3(T), lds_col(), stream, A, to_zp); break;
@@ -399,6 +526,24 @@ void Batch::launch_col_inv(const BArgs &A0) {
     A.a = plan.a; A.ng = plan.col_ng;
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+    if (plan.fused) {
+        const dim3 g1(C / plan.col_ng, 1, zdim());
+        if (plan.col_loge == 3) hipLaunchKernelGGL(kb_col_inv_norm<3>, g1, dim3(T), lds_col(), stream, A);
+        else hipLaunchKernelGGL(kb_col_inv_norm<4>, g1, dim3(T), lds_col(), stream, A);
+        return;
+    }
+    if (plan.pipe) {
+        const size_t lds = lds_col() + ((size_t)4 << plan.a);
+        const void *fn = plan.col_loge == 3 ? (const void *)kb_col_inv_p<3> : plan.col_loge == 4 ? (const void *)kb_col_inv_p<4>
+                                                                                                  : (const void *)kb_col_inv_p<5>;
+        const dim3 g1(persistent_grid(C / plan.col_ng, 3 * zdim(), resident_blocks(fn, T, lds)), 3, zdim());
+        switch (plan.col_loge) {
+            case 3: hipLaunchKernelGGL(kb_col_inv_p<3>, g1, dim3(T), lds, stream, A); break;
+            case 4: hipLaunchKernelGGL(kb_col_inv_p<4>, g1, dim3(T), lds, stream, A); break;
+            default: hipLaunchKernelGGL(kb_col_inv_p<5>, g1, dim3(T), lds, stream, A); break;
+        }
+        return;
+    }
     const dim3 grid(C / plan.col_ng, 3, zdim());
     switch (plan.col_loge) {
         case 3: hipLaunchKernelGGL(kb_col_inv<3>, grid, dim3(T), lds_col(), stream, A); break;
@@ -432,9 +577,22 @@ void Batch::launch_row(const BArgs &A0, int mode) {
     else launch_row_mode<ROW_FWD>(*this, A, grid, T);
 }
 
-void Batch::launch_norm_carry(const BArgs &A) {
+void Batch::launch_norm_carry(const BArgs &A0) {
+    if (plan.fused) {
+        BArgs A = A0;
+        A.ng = plan.col_ng;
+        const size_t nseg = N / (size_t)plan.col_ng;
+        hipLaunchKernelGGL(kb_fold, dim3((unsigned)((nseg + 255) / 256), 1, zdim()), dim3(256), 0, stream, A);
+        return;
+    }
+    const BArgs &A = A0;
     const unsigned nwg = (unsigned)(N >> 11);
-    hipLaunchKernelGGL(kb_norm, dim3(nwg, 1, zdim()), dim3(256), 0, stream, A);
+    if (plan.pipe) {
+        const unsigned g = persistent_grid(nwg, zdim(), resident_blocks((const void *)kb_norm_p, 256, 0));
+        hipLaunchKernelGGL(kb_norm_p, dim3(g, 1, zdim()), dim3(256), 0, stream, A);
+    } else {
+        hipLaunchKernelGGL(kb_norm, dim3(nwg, 1, zdim()), dim3(256), 0, stream, A);
+    }
     hipLaunchKernelGGL(kb_carry_wg, dim3((nwg + 255) / 256, 1, zdim()), dim3(256), 0, stream, A);
 }
 
@@ -483,6 +641,18 @@ uint8_t *Batch::stage_acquire() {
 void Batch::stage_release() {
     (void)hipEventRecord(stage_ev[stage_i], stream);
     stage_i = (stage_i + 1) % NSTAGE;
+}
+
+unsigned Batch::resident_blocks(const void *fn, int threads, size_t lds) {
+    const auto key = std::make_tuple(fn, threads, lds);
+    const auto it = resident.find(key);
+    if (it != resident.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds) != hipSuccess || nb < 1) nb = 1;
+    (void)hipGetLastError();
+    const unsigned r = (unsigned)nb * (unsigned)std::max(cus, 1);
+    resident[key] = r;
+    return r;
 }
 
 void Batch::update_zl() {
@@ -578,7 +748,7 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
     mpz_clear(E);
     // slot table and CRT constants; r0 = 1, r1 = 1 (prp: set(1); copy(1, 0))
     SlotInfo &si = slot_host[s];
-    si.b = b; si.rns = S.rns; si.is32 = S.is32; si.pad = 0;
+    si.b = b; si.rns = S.rns; si.is32 = S.is32; si.rounds = carry_rounds(b, n);
     crt_host[s] = make_crt(S.rns, S.is32, n);
     zl = std::max(zl, (uint32_t)s + 1);
     int rc = check_hip(hipMemcpyAsync(slots_dev + s, &si, sizeof(si), hipMemcpyHostToDevice, stream), "slot");
@@ -817,7 +987,7 @@ extern "C" {
 
 size_t gfe_batch_slot_bytes(uint32_t n) {
     const size_t N = (size_t)1 << n;
-    return 9 * N * 4 + (N >> 11) * 8 + sizeof(gfe::CrtConst) + sizeof(gfe::SlotInfo);
+    return 9 * N * 4 + (N >> 3) * 8 + sizeof(gfe::CrtConst) + sizeof(gfe::SlotInfo);
 }
 
 int gfe_batch_create(uint32_t n, uint32_t capacity, int device, gfe_batch **out) {
@@ -951,21 +1121,7 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     bool any = false;
     for (auto &S : B.slots) any |= S.busy;
     if (!any) return set_error(GFE_EINVAL, "tune needs busy slots");
-    std::vector<gfe::Plan> plans;
-    for (int a = 4; a <= (int)B.n - 4; ++a)
-        for (int cl = 3; cl <= 5; ++cl)
-            for (int cng = 1; cng <= 64; cng *= 2)
-                for (int rl = 3; rl <= 5; ++rl)
-                    for (int rng = 1; rng <= 64; rng *= 2) {
-                        gfe::Plan P;
-                        P.a = a; P.col_loge = cl; P.col_ng = cng; P.row_loge = rl; P.row_ng = rng; P.norm_s = 8;
-                        const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (B.n - a)) / (1 << rl);
-                        if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 512) continue;
-                        if (!gfe::plan_valid(P, B.n)) continue;
-                        plans.push_back(P);
-                        P.row_tw = 1;
-                        plans.push_back(P);
-                    }
+    const std::vector<gfe::Plan> plans = gfe::candidate_plans(B.n);
     // squarings on the busy slots' r0: save and restore register 0 of every slot
     const size_t bytes = (size_t)B.cap * 3 * B.N * 4;
     int32_t *save = nullptr;

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "gfe_kernels.h"
@@ -17,17 +19,22 @@ namespace gfe {
 struct Plan {
     int a = 0, col_loge = 4, col_ng = 16, row_loge = 4, row_ng = 1, norm_s = 8;
     int row_tw = 0;  // 1: row twiddles formed from per-row factors (row_body TWF)
+    int fused = 0;   // 1: all-prime column passes with the CRT and carry in the inverse one
+    int pipe = 0;    // 1: persistent column and normalisation passes that prefetch the next block
 };
 
 Plan default_plan(uint32_t n);
 bool plan_valid(const Plan &P, uint32_t n);
 std::string plan_str(const Plan &P, uint32_t n);
 void plan_to_array(const Plan &P, int32_t out[6]);
+std::vector<Plan> candidate_plans(uint32_t n);
+unsigned persistent_grid(unsigned nblk, unsigned groups, unsigned resident);
 bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out);
 void select_primes(uint32_t b, uint32_t n, int *rns, int *is32);
 uint32_t prime_of(int is32, int k);
 uint32_t mulw_const(uint32_t w, uint32_t p);
 CrtConst make_crt(int rns, int is32, uint32_t n);
+int carry_rounds(uint32_t b, uint32_t n);
 int set_error(int code, const std::string &msg);
 extern thread_local std::string g_last_error;
 
@@ -44,6 +51,10 @@ struct Engine {
     uint32_t *tw = nullptr;
     CrtConst *crt = nullptr;
     uint32_t *bdev = nullptr;
+    int rounds = 0;   // carry rounds of the fused inverse column pass
+    int cus = 0;      // compute units of the device
+    std::map<std::tuple<const void *, int, size_t>, unsigned> resident;  // per (kernel, threads, LDS)
+    unsigned resident_blocks(const void *fn, int threads, size_t lds);
     std::vector<hipEvent_t> events;
     bool dev_ok = false;
 
@@ -52,7 +63,7 @@ struct Engine {
     size_t lds_col() const;
     size_t lds_row() const;
     void launch_col_fwd(uint32_t src, uint32_t *dst);
-    void launch_col_inv();
+    void launch_col_inv(int dup);
     void launch_row(int mode, uint32_t *data);
     void launch_norm(int dup);
     void launch_carry();

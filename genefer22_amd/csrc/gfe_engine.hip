@@ -5,9 +5,11 @@
 //   regs  : num_regs x N int32 digits (register 0 = r0), balanced, |d| <= b/2 + 4
 //   ws    : rns x N uint32 residues (transform-domain work space of one squaring)
 //   zp    : rns x N uint32 residues (multiplicand, transform domain)
-//   carry : N / (S 256) int64 workgroup carries of the normalisation
+//   carry : N / 8 int64: workgroup carries of the normalisation, or segment carries of the fused
+//           inverse column pass
 //   tw    : rns x N uint32 twiddles (Montgomery form)
-// One squaring = k_col_fwd, k_row, k_col_inv, k_norm, k_carry_wg on the engine's stream.
+// One squaring = k_col_fwd, k_row, k_col_inv, k_norm, k_carry_wg on the engine's stream, or with
+// a fused plan k_col_fwd_all, k_row, k_col_inv_norm, k_fold.
 
 #include "gfe_engine.h"
 
@@ -43,6 +45,19 @@ struct CarryArgs {
 __global__ void __launch_bounds__(256) k_carry_wg(CarryArgs A) {
     const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w < A.nwg) carry_wg_body(A.dst, A.carry, A.nwg, A.b, w);
+}
+
+struct FoldArgs {
+    int32_t *dst;
+    const int64_t *seg;
+    size_t nseg;
+    int ng;
+    uint32_t b;
+};
+
+__global__ void __launch_bounds__(256) k_fold(FoldArgs A) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < A.nseg) fold_body(A.dst, A.seg, A.nseg, A.ng, A.b, j);
 }
 
 struct MulScalarArgs {
@@ -127,6 +142,22 @@ void set_lds_attr(const void *fn) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
+// Carry rounds of the fused inverse column pass: the first balanced division leaves quotients
+// |q| <= Q1 = (vmax + b/2) / b, with vmax = 2 N (b/2 + 4)^2 the largest coefficient of a doubled
+// square; a round turns |q| <= Q into |q| <= 1 + floor(Q / b).  Returns the rounds until |q| <= 1.
+int carry_rounds(uint32_t b, uint32_t n) {
+    const long double bb = (long double)b;
+    const long double half = bb / 2 + 4;
+    const long double vmax = 2.0L * (long double)(1ull << n) * half * half;
+    long double Q = floorl((vmax + bb / 2) / bb) + 1;
+    int r = 1;
+    while (Q > 1) {
+        Q = 1 + floorl(Q / bb);
+        ++r;
+    }
+    return r;
+}
+
 CrtConst make_crt(int rns, int is32, uint32_t n) {
     CrtConst c;
     memset(&c, 0, sizeof(c));
@@ -170,10 +201,14 @@ Plan default_plan(uint32_t n) {
 
 bool plan_valid(const Plan &P, uint32_t n) {
     const int gc = P.a, gr = (int)n - P.a;
+    if (P.pipe && P.col_ng * ((1 << gc) + ((1 << gc) >> P.col_loge)) * 4 + (4 << gc) > 160 * 1024) return false;
+    if (P.fused && (P.col_ng < 8 || P.col_ng > 64 || P.col_loge > 4 ||
+                    P.col_ng * (1 << gc) / (1 << P.col_loge) > 512)) return false;
     if (gc < P.col_loge || gr < P.row_loge) return false;
     const int Tc = P.col_ng * (1 << gc) / (1 << P.col_loge);
     const int Tr = P.row_ng * (1 << gr) / (1 << P.row_loge);
     if (Tc < 32 || Tc > 1024 || Tr < 32 || Tr > 1024) return false;
+    if (P.col_loge >= 5 && Tc > 512) return false;
     if (P.col_ng > (1 << gr) || P.row_ng > (1 << gc)) return false;
     const size_t lc = (size_t)P.col_ng * ((1 << gc) + ((1 << gc) >> P.col_loge)) * 4;
     const size_t lr = (size_t)P.row_ng * ((1 << gr) + ((1 << gr) >> P.row_loge)) * 4;
@@ -181,17 +216,50 @@ bool plan_valid(const Plan &P, uint32_t n) {
     return true;
 }
 
+// Plans the autotuners time: column levels a in [4, n-4], 8..32 points per thread, 1..64
+// columns or rows per workgroup, 64..1024 threads, table or factored row twiddles, separate or
+// fused (all-prime) column passes.
+// Workgroups per group of a persistent launch: each takes ceil(nblk / grid) blocks, and groups x
+// grid stays within the resident workgroups whenever nblk allows.
+unsigned persistent_grid(unsigned nblk, unsigned groups, unsigned resident) {
+    const unsigned per = std::max(1u, resident / std::max(1u, groups));
+    const unsigned u = (nblk + per - 1) / per;
+    return (nblk + u - 1) / u;
+}
+
+std::vector<Plan> candidate_plans(uint32_t n) {
+    std::vector<Plan> plans;
+    for (int a = 4; a <= (int)n - 4; ++a)
+        for (int cl = 3; cl <= 5; ++cl)
+            for (int cng = 1; cng <= 64; cng *= 2)
+                for (int rl = 3; rl <= 5; ++rl)
+                    for (int rng = 1; rng <= 64; rng *= 2)
+                        for (int f = 0; f < 3; ++f)
+                            for (int tw = 0; tw < 2; ++tw) {
+                                Plan P;
+                                P.a = a; P.col_loge = cl; P.col_ng = cng; P.row_loge = rl; P.row_ng = rng;
+                                P.norm_s = 8; P.fused = (f == 1); P.pipe = (f == 2); P.row_tw = tw;
+                                const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (n - a)) / (1 << rl);
+                                if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 1024) continue;
+                                if (plan_valid(P, n)) plans.push_back(P);
+                            }
+    return plans;
+}
+
 void plan_to_array(const Plan &P, int32_t out[6]) {
-    out[0] = P.a; out[1] = P.col_loge; out[2] = P.col_ng; out[3] = P.row_loge; out[4] = P.row_ng; out[5] = P.row_tw;
+    out[0] = P.a; out[1] = P.col_loge; out[2] = P.col_ng; out[3] = P.row_loge; out[4] = P.row_ng;
+    out[5] = P.row_tw | (P.fused << 1) | (P.pipe << 2);
 }
 
 bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out) {
     Plan P;
-    P.a = in[0]; P.col_loge = in[1]; P.col_ng = in[2]; P.row_loge = in[3]; P.row_ng = in[4]; P.row_tw = in[5];
+    if (in[5] < 0 || in[5] > 7 || (in[5] & 6) == 6) return false;
+    P.a = in[0]; P.col_loge = in[1]; P.col_ng = in[2]; P.row_loge = in[3]; P.row_ng = in[4];
+    P.row_tw = in[5] & 1; P.fused = (in[5] >> 1) & 1; P.pipe = (in[5] >> 2) & 1;
     P.norm_s = 8;
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
     if (P.a < 1 || P.a >= (int)n || P.col_loge < 3 || P.col_loge > 5 || P.row_loge < 3 || P.row_loge > 5) return false;
-    if (!pow2(P.col_ng) || !pow2(P.row_ng) || (P.row_tw != 0 && P.row_tw != 1)) return false;
+    if (!pow2(P.col_ng) || !pow2(P.row_ng)) return false;
     if (!plan_valid(P, n)) return false;
     *out = P;
     return true;
@@ -201,7 +269,8 @@ std::string plan_str(const Plan &P, uint32_t n) {
     std::ostringstream ss;
     ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (n - P.a)
        << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg" << (P.row_tw ? ", factored twiddles" : "")
-       << "), norm " << P.norm_s << " digits/thread";
+       << "), " << (P.fused ? "all-prime columns with CRT + carry" : "norm 8 digits/thread")
+       << (P.pipe ? ", pipelined columns and norm" : "");
     return ss.str();
 }
 
@@ -237,11 +306,29 @@ template <class M, int MODE> static RowFn row_fn(int loge, int twf) {
     return twf ? row_fn_t<M, MODE, 1>(loge) : row_fn_t<M, MODE, 0>(loge);
 }
 
+typedef void (*ColAllFn)(ColAllArgs);
+
 struct KernelSet {
     ColFn col_fwd, col_inv;
     RowFn row_sq, row_mul, row_fwd;
     NormFn norm;
+    ColAllFn col_fwd_all, col_inv_norm;
+    ColFn col_fwd_p, col_inv_p;
+    NormFn norm_p;
 };
+
+template <class M> static void pipe_fns(int loge, ColFn *f, ColFn *i) {
+    switch (loge) {
+        case 3: *f = k_col_fwd_p<M, 3>; *i = k_col_inv_p<M, 3>; break;
+        case 4: *f = k_col_fwd_p<M, 4>; *i = k_col_inv_p<M, 4>; break;
+        default: *f = k_col_fwd_p<M, 5>; *i = k_col_inv_p<M, 5>; break;
+    }
+}
+
+template <int RNS, class M> static void fused_fns(int loge, ColAllFn *f, ColAllFn *i) {
+    if (loge == 3) { *f = k_col_fwd_all<RNS, M, 3>; *i = k_col_inv_norm<RNS, M, 3>; }
+    else { *f = k_col_fwd_all<RNS, M, 4>; *i = k_col_inv_norm<RNS, M, 4>; }
+}
 
 static KernelSet kernels_for(int rns, int is32, const Plan &P) {
     KernelSet K;
@@ -252,6 +339,10 @@ static KernelSet kernels_for(int rns, int is32, const Plan &P) {
         K.row_mul = row_fn<ModU, ROW_MUL>(P.row_loge, P.row_tw);
         K.row_fwd = row_fn<ModU, ROW_FWD>(P.row_loge, P.row_tw);
         K.norm = (rns == 2) ? k_norm<2, ModU, 8> : k_norm<3, ModU, 8>;
+        if (rns == 2) fused_fns<2, ModU>(P.col_loge, &K.col_fwd_all, &K.col_inv_norm);
+        else fused_fns<3, ModU>(P.col_loge, &K.col_fwd_all, &K.col_inv_norm);
+        pipe_fns<ModU>(P.col_loge, &K.col_fwd_p, &K.col_inv_p);
+        K.norm_p = (rns == 2) ? k_norm_p<2, ModU, 8> : k_norm_p<3, ModU, 8>;
     } else {
         K.col_fwd = col_fwd_fn<ModS>(P.col_loge);
         K.col_inv = col_inv_fn<ModS>(P.col_loge);
@@ -259,6 +350,10 @@ static KernelSet kernels_for(int rns, int is32, const Plan &P) {
         K.row_mul = row_fn<ModS, ROW_MUL>(P.row_loge, P.row_tw);
         K.row_fwd = row_fn<ModS, ROW_FWD>(P.row_loge, P.row_tw);
         K.norm = (rns == 2) ? k_norm<2, ModS, 8> : k_norm<3, ModS, 8>;
+        if (rns == 2) fused_fns<2, ModS>(P.col_loge, &K.col_fwd_all, &K.col_inv_norm);
+        else fused_fns<3, ModS>(P.col_loge, &K.col_fwd_all, &K.col_inv_norm);
+        pipe_fns<ModS>(P.col_loge, &K.col_fwd_p, &K.col_inv_p);
+        K.norm_p = (rns == 2) ? k_norm_p<2, ModS, 8> : k_norm_p<3, ModS, 8>;
     }
     return K;
 }
@@ -282,6 +377,13 @@ static void set_lds_attributes() {
                 (void)hipFuncSetAttribute((const void *)K.row_sq, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                 (void)hipFuncSetAttribute((const void *)K.row_mul, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                 (void)hipFuncSetAttribute((const void *)K.row_fwd, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                for (int r = 2; r <= 3; ++r) {
+                    const KernelSet F = kernels_for(r, is32, P);
+                    (void)hipFuncSetAttribute((const void *)F.col_fwd_all, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                    (void)hipFuncSetAttribute((const void *)F.col_inv_norm, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                }
+                (void)hipFuncSetAttribute((const void *)K.col_fwd_p, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                (void)hipFuncSetAttribute((const void *)K.col_inv_p, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
             }
         }
         (void)hipGetLastError();
@@ -313,9 +415,10 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     set_lds_attributes();
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
     plan = default_plan(n);
-    const size_t nwg = N / (plan.norm_s * 256);
+    rounds = carry_rounds(b, n);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
     if (hipMalloc(&regs, (size_t)num_regs * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)rns * N * 4) != hipSuccess ||
-        hipMalloc(&zp, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&carry, nwg * 8) != hipSuccess ||
+        hipMalloc(&zp, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&carry, (N / 8) * 8) != hipSuccess ||
 
         hipMalloc(&tw, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&crt, sizeof(CrtConst)) != hipSuccess ||
         hipMalloc(&bdev, 4) != hipSuccess) {
@@ -347,9 +450,43 @@ size_t Engine::lds_row() const {
     return (size_t)plan.row_ng * (G + (G >> plan.row_loge)) * 4;
 }
 
+// Device-resident workgroups of a kernel configuration (cached).
+unsigned Engine::resident_blocks(const void *fn, int threads, size_t lds) {
+    const auto key = std::make_tuple(fn, threads, lds);
+    const auto it = resident.find(key);
+    if (it != resident.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds) != hipSuccess || nb < 1) nb = 1;
+    (void)hipGetLastError();
+    const unsigned r = (unsigned)nb * (unsigned)std::max(cus, 1);
+    resident[key] = r;
+    return r;
+}
+
 // Forward column pass from register `src` into `dst` (ws or zp).
 void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
     const KernelSet K = kernels_for(rns, is32, plan);
+    if (plan.pipe && !plan.fused) {
+        ColArgs A;
+        A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
+        A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
+        const unsigned nblk = (1u << (n - plan.a)) / plan.col_ng;
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        const size_t lds = lds_col() + ((size_t)4 << plan.a);
+        const unsigned g = persistent_grid(nblk, rns, resident_blocks((const void *)K.col_fwd_p, T, lds));
+        hipLaunchKernelGGL(K.col_fwd_p, dim3(g, rns, 1), dim3(T), lds, stream, A);
+        return;
+    }
+    if (plan.fused) {
+        ColAllArgs A;
+        memset(&A, 0, sizeof(A));
+        A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
+        A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
+        const uint32_t C = 1u << (n - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(K.col_fwd_all, dim3(C / plan.col_ng, 1, 1), dim3(T), lds_col(), stream, A);
+        return;
+    }
     ColArgs A;
     A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
     A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
@@ -358,8 +495,30 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
     hipLaunchKernelGGL(K.col_fwd, dim3(C / plan.col_ng, rns, 1), dim3(T), lds_col(), stream, A);
 }
 
-void Engine::launch_col_inv() {
+// Inverse column pass; with a fused plan it also normalises register 0 (dup: doubled).
+void Engine::launch_col_inv(int dup) {
     const KernelSet K = kernels_for(rns, is32, plan);
+    if (plan.pipe && !plan.fused) {
+        ColArgs A;
+        A.src = nullptr; A.ws = ws; A.tw = tw; A.pr = pr;
+        A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
+        const unsigned nblk = (1u << (n - plan.a)) / plan.col_ng;
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        const size_t lds = lds_col() + ((size_t)4 << plan.a);
+        const unsigned g = persistent_grid(nblk, rns, resident_blocks((const void *)K.col_inv_p, T, lds));
+        hipLaunchKernelGGL(K.col_inv_p, dim3(g, rns, 1), dim3(T), lds, stream, A);
+        return;
+    }
+    if (plan.fused) {
+        ColAllArgs A;
+        memset(&A, 0, sizeof(A));
+        A.ws = ws; A.tw = tw; A.dst = regs; A.seg = carry; A.crt = crt; A.pr = pr; A.b = b;
+        A.dup = dup; A.rounds = rounds; A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
+        const uint32_t C = 1u << (n - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(K.col_inv_norm, dim3(C / plan.col_ng, 1, 1), dim3(T), lds_col(), stream, A);
+        return;
+    }
     ColArgs A;
     A.src = nullptr; A.ws = ws; A.tw = tw; A.pr = pr;
     A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
@@ -379,12 +538,24 @@ void Engine::launch_row(int mode, uint32_t *data) {
     hipLaunchKernelGGL(f, dim3(R / plan.row_ng, rns, 1), dim3(T), lds_row(), stream, A);
 }
 
+// Normalisation of register 0 (fused plan: the cross-segment carries of k_col_inv_norm).
 void Engine::launch_norm(int dup) {
+    if (plan.fused) {
+        FoldArgs A;
+        A.dst = regs; A.seg = carry; A.nseg = N / (size_t)plan.col_ng; A.ng = plan.col_ng; A.b = b;
+        hipLaunchKernelGGL(k_fold, dim3((unsigned)((A.nseg + 255) / 256), 1, 1), dim3(256), 0, stream, A);
+        return;
+    }
     const KernelSet K = kernels_for(rns, is32, plan);
     NormArgs A;
     A.ws = ws; A.dst = regs; A.carry = carry; A.crt = crt; A.b = b; A.n = (int)n; A.dup = dup;
     const unsigned nwg = (unsigned)(N / (plan.norm_s * 256));
-    hipLaunchKernelGGL(K.norm, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
+    if (plan.pipe) {
+        const unsigned g = persistent_grid(nwg, 1, resident_blocks((const void *)K.norm_p, 256, 0));
+        hipLaunchKernelGGL(K.norm_p, dim3(g, 1, 1), dim3(256), 0, stream, A);
+    } else {
+        hipLaunchKernelGGL(K.norm, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
+    }
     launch_carry();
 }
 
@@ -398,14 +569,14 @@ void Engine::launch_carry() {
 void Engine::square(int dup) {
     launch_col_fwd(0, ws);
     launch_row(ROW_SQUARE, ws);
-    launch_col_inv();
+    launch_col_inv(dup);
     launch_norm(dup);
 }
 
 void Engine::mul() {
     launch_col_fwd(0, ws);
     launch_row(ROW_MUL, ws);
-    launch_col_inv();
+    launch_col_inv(0);
     launch_norm(0);
 }
 
@@ -656,7 +827,7 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
             (void)hipEventRecord(g.events[3], g.stream);
             g.launch_row(gfe::ROW_SQUARE, g.ws);
             (void)hipEventRecord(g.events[4], g.stream);
-            g.launch_col_inv();
+            g.launch_col_inv(dup);
             (void)hipEventRecord(g.events[5], g.stream);
             g.launch_norm(dup);
             (void)hipEventRecord(g.events[6], g.stream);
@@ -688,21 +859,7 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
         x ^= x << 13; x ^= x >> 7; x ^= x << 17;
         trial[i] = (int32_t)((int64_t)(x % g.b) - half + ((g.b & 1) ? 0 : 1));
     }
-    std::vector<gfe::Plan> plans;
-    for (int a = 4; a <= (int)g.n - 4; ++a)
-        for (int cl = 3; cl <= 5; ++cl)
-            for (int cng = 1; cng <= 64; cng *= 2)
-                for (int rl = 3; rl <= 5; ++rl)
-                    for (int rng = 1; rng <= 64; rng *= 2) {
-                        gfe::Plan P;
-                        P.a = a; P.col_loge = cl; P.col_ng = cng; P.row_loge = rl; P.row_ng = rng; P.norm_s = 8;
-                        const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (g.n - a)) / (1 << rl);
-                        if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 512) continue;
-                        if (!gfe::plan_valid(P, g.n)) continue;
-                        plans.push_back(P);
-                        P.row_tw = 1;
-                        plans.push_back(P);
-                    }
+    const std::vector<gfe::Plan> plans = gfe::candidate_plans(g.n);
     const gfe::Plan orig = g.plan;
     gfe::Plan best = orig;
     double best_ms = 1e30;

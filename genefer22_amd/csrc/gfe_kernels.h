@@ -59,12 +59,52 @@ __device__ __forceinline__ void fwd_levels_t(const M &md, uint32_t (&x)[1 << LOG
     }
 }
 
+// cnt (a power of two <= MAXC) consecutive table entries from p into w[0..cnt); p is aligned to
+// cnt words, so runs of 4 and more load as 16-byte vectors.
+template <int MAXC>
+__device__ __forceinline__ void load_run(const uint32_t *__restrict__ p, int cnt, uint32_t (&w)[MAXC]) {
+    if (cnt >= 4) {
+#pragma unroll
+        for (int i = 0; i < MAXC; i += 4) {
+            if (i < cnt) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(p + i);
+                w[i] = v.x; w[i + 1] = v.y; w[i + 2] = v.z; w[i + 3] = v.w;
+            }
+        }
+    } else if (cnt == 2) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p);
+        w[0] = v.x;
+        if (MAXC > 1) w[MAXC > 1 ? 1 : 0] = v.y;
+    } else {
+        w[0] = p[0];
+    }
+}
+
 // Table twiddles: level mu, block bi of a sub-transform whose first-level block index is t0
-// uses Psi[(t0 << mu) + bi].
+// uses Psi[(t0 << mu) + bi]; a level's 2^mu twiddles are one aligned run of the table.
 template <class M, int LOGE>
 __device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
                                            uint32_t t0, int ne) {
-    fwd_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return tw[(t0 << mu) + bi]; }, ne);
+    constexpr int E = 1 << LOGE;
+    constexpr int HW = (E / 2 > 0) ? E / 2 : 1;
+#pragma unroll
+    for (int mu = 0; mu < LOGE; ++mu) {
+        if (mu < ne) {
+            uint32_t w[HW];
+            load_run<HW>(tw + (t0 << mu), 1 << mu, w);
+            const int h = E >> (mu + 1);
+#pragma unroll
+            for (int bi = 0; bi < (1 << mu); ++bi) {
+#pragma unroll
+                for (int i = 0; i < h; ++i) {
+                    const int i0 = bi * 2 * h + i, i1 = i0 + h;
+                    const uint32_t u = x[i0], v = md.mulw(x[i1], w[bi]);
+                    x[i0] = md.add(u, v);
+                    x[i1] = md.sub(u, v);
+                }
+            }
+        }
+    }
 }
 
 // Inverse GS levels mu = ne-1..0 (the exact inverse of fwd_levels_t without the 1/2 factors);
@@ -92,10 +132,32 @@ __device__ __forceinline__ void inv_levels_t(const M &md, uint32_t (&x)[1 << LOG
 }
 
 // Table twiddles of the inverse: Psi[m + j]^-1 = -Psi[2m - 1 - j] (lvl0 = global level of mu = 0).
+// Block bi of level mu reads Psi[3 2^(lvl0+mu) - 1 - (t0 << mu) - bi]: the level's twiddles are
+// the aligned run from 3 2^(lvl0+mu) - (t0 + 1) 2^mu, in reverse.
 template <class M, int LOGE>
 __device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
                                            uint32_t t0, int lvl0, int ne) {
-    inv_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return tw[(3u << (lvl0 + mu)) - 1u - ((t0 << mu) + bi)]; }, ne);
+    constexpr int E = 1 << LOGE;
+    constexpr int HW = (E / 2 > 0) ? E / 2 : 1;
+#pragma unroll
+    for (int mu = LOGE - 1; mu >= 0; --mu) {
+        if (mu < ne) {
+            uint32_t w[HW];
+            load_run<HW>(tw + ((3u << (lvl0 + mu)) - ((t0 + 1u) << mu)), 1 << mu, w);
+            const int h = E >> (mu + 1);
+#pragma unroll
+            for (int bi = 0; bi < (1 << mu); ++bi) {
+                const uint32_t wb = w[(1 << mu) - 1 - bi];
+#pragma unroll
+                for (int i = 0; i < h; ++i) {
+                    const int i0 = bi * 2 * h + i, i1 = i0 + h;
+                    const uint32_t u = x[i0], v = x[i1];
+                    x[i0] = md.add(u, v);
+                    x[i1] = md.mulw(md.sub(v, u), wb);
+                }
+            }
+        }
+    }
 }
 
 // Chunk schedule of a g-level pass: a partial first chunk of r = g mod LOGE levels (if r > 0),
@@ -111,29 +173,23 @@ struct Chunking {
 // Column passes.  Thread t -> column q = t % NG, sub-transform s = t / NG of the column block
 // starting at column blk * NG.  LDS element (q, k) at ((k + (k >> LOGE)) * NG + q).
 
-// Forward levels 0..a-1 from signed digits `src` (N int32) to residues `dst` (N uint32).
+// Forward levels 0..a-1 of one prime; x[] holds the thread's chunk-0 elements (column c0 + q,
+// rows s + rr 2^(g - LOGE)); the residues are stored to `dst` (N uint32).
 template <class M, int LOGE>
-__device__ __forceinline__ void col_fwd_body(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                             const uint32_t *__restrict__ tw, int n, int g, int NG, unsigned blk,
-                                             uint32_t *lds) {
+__device__ __forceinline__ void col_fwd_chunks(const M &md, uint32_t (&x)[1 << LOGE], uint32_t *__restrict__ dst,
+                                               const uint32_t *__restrict__ tw, int n, int g, int NG, unsigned blk,
+                                               uint32_t *lds) {
     constexpr int E = 1 << LOGE;
     const uint32_t C = 1u << (n - g);
     const uint32_t c0 = blk * NG;
     const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
     const Chunking ch(g, LOGE);
-    uint32_t x[E];
     for (int ci = 0; ci < ch.nch; ++ci) {
         const int lam0 = ch.lam0(ci, LOGE), ne = ch.ne(ci, LOGE);
         const int ylog = g - lam0 - LOGE;
         const int hi = s >> ylog, lo = s & ((1 << ylog) - 1);
         const int base = (hi << (g - lam0)) + lo;
-        if (ci == 0) {
-#pragma unroll
-            for (int rr = 0; rr < E; ++rr) {
-                const uint32_t k = base + (rr << ylog);
-                x[rr] = to_res(src[(size_t)k * C + c0 + q], md.p);
-            }
-        } else {
+        if (ci > 0) {
             __syncthreads();
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) {
@@ -159,26 +215,55 @@ __device__ __forceinline__ void col_fwd_body(const M &md, const int32_t *__restr
     }
 }
 
-// Inverse levels a-1..0, in place on the residues `dst`.
+// The thread's chunk-0 digits of column c0 + q (rows s + rr 2^(g - LOGE)).
+template <int LOGE>
+__device__ __forceinline__ void col_load_digits(int32_t (&dg)[1 << LOGE], const int32_t *__restrict__ src, int n, int g,
+                                                int NG, unsigned blk) {
+    constexpr int E = 1 << LOGE;
+    const uint32_t C = 1u << (n - g);
+    const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
+    const int ylog = g - LOGE;
+#pragma unroll
+    for (int rr = 0; rr < E; ++rr) dg[rr] = src[(size_t)(s + (rr << ylog)) * C + blk * NG + q];
+}
+
+// Forward levels 0..a-1 from signed digits `src` (N int32) to residues `dst` (N uint32).
 template <class M, int LOGE>
-__device__ __forceinline__ void col_inv_body(const M &md, uint32_t *__restrict__ dst, const uint32_t *__restrict__ tw,
-                                             int n, int g, int NG, unsigned blk, uint32_t *lds) {
+__device__ __forceinline__ void col_fwd_body(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                             const uint32_t *__restrict__ tw, int n, int g, int NG, unsigned blk,
+                                             uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    int32_t dg[E];
+    col_load_digits<LOGE>(dg, src, n, g, NG, blk);
+    uint32_t x[E];
+#pragma unroll
+    for (int rr = 0; rr < E; ++rr) x[rr] = to_res(dg[rr], md.p);
+    col_fwd_chunks<M, LOGE>(md, x, dst, tw, n, g, NG, blk, lds);
+}
+
+// Inverse levels a-1..0 of one prime from the residues `src`; x[] receives the thread's final
+// elements (column c0 + q, rows s + rr 2^(g - LOGE)).
+template <class M, int LOGE>
+__device__ __forceinline__ void col_inv_chunks(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ src,
+                                               const uint32_t *__restrict__ tw, int n, int g, int NG, unsigned blk,
+                                               uint32_t *lds, bool preloaded = false) {
     constexpr int E = 1 << LOGE;
     const uint32_t C = 1u << (n - g);
     const uint32_t c0 = blk * NG;
     const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
     const Chunking ch(g, LOGE);
-    uint32_t x[E];
     for (int ci = ch.nch - 1; ci >= 0; --ci) {
         const int lam0 = ch.lam0(ci, LOGE), ne = ch.ne(ci, LOGE);
         const int ylog = g - lam0 - LOGE;
         const int hi = s >> ylog, lo = s & ((1 << ylog) - 1);
         const int base = (hi << (g - lam0)) + lo;
         if (ci == ch.nch - 1) {
+            if (!preloaded) {
 #pragma unroll
-            for (int rr = 0; rr < E; ++rr) {
-                const uint32_t k = base + (rr << ylog);
-                x[rr] = dst[(size_t)k * C + c0 + q];
+                for (int rr = 0; rr < E; ++rr) {
+                    const uint32_t k = base + (rr << ylog);
+                    x[rr] = src[(size_t)k * C + c0 + q];
+                }
             }
         } else {
             __syncthreads();
@@ -189,13 +274,7 @@ __device__ __forceinline__ void col_inv_body(const M &md, uint32_t *__restrict__
             }
         }
         inv_levels<M, LOGE>(md, x, tw, (1u << lam0) + hi, lam0, ne);
-        if (ci == 0) {
-#pragma unroll
-            for (int rr = 0; rr < E; ++rr) {
-                const uint32_t k = base + (rr << ylog);
-                dst[(size_t)k * C + c0 + q] = x[rr];
-            }
-        } else {
+        if (ci > 0) {
             __syncthreads();
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) {
@@ -204,6 +283,194 @@ __device__ __forceinline__ void col_inv_body(const M &md, uint32_t *__restrict__
             }
         }
     }
+}
+
+// Inverse levels a-1..0, in place on the residues `dst`.
+template <class M, int LOGE>
+__device__ __forceinline__ void col_inv_body(const M &md, uint32_t *__restrict__ dst, const uint32_t *__restrict__ tw,
+                                             int n, int g, int NG, unsigned blk, uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    uint32_t x[E];
+    col_inv_chunks<M, LOGE>(md, x, dst, tw, n, g, NG, blk, lds);
+    const uint32_t C = 1u << (n - g);
+    const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
+    const int ylog = g - LOGE;
+#pragma unroll
+    for (int rr = 0; rr < E; ++rr) dst[(size_t)(s + (rr << ylog)) * C + blk * NG + q] = x[rr];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pipelined column passes.  A persistent workgroup transforms the column blocks blk = first,
+// first + stride, ... (< C / NG) of one prime.  The twiddles of levels 0..a-1 -- the table's first
+// 2^a entries, shared by every column -- are copied into LDS once, and the next block's elements
+// are loaded while the current block is transformed, so that memory traffic overlaps the
+// butterflies.  (Vector-memory loads complete in issue order: a twiddle load inside the loop
+// would wait for the prefetch, hence the LDS copy.)  LDS: [0, 2^a) twiddles, then the exchange
+// buffer of col_fwd_chunks / col_inv_chunks.
+
+// The thread's elements of the first inverse chunk (the last forward chunk) of column block blk.
+template <int LOGE>
+__device__ __forceinline__ void col_load_inv_first(uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ src, int n,
+                                                   int g, int NG, unsigned blk) {
+    constexpr int E = 1 << LOGE;
+    const uint32_t C = 1u << (n - g);
+    const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
+    const Chunking ch(g, LOGE);
+    const int lam0 = ch.lam0(ch.nch - 1, LOGE);
+    const int ylog = g - lam0 - LOGE;
+    const int hi = s >> ylog, lo = s & ((1 << ylog) - 1);
+    const int base = (hi << (g - lam0)) + lo;
+#pragma unroll
+    for (int rr = 0; rr < E; ++rr) x[rr] = src[(size_t)(base + (rr << ylog)) * C + blk * NG + q];
+}
+
+__device__ __forceinline__ uint32_t *col_twiddles_to_lds(const uint32_t *__restrict__ tw, int g, uint32_t *lds) {
+    for (unsigned i = threadIdx.x; i < (1u << g); i += blockDim.x) lds[i] = tw[i];
+    __syncthreads();
+    return lds + (1u << g);
+}
+
+template <class M, int LOGE>
+__device__ __forceinline__ void col_fwd_loop(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                             const uint32_t *__restrict__ tw, int n, int g, int NG, unsigned first,
+                                             unsigned stride, uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    uint32_t *lx = col_twiddles_to_lds(tw, g, lds);
+    const unsigned nblk = (1u << (n - g)) / NG;
+    if (first >= nblk) return;
+    int32_t dg[E];
+    col_load_digits<LOGE>(dg, src, n, g, NG, first);
+    for (unsigned blk = first; blk < nblk; blk += stride) {
+        const bool more = blk + stride < nblk;
+        int32_t dn[E];
+        if (more) col_load_digits<LOGE>(dn, src, n, g, NG, blk + stride);
+        uint32_t x[E];
+#pragma unroll
+        for (int rr = 0; rr < E; ++rr) x[rr] = to_res(dg[rr], md.p);
+        col_fwd_chunks<M, LOGE>(md, x, dst, lds, n, g, NG, blk, lx);
+        if (more) {
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) dg[rr] = dn[rr];
+        }
+    }
+}
+
+template <class M, int LOGE>
+__device__ __forceinline__ void col_inv_loop(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ tw,
+                                             int n, int g, int NG, unsigned first, unsigned stride, uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    uint32_t *lx = col_twiddles_to_lds(tw, g, lds);
+    const unsigned nblk = (1u << (n - g)) / NG;
+    if (first >= nblk) return;
+    const uint32_t C = 1u << (n - g);
+    const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
+    const int ylog = g - LOGE;
+    uint32_t xc[E];
+    col_load_inv_first<LOGE>(xc, data, n, g, NG, first);
+    for (unsigned blk = first; blk < nblk; blk += stride) {
+        const bool more = blk + stride < nblk;
+        uint32_t xn[E];
+        if (more) col_load_inv_first<LOGE>(xn, data, n, g, NG, blk + stride);
+        col_inv_chunks<M, LOGE>(md, xc, data, lds, n, g, NG, blk, lx, true);
+#pragma unroll
+        for (int rr = 0; rr < E; ++rr) data[(size_t)(s + (rr << ylog)) * C + blk * NG + q] = xc[rr];
+        if (more) {
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) xc[rr] = xn[rr];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Column passes over every prime of a candidate.  One workgroup holds NG columns (all 2^a rows)
+// and runs the primes one after another through the same LDS: the forward pass reads each digit
+// once, and the inverse pass ends with the scale, the Garner CRT and the carry in registers, so
+// that one squaring moves 56 N bytes instead of 80 N (no separate normalisation pass).
+//
+// Carry.  Digit i = k C + c of row k is held by lane q = c - c0, so a row segment
+// (k, c0 .. c0 + NG - 1) lies in NG consecutive lanes of one wavefront (8 <= NG <= 64).  The exact
+// coefficient v is divided by b (balanced) and every quotient moves one lane up, `rounds` times,
+// all lanes at once (cross-lane shuffles); the quotients leaving the segment's last lane are summed
+// into the segment's carry-out, which fold_body adds into the next segment after the kernel (the
+// reference hands carries over in the same two steps, within a work-item group and then across
+// groups: ocl/kernel.cl:2366-2459).  `rounds` is the smallest count after which every quotient is
+// bounded by 1 (host: carry_rounds), so the digits end with |d| <= b/2 + 1.
+
+template <int RNS, class M, int LOGE>
+__device__ __forceinline__ void col_fwd_all_body(const PrimeArgs &pr, const int32_t *__restrict__ src,
+                                                 uint32_t *__restrict__ ws, const uint32_t *__restrict__ tw, int n,
+                                                 int g, int NG, unsigned blk, uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    const size_t N = (size_t)1 << n;
+    int32_t dg[E];
+    col_load_digits<LOGE>(dg, src, n, g, NG, blk);
+#pragma unroll
+    for (int k = 0; k < RNS; ++k) {
+        const M md{pr.p[k], pr.q[k]};
+        uint32_t x[E];
+#pragma unroll
+        for (int rr = 0; rr < E; ++rr) x[rr] = to_res(dg[rr], md.p);
+        col_fwd_chunks<M, LOGE>(md, x, ws + k * N, tw + k * N, n, g, NG, blk, lds);
+    }
+}
+
+template <int RNS, class M, int LOGE>
+__device__ __forceinline__ void col_inv_norm_body(const CrtConst &c, const uint32_t *__restrict__ ws,
+                                                  const uint32_t *__restrict__ tw, int32_t *__restrict__ dst,
+                                                  int64_t *__restrict__ seg, uint32_t b, bool dup, int rounds, int n,
+                                                  int g, int NG, unsigned blk, uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    const size_t N = (size_t)1 << n;
+    const uint32_t C = 1u << (n - g);
+    const int t = threadIdx.x, q = t & (NG - 1), s = t / NG;
+    const int ylog = g - LOGE;
+    const M m1{c.p[0], c.q[0]}, m2{c.p[1], c.q[1]}, m3{c.p[2], c.q[2]};
+    uint32_t xs[RNS][E];
+    col_inv_chunks<M, LOGE>(m1, xs[0], ws, tw, n, g, NG, blk, lds);
+    col_inv_chunks<M, LOGE>(m2, xs[1], ws + N, tw + N, n, g, NG, blk, lds);
+    if (RNS == 3) col_inv_chunks<M, LOGE>(m3, xs[RNS - 1], ws + 2 * N, tw + 2 * N, n, g, NG, blk, lds);
+    const double binv = 1.0 / (double)b;
+    const uint32_t segs_per_row = C / NG;
+#pragma unroll
+    for (int rr = 0; rr < E; ++rr) {
+        const uint32_t k = s + (rr << ylog);
+        const uint32_t r1 = m1.mulw(xs[0][rr], c.scale[0]);
+        const uint32_t r2 = m2.mulw(xs[1][rr], c.scale[1]);
+        const uint32_t r3 = (RNS == 3) ? m3.mulw(xs[RNS - 1][rr], c.scale[2]) : 0u;
+        uint64_t x64;
+        double xd;
+        garner<RNS>(c, m1, m2, m3, r1, r2, r3, &x64, &xd);
+        if (dup) { x64 <<= 1; xd *= 2.0; }
+        int64_t qc;
+        int32_t r = div_bal(x64, xd, b, binv, &qc);
+        int64_t acc = 0;
+        for (int j = 1; j < rounds; ++j) {
+            int64_t qin = __shfl_up(qc, 1u, NG);
+            if (q == 0) qin = 0;
+            acc += qc;
+            const int64_t w = (int64_t)r + qin;
+            r = div_bal((uint64_t)w, (double)w, b, binv, &qc);
+        }
+        int64_t qin = __shfl_up(qc, 1u, NG);
+        if (q == 0) qin = 0;
+        acc += qc;
+        dst[(size_t)k * C + blk * NG + q] = r + (int32_t)qin;
+        if (q == NG - 1) seg[(size_t)k * segs_per_row + blk] = acc;
+    }
+}
+
+// Adds segment j - 1's carry-out (the last segment's, negated, for j = 0: x^N = -1) into the
+// digits of segment j (NG digits from j NG), digit by digit while a carry remains.
+__device__ __forceinline__ void fold_body(int32_t *dc, const int64_t *seg, size_t nseg, int NG, uint32_t b, size_t j) {
+    int64_t cin = (j == 0) ? -seg[nseg - 1] : seg[j - 1];
+    if (cin == 0) return;
+    int32_t *p = dc + j * (size_t)NG;
+    const double binv = 1.0 / (double)b;
+    for (int i = 0; i < NG - 1 && cin != 0; ++i) {
+        const int64_t f = (int64_t)p[i] + cin;
+        p[i] = div_bal((uint64_t)f, (double)f, b, binv, &cin);
+    }
+    p[NG - 1] += (int32_t)cin;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -396,6 +663,57 @@ __device__ __forceinline__ void norm_body(const CrtConst &c, const uint32_t *__r
     carry_tail<S>(d, cr, dst, i0, wc, blk, b, binv);
 }
 
+// Persistent normalisation: workgroup blocks blk = first, first + stride, ... (< N / (256 S)),
+// the next block's residues loaded while the current block is converted.
+template <int RNS, class M, int S>
+__device__ __forceinline__ void norm_loop(const CrtConst &c, const uint32_t *__restrict__ ws, int32_t *dst, int64_t *wc,
+                                          uint32_t b, bool dup, int n, unsigned first, unsigned stride) {
+    const size_t N = (size_t)1 << n;
+    const unsigned nblk = (unsigned)(N / (256 * S));
+    if (first >= nblk) return;
+    const double binv = 1.0 / (double)b;
+    const M m1{c.p[0], c.q[0]}, m2{c.p[1], c.q[1]}, m3{c.p[2], c.q[2]};
+    uint32_t a[MAX_RNS][S], an[MAX_RNS][S];
+    auto load = [&](uint32_t (&v)[MAX_RNS][S], unsigned blk) {
+        const uint32_t *w0 = ws + ((size_t)blk * 256 + threadIdx.x) * S;
+#pragma unroll
+        for (int k = 0; k < RNS; ++k) {
+#pragma unroll
+            for (int j = 0; j < S; j += 4) {
+                const uint4 u = *reinterpret_cast<const uint4 *>(w0 + (size_t)k * N + j);
+                v[k][j] = u.x; v[k][j + 1] = u.y; v[k][j + 2] = u.z; v[k][j + 3] = u.w;
+            }
+        }
+    };
+    load(a, first);
+    for (unsigned blk = first; blk < nblk; blk += stride) {
+        const bool more = blk + stride < nblk;
+        if (more) load(an, blk + stride);
+        int32_t d[S];
+        int64_t cr = 0;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const uint32_t r1 = m1.mulw(a[0][j], c.scale[0]);
+            const uint32_t r2 = m2.mulw(a[1][j], c.scale[1]);
+            const uint32_t r3 = (RNS == 3) ? m3.mulw(a[RNS - 1][j], c.scale[2]) : 0u;
+            uint64_t x64; double xd;
+            garner<RNS>(c, m1, m2, m3, r1, r2, r3, &x64, &xd);
+            if (dup) { x64 <<= 1; xd *= 2.0; }
+            x64 += (uint64_t)cr;
+            xd += (double)cr;
+            d[j] = div_bal(x64, xd, b, binv, &cr);
+        }
+        carry_tail<S>(d, cr, dst, ((size_t)blk * 256 + threadIdx.x) * S, wc, blk, b, binv);
+        __syncthreads();  // carry_tail's LDS is rewritten by the next block
+        if (more) {
+#pragma unroll
+            for (int k = 0; k < RNS; ++k)
+#pragma unroll
+                for (int j = 0; j < S; ++j) a[k][j] = an[k][j];
+        }
+    }
+}
+
 // r = a r on the balanced digits (the reference's mulscalar, ocl/kernel.cl:2461-2492, run after
 // the normalisation as baseModMul does, src/transformGPU.h:828-837), 8 digits per thread.
 __device__ __forceinline__ void mulscalar_body(int32_t *dc, int64_t *wc, uint32_t b, int32_t a, unsigned blk) {
@@ -440,8 +758,10 @@ struct ColArgs {
     int n, a, ng;
 };
 
+// 32 points per thread need more than the 128 registers a 1024-thread workgroup leaves: such plans
+// run at most 512 threads.
 template <class M, int LOGE>
-__global__ void __launch_bounds__(1024) k_col_fwd(ColArgs A) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) k_col_fwd(ColArgs A) {
     extern __shared__ uint32_t lds[];
     const int prime = blockIdx.y;
     const M md{A.pr.p[prime], A.pr.q[prime]};
@@ -450,12 +770,57 @@ __global__ void __launch_bounds__(1024) k_col_fwd(ColArgs A) {
 }
 
 template <class M, int LOGE>
-__global__ void __launch_bounds__(1024) k_col_inv(ColArgs A) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) k_col_inv(ColArgs A) {
     extern __shared__ uint32_t lds[];
     const int prime = blockIdx.y;
     const M md{A.pr.p[prime], A.pr.q[prime]};
     const size_t N = (size_t)1 << A.n;
     col_inv_body<M, LOGE>(md, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+// Pipelined (persistent) column kernels: grid (workgroups per prime, rns).
+template <class M, int LOGE>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) k_col_fwd_p(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    col_fwd_loop<M, LOGE>(md, A.src, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
+}
+
+template <class M, int LOGE>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) k_col_inv_p(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    col_inv_loop<M, LOGE>(md, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
+}
+
+struct ColAllArgs {
+    const int32_t *src;   // digits (forward)
+    uint32_t *ws;         // residues [prime][N]
+    const uint32_t *tw;   // [prime][N]
+    int32_t *dst;         // digits (inverse)
+    int64_t *seg;         // segment carries
+    const CrtConst *crt;
+    PrimeArgs pr;
+    uint32_t b;
+    int dup, rounds, n, a, ng;
+};
+
+// At most 512 threads (fused plans): the inverse holds every prime's elements in registers.
+template <int RNS, class M, int LOGE>
+__global__ void __launch_bounds__(512) k_col_fwd_all(ColAllArgs A) {
+    extern __shared__ uint32_t lds[];
+    col_fwd_all_body<RNS, M, LOGE>(A.pr, A.src, A.ws, A.tw, A.n, A.a, A.ng, blockIdx.x, lds);
+}
+
+template <int RNS, class M, int LOGE>
+__global__ void __launch_bounds__(512) k_col_inv_norm(ColAllArgs A) {
+    extern __shared__ uint32_t lds[];
+    col_inv_norm_body<RNS, M, LOGE>(*A.crt, A.ws, A.tw, A.dst, A.seg, A.b, A.dup != 0, A.rounds, A.n, A.a, A.ng,
+                                    blockIdx.x, lds);
 }
 
 struct RowArgs {
@@ -483,6 +848,11 @@ struct NormArgs {
     uint32_t b;
     int n, dup;
 };
+
+template <int RNS, class M, int S>
+__global__ void __launch_bounds__(256) k_norm_p(NormArgs A) {
+    norm_loop<RNS, M, S>(*A.crt, A.ws, A.dst, A.carry, A.b, A.dup != 0, A.n, blockIdx.x, gridDim.x);
+}
 
 template <int RNS, class M, int S>
 __global__ void __launch_bounds__(256) k_norm(NormArgs A) {

@@ -88,7 +88,8 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
 int gfe_tune(gfe_engine *e, uint32_t reps);
 
 /* The pass plan as {a (column levels), column E log2, columns per workgroup, row E log2, rows per
- * workgroup, factored row twiddles 0/1}.  No reference counterpart (the reference's tuned split is
+ * workgroup, flags: bit 0 factored row twiddles, bit 1 all-prime column passes with the CRT and
+ * carry fused into the inverse one}.  No reference counterpart (the reference's tuned split is
  * internal to its engines): fixes a plan for reproducible benchmarks and for tests that cover
  * every kernel variant.  gfe_set_plan returns GFE_EINVAL for a plan not admissible at this n. */
 int gfe_get_plan(const gfe_engine *e, int32_t plan[6]);

@@ -92,14 +92,17 @@ def _plans(n):
             for rl in (3, 4, 5):
                 for cng in (1, 8, 32):
                     for rng in (1, 4):
-                        for tw in (0, 1):
-                            yield (a, cl, cng, rl, rng, tw)
+                        # bit 0 factored row twiddles, bit 1 fused columns, bit 2 pipelined columns
+                        for flags in (0, 1, 2, 3, 4, 5):
+                            yield (a, cl, cng, rl, rng, flags)
 
 
 @pytest.mark.parametrize("b,n", [(1534, 12), (1999999266, 12), (999999618, 14), (30000000, 17)])
 def test_every_plan_variant(gfe, oracle, b, n):
     """Every admissible pass plan (column/row split, points per thread, sub-transforms per
-    workgroup, table or factored row twiddles) gives the oracle's residue."""
+    workgroup, table or factored row twiddles, separate or fused all-prime column passes with the
+    CRT and carry, pipelined persistent column passes) gives the oracle's residue, digits within
+    the balanced bound."""
     iters = 12
     bits = prp_bits(b, n, iters)
     o = oracle.Engine(b, n)
@@ -118,18 +121,22 @@ def test_every_plan_variant(gfe, oracle, b, n):
         t.set(1)
         for bit in bits:
             t.square_dup(bit)
-        assert np.array_equal(canon(oracle, t.get_digits(0), n, b), want), plan
+        got = t.get_digits(0)
+        assert np.abs(got).max() <= b // 2 + 4, plan
+        assert np.array_equal(canon(oracle, got, n, b), want), plan
         tried += 1
     assert tried >= 20
     with pytest.raises(gfe.GfeError):
         t.set_plan((n, 4, 1, 4, 1, 0))  # no row levels left
 
 
-def test_multiplicand_with_factored_twiddles(gfe, oracle):
-    """initMultiplicand + mul (row pass forward-only and product modes) with factored twiddles."""
+@pytest.mark.parametrize("flags", [1, 2, 3, 4, 5])
+def test_multiplicand_plan_variants(gfe, oracle, flags):
+    """initMultiplicand + mul (row pass forward-only and product modes) under factored twiddles
+    and fused column passes."""
     b, n = 1000000000, 14
     t = gfe.Transform(b, n)
-    t.set_plan((7, 4, 8, 4, 1, 1))
+    t.set_plan((7, 4, 8, 4, 8, flags))
     o = oracle.Engine(b, n)
     t.set(3); o.set(3)
     for _ in range(5):
