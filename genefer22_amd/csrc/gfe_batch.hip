@@ -26,38 +26,13 @@
 
 #include "../../include/gfe.h"
 #include "../../include/gfe_batch.h"
+#include "gfe_batch_dev.h"
 #include "gfe_engine.h"
+#include "gfe_spec.h"
 #include "genefer.h"
 #include "gint.h"
 
 namespace gfe {
-
-struct SlotInfo {
-    uint32_t b;
-    int32_t rns, is32, rounds;   // rounds: carry rounds of the fused inverse column pass
-};
-
-struct BArgs {
-    int32_t *regs;
-    uint32_t *ws, *zp;
-    int64_t *carry;
-    const CrtConst *crt;
-    const SlotInfo *slots;
-    const uint32_t *tw;      // [2][3][N]
-    const uint8_t *active;   // [cap] or null (all)
-    const uint8_t *dup;      // [cap] or null
-    PrimeArgs prS, prU;
-    long long reg_stride;    // 3 N
-    int n, a, ng;
-    uint32_t src_reg, dst_reg;
-    int32_t value;
-};
-
-__device__ __forceinline__ int bslot(const BArgs &A) {
-    const int s = (int)blockIdx.z;
-    if (A.active && !A.active[s]) return -1;
-    return s;
-}
 
 template <int LOGE>
 __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_fwd(BArgs A, int to_zp) {
@@ -71,8 +46,8 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_fwd(BArgs A, in
     const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * N;
     uint32_t *dst = (to_zp ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
     const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
-    if (si.is32) col_fwd_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
-    else col_fwd_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    if (si.is32) col_fwd_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
+    else col_fwd_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 template <int LOGE>
@@ -86,8 +61,8 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_inv(BArgs A) {
     const size_t N = (size_t)1 << A.n;
     uint32_t *dst = A.ws + ((size_t)s * 3 + prime) * N;
     const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
-    if (si.is32) col_inv_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
-    else col_inv_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    if (si.is32) col_inv_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
+    else col_inv_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 template <int LOGE, int MODE, int TWF>
@@ -169,11 +144,11 @@ __global__ void __launch_bounds__(512) kb_col_fwd_all(BArgs A, int to_zp) {
     uint32_t *dst = (to_zp ? A.zp : A.ws) + (size_t)s * 3 * ((size_t)1 << A.n);
     const uint32_t *tw = A.tw + (size_t)si.is32 * 3 * ((size_t)1 << A.n);
     if (si.rns == 3) {
-        if (si.is32) col_fwd_all_body<3, ModU, LOGE>(A.prU, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
-        else col_fwd_all_body<3, ModS, LOGE>(A.prS, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+        if (si.is32) col_fwd_all_body<3, ModU, LOGE>(A.prU, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
+        else col_fwd_all_body<3, ModS, LOGE>(A.prS, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
     } else {
-        if (si.is32) col_fwd_all_body<2, ModU, LOGE>(A.prU, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
-        else col_fwd_all_body<2, ModS, LOGE>(A.prS, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+        if (si.is32) col_fwd_all_body<2, ModU, LOGE>(A.prU, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
+        else col_fwd_all_body<2, ModS, LOGE>(A.prS, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
     }
 }
 
@@ -191,11 +166,11 @@ __global__ void __launch_bounds__(512) kb_col_inv_norm(BArgs A) {
     int32_t *dst = A.regs + s * A.reg_stride;
     int64_t *seg = A.carry + (size_t)s * (N >> 3);
     if (si.rns == 3) {
-        if (si.is32) col_inv_norm_body<3, ModU, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
-        else col_inv_norm_body<3, ModS, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
+        if (si.is32) col_inv_norm_body<3, ModU, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
+        else col_inv_norm_body<3, ModS, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
     } else {
-        if (si.is32) col_inv_norm_body<2, ModU, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
-        else col_inv_norm_body<2, ModS, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, blockIdx.x, lds);
+        if (si.is32) col_inv_norm_body<2, ModU, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
+        else col_inv_norm_body<2, ModS, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
     }
 }
 
@@ -472,6 +447,7 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     set_lds_attr((const void *)kb_col_inv<3>); set_lds_attr((const void *)kb_col_inv<4>); set_lds_attr((const void *)kb_col_inv<5>);
     set_row_attrs<0>();
     set_row_attrs<1>();
+    spec_batch_set_lds_attributes();
     set_lds_attr((const void *)kb_col_fwd_p<3>); set_lds_attr((const void *)kb_col_fwd_p<4>); set_lds_attr((const void *)kb_col_fwd_p<5>);
     set_lds_attr((const void *)kb_col_inv_p<3>); set_lds_attr((const void *)kb_col_inv_p<4>); set_lds_attr((const void *)kb_col_inv_p<5>);
     set_lds_attr((const void *)kb_col_fwd_all<3>); set_lds_attr((const void *)kb_col_fwd_all<4>);
@@ -493,6 +469,13 @@ BArgs Batch::args(const uint8_t *active, const uint8_t *dup) const {
 void Batch::launch_col_fwd(const BArgs &A0, int to_zp) {
     BArgs A = A0;
     A.a = plan.a; A.ng = plan.col_ng;
+    if (plan.spec) {
+        const uint32_t C = 1u << (n - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(spec_batch_col_fwd(plan.col_loge, plan.a), dim3(C / plan.col_ng, 3, zdim()), dim3(T),
+                           col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A, to_zp);
+        return;
+    }
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
     if (plan.fused) {
@@ -524,6 +507,13 @@ void Batch::launch_col_fwd(const BArgs &A0, int to_zp) {
 void Batch::launch_col_inv(const BArgs &A0) {
     BArgs A = A0;
     A.a = plan.a; A.ng = plan.col_ng;
+    if (plan.spec) {
+        const uint32_t C = 1u << (n - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(spec_batch_col_inv(plan.col_loge, plan.a), dim3(C / plan.col_ng, 3, zdim()), dim3(T),
+                           col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A, 0);
+        return;
+    }
     const uint32_t C = 1u << (n - plan.a);
     const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
     if (plan.fused) {
@@ -572,6 +562,11 @@ void Batch::launch_row(const BArgs &A0, int mode) {
     const uint32_t R = 1u << plan.a, G = 1u << (n - plan.a);
     const int T = plan.row_ng * G / (1 << plan.row_loge);
     const dim3 grid(R / plan.row_ng, 3, zdim());
+    if (plan.spec && mode == ROW_SQUARE) {
+        hipLaunchKernelGGL(spec_batch_row_square(plan.row_loge, (int)n - plan.a), grid, dim3(T),
+                           row_pass_lds_words((int)n - plan.a, plan.row_loge, plan.row_ng) * 4, stream, A);
+        return;
+    }
     if (mode == ROW_SQUARE) launch_row_mode<ROW_SQUARE>(*this, A, grid, T);
     else if (mode == ROW_MUL) launch_row_mode<ROW_MUL>(*this, A, grid, T);
     else launch_row_mode<ROW_FWD>(*this, A, grid, T);

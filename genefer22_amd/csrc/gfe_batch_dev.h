@@ -1,0 +1,36 @@
+// gfe_batch_dev.h -- device-side argument block of the batched engine (gfe_batch.hip and the
+// compile-time pass kernels of gfe_spec_batch.hip).
+#pragma once
+
+#include "gfe_kernels.h"
+
+namespace gfe {
+
+struct SlotInfo {
+    uint32_t b;
+    int32_t rns, is32, rounds;   // rounds: carry rounds of the fused inverse column pass
+};
+
+struct BArgs {
+    int32_t *regs;
+    uint32_t *ws, *zp;
+    int64_t *carry;
+    const CrtConst *crt;
+    const SlotInfo *slots;
+    const uint32_t *tw;      // [2][3][N]
+    const uint8_t *active;   // [cap] or null (all)
+    const uint8_t *dup;      // [cap] or null
+    PrimeArgs prS, prU;
+    long long reg_stride;    // 3 N
+    int n, a, ng;
+    uint32_t src_reg, dst_reg;
+    int32_t value;
+};
+
+__device__ __forceinline__ int bslot(const BArgs &A) {
+    const int s = (int)blockIdx.z;
+    if (A.active && !A.active[s]) return -1;
+    return s;
+}
+
+}  // namespace gfe

@@ -40,6 +40,13 @@ struct ModS {  // p < 2^31
         const uint32_t r = (uint32_t)(t >> 32) - __umulhi(m, p);
         return min(r, r + p);
     }
+    // a * w mod p with wm = w 2^32 mod p and wq = wm q mod 2^32: the Montgomery quotient is
+    // a wq mod 2^32, so the product needs no 64-bit multiply (three full-rate 32-bit multiplies
+    // instead of a half-rate v_mad_u64_u32 and two).
+    __device__ __forceinline__ uint32_t mulp(uint32_t a, uint32_t wm, uint32_t wq) const {
+        const uint32_t r = __umulhi(a, wm) - __umulhi(a * wq, p);
+        return min(r, r + p);
+    }
 };
 
 struct ModU {  // 2^31 < p < 2^32
@@ -61,7 +68,21 @@ struct ModU {  // 2^31 < p < 2^32
     }
     // a * w mod p with wm = w 2^32 mod p (Montgomery form of w).
     __device__ __forceinline__ uint32_t mulw(uint32_t a, uint32_t wm) const { return mont(a, wm); }
+    // As ModS::mulp (wq = wm q mod 2^32).
+    __device__ __forceinline__ uint32_t mulp(uint32_t a, uint32_t wm, uint32_t wq) const {
+        const uint32_t hi = __umulhi(a, wm), mh = __umulhi(a * wq, p);
+        const uint32_t r = hi - mh;
+        return (hi < mh) ? r + p : r;
+    }
 };
+
+// Workgroups are dispatched to the 8 XCDs round-robin by linear workgroup id, and each XCD has its
+// own L2.  When a grid's x extent is a multiple of 8 the XCD of workgroup (x, y, z) is x mod 8;
+// this maps x to block (x mod 8) (nx / 8) + x / 8, so that every XCD runs one contiguous range of
+// blocks and neighbouring blocks that share cache lines meet in the same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned x, unsigned nx) {
+    return (nx & 7u) ? x : (x & 7u) * (nx >> 3) + (x >> 3);
+}
 
 // Residue of a signed digit.
 __device__ __forceinline__ uint32_t to_res(int32_t d, uint32_t p) {

@@ -10,6 +10,8 @@
 #include <vector>
 
 #include "gfe_kernels.h"
+#include "gfe_pass.h"
+#include "gfe_spec.h"
 
 namespace gfe {
 
@@ -21,6 +23,7 @@ struct Plan {
     int row_tw = 0;  // 1: row twiddles formed from per-row factors (row_body TWF)
     int fused = 0;   // 1: all-prime column passes with the CRT and carry in the inverse one
     int pipe = 0;    // 1: persistent column and normalisation passes that prefetch the next block
+    int spec = 0;    // 1: column passes and the squaring row pass with compile-time geometry (gfe_pass.h)
 };
 
 Plan default_plan(uint32_t n);

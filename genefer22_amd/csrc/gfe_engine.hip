@@ -201,6 +201,21 @@ Plan default_plan(uint32_t n) {
 
 bool plan_valid(const Plan &P, uint32_t n) {
     const int gc = P.a, gr = (int)n - P.a;
+    if (P.spec) {
+        if (P.fused || P.pipe || P.row_tw) return false;
+        if (!spec_col_ok(gc, P.col_loge, P.col_ng) || !spec_row_ok(gr, P.row_loge)) return false;
+        if (P.col_ng > (1 << gr) || P.row_ng > (1 << gc)) return false;
+        const int Tc = P.col_ng * (1 << gc) / (1 << P.col_loge);
+        const int Tr = P.row_ng * (1 << gr) / (1 << P.row_loge);
+        const int maxc = P.col_loge >= 5 ? 512 : 1024, maxr = P.row_loge >= 5 ? 512 : 1024;
+        if (Tc < 64 || Tc > maxc || Tr < 64 || Tr > maxr) return false;
+        // the ROW_FWD / ROW_MUL passes of the multiplicand keep the runtime-geometry row kernel
+        const size_t lr_rt = (size_t)P.row_ng * ((1 << gr) + ((1 << gr) >> P.row_loge)) * 4;
+        if (col_pass_lds_words(gc, P.col_loge, P.col_ng) * 4 > 160 * 1024 ||
+            row_pass_lds_words(gr, P.row_loge, P.row_ng) * 4 > 160 * 1024 || lr_rt > 160 * 1024)
+            return false;
+        return true;
+    }
     if (P.pipe && P.col_ng * ((1 << gc) + ((1 << gc) >> P.col_loge)) * 4 + (4 << gc) > 160 * 1024) return false;
     if (P.fused && (P.col_ng < 8 || P.col_ng > 64 || P.col_loge > 4 ||
                     P.col_ng * (1 << gc) / (1 << P.col_loge) > 512)) return false;
@@ -234,11 +249,12 @@ std::vector<Plan> candidate_plans(uint32_t n) {
             for (int cng = 1; cng <= 64; cng *= 2)
                 for (int rl = 3; rl <= 5; ++rl)
                     for (int rng = 1; rng <= 64; rng *= 2)
-                        for (int f = 0; f < 3; ++f)
+                        for (int f = 0; f < 4; ++f)
                             for (int tw = 0; tw < 2; ++tw) {
+                                if (f == 3 && tw) continue;
                                 Plan P;
                                 P.a = a; P.col_loge = cl; P.col_ng = cng; P.row_loge = rl; P.row_ng = rng;
-                                P.norm_s = 8; P.fused = (f == 1); P.pipe = (f == 2); P.row_tw = tw;
+                                P.norm_s = 8; P.fused = (f == 1); P.pipe = (f == 2); P.spec = (f == 3); P.row_tw = tw;
                                 const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (n - a)) / (1 << rl);
                                 if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 1024) continue;
                                 if (plan_valid(P, n)) plans.push_back(P);
@@ -248,14 +264,14 @@ std::vector<Plan> candidate_plans(uint32_t n) {
 
 void plan_to_array(const Plan &P, int32_t out[6]) {
     out[0] = P.a; out[1] = P.col_loge; out[2] = P.col_ng; out[3] = P.row_loge; out[4] = P.row_ng;
-    out[5] = P.row_tw | (P.fused << 1) | (P.pipe << 2);
+    out[5] = P.row_tw | (P.fused << 1) | (P.pipe << 2) | (P.spec << 3);
 }
 
 bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out) {
     Plan P;
-    if (in[5] < 0 || in[5] > 7 || (in[5] & 6) == 6) return false;
+    if (in[5] < 0 || in[5] > 15 || (in[5] & 6) == 6 || ((in[5] & 8) && in[5] != 8)) return false;
     P.a = in[0]; P.col_loge = in[1]; P.col_ng = in[2]; P.row_loge = in[3]; P.row_ng = in[4];
-    P.row_tw = in[5] & 1; P.fused = (in[5] >> 1) & 1; P.pipe = (in[5] >> 2) & 1;
+    P.row_tw = in[5] & 1; P.fused = (in[5] >> 1) & 1; P.pipe = (in[5] >> 2) & 1; P.spec = (in[5] >> 3) & 1;
     P.norm_s = 8;
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
     if (P.a < 1 || P.a >= (int)n || P.col_loge < 3 || P.col_loge > 5 || P.row_loge < 3 || P.row_loge > 5) return false;
@@ -270,7 +286,7 @@ std::string plan_str(const Plan &P, uint32_t n) {
     ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (n - P.a)
        << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg" << (P.row_tw ? ", factored twiddles" : "")
        << "), " << (P.fused ? "all-prime columns with CRT + carry" : "norm 8 digits/thread")
-       << (P.pipe ? ", pipelined columns and norm" : "");
+       << (P.pipe ? ", pipelined columns and norm" : "") << (P.spec ? ", compile-time passes" : "");
     return ss.str();
 }
 
@@ -413,6 +429,7 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     if (device < 0 || device >= ndev) return set_error(GFE_ENODEV, "device index out of range");
     if (hipSetDevice(device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice failed");
     set_lds_attributes();
+    spec_set_lds_attributes();
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
     plan = default_plan(n);
     rounds = carry_rounds(b, n);
@@ -465,6 +482,16 @@ unsigned Engine::resident_blocks(const void *fn, int threads, size_t lds) {
 
 // Forward column pass from register `src` into `dst` (ws or zp).
 void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
+    if (plan.spec) {
+        ColArgs A;
+        A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
+        A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
+        const uint32_t C = 1u << (n - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(spec_col_fwd(is32, plan.col_loge, plan.a), dim3(C / plan.col_ng, rns, 1), dim3(T),
+                           col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A);
+        return;
+    }
     const KernelSet K = kernels_for(rns, is32, plan);
     if (plan.pipe && !plan.fused) {
         ColArgs A;
@@ -497,6 +524,16 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
 
 // Inverse column pass; with a fused plan it also normalises register 0 (dup: doubled).
 void Engine::launch_col_inv(int dup) {
+    if (plan.spec) {
+        ColArgs A;
+        A.src = nullptr; A.ws = ws; A.tw = tw; A.pr = pr;
+        A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
+        const uint32_t C = 1u << (n - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(spec_col_inv(is32, plan.col_loge, plan.a), dim3(C / plan.col_ng, rns, 1), dim3(T),
+                           col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A);
+        return;
+    }
     const KernelSet K = kernels_for(rns, is32, plan);
     if (plan.pipe && !plan.fused) {
         ColArgs A;
@@ -534,6 +571,11 @@ void Engine::launch_row(int mode, uint32_t *data) {
     A.n = (int)n; A.a = plan.a; A.ng = plan.row_ng;
     const uint32_t R = 1u << plan.a, G = 1u << (n - plan.a);
     const int T = plan.row_ng * G / (1 << plan.row_loge);
+    if (plan.spec && mode == ROW_SQUARE) {
+        hipLaunchKernelGGL(spec_row_square(is32, plan.row_loge, (int)n - plan.a), dim3(R / plan.row_ng, rns, 1), dim3(T),
+                           row_pass_lds_words((int)n - plan.a, plan.row_loge, plan.row_ng) * 4, stream, A);
+        return;
+    }
     RowFn f = (mode == ROW_SQUARE) ? K.row_sq : (mode == ROW_MUL) ? K.row_mul : K.row_fwd;
     hipLaunchKernelGGL(f, dim3(R / plan.row_ng, rns, 1), dim3(T), lds_row(), stream, A);
 }

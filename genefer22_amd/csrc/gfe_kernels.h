@@ -36,7 +36,8 @@ struct PrimeArgs {
 };
 
 // Forward CT levels mu = 0..ne-1 of a 2^LOGE-point sub-transform held in x[].  twf(mu, bi) is
-// the twiddle (Montgomery form) of level mu, register block bi.
+// the twiddle (Montgomery form) of level mu, register block bi; its Montgomery quotient factor
+// w q is formed once per twiddle and shared by the block's butterflies (ModS::mulp).
 template <class M, int LOGE, class TWF>
 __device__ __forceinline__ void fwd_levels_t(const M &md, uint32_t (&x)[1 << LOGE], const TWF &twf, int ne) {
     constexpr int E = 1 << LOGE;
@@ -46,11 +47,11 @@ __device__ __forceinline__ void fwd_levels_t(const M &md, uint32_t (&x)[1 << LOG
             const int h = E >> (mu + 1);
 #pragma unroll
             for (int bi = 0; bi < (1 << mu); ++bi) {
-                const uint32_t w = twf(mu, bi);
+                const uint32_t w = twf(mu, bi), wq = w * md.q;
 #pragma unroll
                 for (int i = 0; i < h; ++i) {
                     const int i0 = bi * 2 * h + i, i1 = i0 + h;
-                    const uint32_t u = x[i0], v = md.mulw(x[i1], w);
+                    const uint32_t u = x[i0], v = md.mulp(x[i1], w, wq);
                     x[i0] = md.add(u, v);
                     x[i1] = md.sub(u, v);
                 }
@@ -59,52 +60,51 @@ __device__ __forceinline__ void fwd_levels_t(const M &md, uint32_t (&x)[1 << LOG
     }
 }
 
-// cnt (a power of two <= MAXC) consecutive table entries from p into w[0..cnt); p is aligned to
-// cnt words, so runs of 4 and more load as 16-byte vectors.
-template <int MAXC>
-__device__ __forceinline__ void load_run(const uint32_t *__restrict__ p, int cnt, uint32_t (&w)[MAXC]) {
-    if (cnt >= 4) {
-#pragma unroll
-        for (int i = 0; i < MAXC; i += 4) {
-            if (i < cnt) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(p + i);
-                w[i] = v.x; w[i + 1] = v.y; w[i + 2] = v.z; w[i + 3] = v.w;
-            }
-        }
-    } else if (cnt == 2) {
-        const uint2 v = *reinterpret_cast<const uint2 *>(p);
-        w[0] = v.x;
-        if (MAXC > 1) w[MAXC > 1 ? 1 : 0] = v.y;
-    } else {
-        w[0] = p[0];
-    }
-}
-
-// Table twiddles: level mu, block bi of a sub-transform whose first-level block index is t0
-// uses Psi[(t0 << mu) + bi]; a level's 2^mu twiddles are one aligned run of the table.
-template <class M, int LOGE>
-__device__ __forceinline__ void fwd_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
-                                           uint32_t t0, int ne) {
-    constexpr int E = 1 << LOGE;
-    constexpr int HW = (E / 2 > 0) ? E / 2 : 1;
+// A chunk's twiddles in registers: level mu (< ne) occupies w[2^mu .. 2^(mu+1)), one aligned run
+// of 2^mu table entries starting at `start(mu)`, loaded as 16-byte vectors where the run allows.
+// Issued before a chunk's LDS exchange, the loads overlap the barrier.
+template <int LOGE, class START>
+__device__ __forceinline__ void load_chunk_tw(uint32_t (&w)[1 << LOGE], const uint32_t *__restrict__ tw, const START &start,
+                                              int ne) {
 #pragma unroll
     for (int mu = 0; mu < LOGE; ++mu) {
         if (mu < ne) {
-            uint32_t w[HW];
-            load_run<HW>(tw + (t0 << mu), 1 << mu, w);
-            const int h = E >> (mu + 1);
+            const uint32_t *p = tw + start(mu);
+            const int c = 1 << mu;
+            if (c >= 4) {
 #pragma unroll
-            for (int bi = 0; bi < (1 << mu); ++bi) {
-#pragma unroll
-                for (int i = 0; i < h; ++i) {
-                    const int i0 = bi * 2 * h + i, i1 = i0 + h;
-                    const uint32_t u = x[i0], v = md.mulw(x[i1], w[bi]);
-                    x[i0] = md.add(u, v);
-                    x[i1] = md.sub(u, v);
+                for (int i = 0; i < c; i += 4) {
+                    const uint4 v = *reinterpret_cast<const uint4 *>(p + i);
+                    w[c + i] = v.x; w[c + i + 1] = v.y; w[c + i + 2] = v.z; w[c + i + 3] = v.w;
                 }
+            } else if (c == 2) {
+                const uint2 v = *reinterpret_cast<const uint2 *>(p);
+                w[2] = v.x; w[3] = v.y;
+            } else {
+                w[1] = p[0];
             }
         }
     }
+}
+
+// Forward twiddles of a chunk whose first-level block index is t0: level mu, block bi uses
+// Psi[(t0 << mu) + bi].
+template <int LOGE>
+__device__ __forceinline__ void load_fwd_tw(uint32_t (&w)[1 << LOGE], const uint32_t *__restrict__ tw, uint32_t t0, int ne) {
+    load_chunk_tw<LOGE>(w, tw, [&](int mu) { return t0 << mu; }, ne);
+}
+
+// Inverse twiddles: level mu, block bi uses Psi[3 2^(lvl0+mu) - 1 - (t0 << mu) - bi], i.e. entry
+// 2^mu - 1 - bi of the aligned run from 3 2^(lvl0+mu) - (t0 + 1) 2^mu.
+template <int LOGE>
+__device__ __forceinline__ void load_inv_tw(uint32_t (&w)[1 << LOGE], const uint32_t *__restrict__ tw, uint32_t t0, int lvl0,
+                                            int ne) {
+    load_chunk_tw<LOGE>(w, tw, [&](int mu) { return (3u << (lvl0 + mu)) - ((t0 + 1u) << mu); }, ne);
+}
+
+template <class M, int LOGE>
+__device__ __forceinline__ void fwd_levels_r(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t (&w)[1 << LOGE], int ne) {
+    fwd_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return w[(1 << mu) + bi]; }, ne);
 }
 
 // Inverse GS levels mu = ne-1..0 (the exact inverse of fwd_levels_t without the 1/2 factors);
@@ -118,46 +118,22 @@ __device__ __forceinline__ void inv_levels_t(const M &md, uint32_t (&x)[1 << LOG
             const int h = E >> (mu + 1);
 #pragma unroll
             for (int bi = 0; bi < (1 << mu); ++bi) {
-                const uint32_t w = twf(mu, bi);
+                const uint32_t w = twf(mu, bi), wq = w * md.q;
 #pragma unroll
                 for (int i = 0; i < h; ++i) {
                     const int i0 = bi * 2 * h + i, i1 = i0 + h;
                     const uint32_t u = x[i0], v = x[i1];
                     x[i0] = md.add(u, v);
-                    x[i1] = md.mulw(md.sub(v, u), w);
+                    x[i1] = md.mulp(md.sub(v, u), w, wq);
                 }
             }
         }
     }
 }
 
-// Table twiddles of the inverse: Psi[m + j]^-1 = -Psi[2m - 1 - j] (lvl0 = global level of mu = 0).
-// Block bi of level mu reads Psi[3 2^(lvl0+mu) - 1 - (t0 << mu) - bi]: the level's twiddles are
-// the aligned run from 3 2^(lvl0+mu) - (t0 + 1) 2^mu, in reverse.
 template <class M, int LOGE>
-__device__ __forceinline__ void inv_levels(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t *__restrict__ tw,
-                                           uint32_t t0, int lvl0, int ne) {
-    constexpr int E = 1 << LOGE;
-    constexpr int HW = (E / 2 > 0) ? E / 2 : 1;
-#pragma unroll
-    for (int mu = LOGE - 1; mu >= 0; --mu) {
-        if (mu < ne) {
-            uint32_t w[HW];
-            load_run<HW>(tw + ((3u << (lvl0 + mu)) - ((t0 + 1u) << mu)), 1 << mu, w);
-            const int h = E >> (mu + 1);
-#pragma unroll
-            for (int bi = 0; bi < (1 << mu); ++bi) {
-                const uint32_t wb = w[(1 << mu) - 1 - bi];
-#pragma unroll
-                for (int i = 0; i < h; ++i) {
-                    const int i0 = bi * 2 * h + i, i1 = i0 + h;
-                    const uint32_t u = x[i0], v = x[i1];
-                    x[i0] = md.add(u, v);
-                    x[i1] = md.mulw(md.sub(v, u), wb);
-                }
-            }
-        }
-    }
+__device__ __forceinline__ void inv_levels_r(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t (&w)[1 << LOGE], int ne) {
+    inv_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return w[(2 << mu) - 1 - bi]; }, ne);
 }
 
 // Chunk schedule of a g-level pass: a partial first chunk of r = g mod LOGE levels (if r > 0),
@@ -189,6 +165,8 @@ __device__ __forceinline__ void col_fwd_chunks(const M &md, uint32_t (&x)[1 << L
         const int ylog = g - lam0 - LOGE;
         const int hi = s >> ylog, lo = s & ((1 << ylog) - 1);
         const int base = (hi << (g - lam0)) + lo;
+        uint32_t w[E];
+        load_fwd_tw<LOGE>(w, tw, (1u << lam0) + hi, ne);
         if (ci > 0) {
             __syncthreads();
 #pragma unroll
@@ -197,7 +175,7 @@ __device__ __forceinline__ void col_fwd_chunks(const M &md, uint32_t (&x)[1 << L
                 x[rr] = lds[(k + (k >> LOGE)) * NG + q];
             }
         }
-        fwd_levels<M, LOGE>(md, x, tw, (1u << lam0) + hi, ne);
+        fwd_levels_r<M, LOGE>(md, x, w, ne);
         if (ci == ch.nch - 1) {
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) {
@@ -257,6 +235,8 @@ __device__ __forceinline__ void col_inv_chunks(const M &md, uint32_t (&x)[1 << L
         const int ylog = g - lam0 - LOGE;
         const int hi = s >> ylog, lo = s & ((1 << ylog) - 1);
         const int base = (hi << (g - lam0)) + lo;
+        uint32_t w[E];
+        load_inv_tw<LOGE>(w, tw, (1u << lam0) + hi, lam0, ne);
         if (ci == ch.nch - 1) {
             if (!preloaded) {
 #pragma unroll
@@ -273,7 +253,7 @@ __device__ __forceinline__ void col_inv_chunks(const M &md, uint32_t (&x)[1 << L
                 x[rr] = lds[(k + (k >> LOGE)) * NG + q];
             }
         }
-        inv_levels<M, LOGE>(md, x, tw, (1u << lam0) + hi, lam0, ne);
+        inv_levels_r<M, LOGE>(md, x, w, ne);
         if (ci > 0) {
             __syncthreads();
 #pragma unroll
@@ -501,14 +481,27 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
     uint32_t *dst = data + (size_t)row * G;
     uint32_t *L = lds + q * (G + spr);
     const Chunking ch(g, LOGE);
-    uint32_t x[E];
-    int hi = 0, base = 0, ylog = 0;
-    for (int ci = 0; ci < ch.nch; ++ci) {
-        const int lam0 = ch.lam0(ci, LOGE), ne = ch.ne(ci, LOGE);
+    // chunk geometry: chunk ci spans levels lam0 .. lam0 + LOGE - 1 of the row transform; the
+    // thread's elements are base + rr 2^ylog, its first-level block index within the chunk is hi
+    auto geom = [&](int ci, int &lam0, int &ne, int &ylog, int &hi, int &base) {
+        lam0 = ch.lam0(ci, LOGE);
+        ne = ch.ne(ci, LOGE);
         ylog = g - lam0 - LOGE;
         hi = s >> ylog;
-        const int lo = s & ((1 << ylog) - 1);
-        base = (hi << (g - lam0)) + lo;
+        base = (hi << (g - lam0)) + (s & ((1 << ylog) - 1));
+    };
+    // table twiddles of the chunk, loaded before the chunk's LDS exchange (TWF = 0)
+    auto fwd_tw = [&](uint32_t (&w)[E], int lam0, int hi, int ne) {
+        load_fwd_tw<LOGE>(w, tw, (1u << (a + lam0)) + (row << lam0) + (uint32_t)hi, ne);
+    };
+    auto inv_tw = [&](uint32_t (&w)[E], int lam0, int hi, int ne) {
+        load_inv_tw<LOGE>(w, tw, (1u << (a + lam0)) + (row << lam0) + (uint32_t)hi, a + lam0, ne);
+    };
+    uint32_t x[E], w[E];
+    int lam0 = 0, ne = 0, hi = 0, base = 0, ylog = 0;
+    for (int ci = 0; ci < ch.nch; ++ci) {
+        geom(ci, lam0, ne, ylog, hi, base);
+        if (!TWF) fwd_tw(w, lam0, hi, ne);
         if (ci == 0) {
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) x[rr] = dst[base + (rr << ylog)];
@@ -526,7 +519,7 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
             for (int mu = 0; mu < LOGE; ++mu) rb[mu] = (mu < ne) ? tw[(1u << (a + lam0 + mu)) + (row << (lam0 + mu))] : 0u;
             fwd_levels_t<M, LOGE>(md, x, [&](int mu, int bi) { return md.mont(rb[mu], tw[((uint32_t)hi << mu) + bi]); }, ne);
         } else {
-            fwd_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), ne);
+            fwd_levels_r<M, LOGE>(md, x, w, ne);
         }
         if (ci < ch.nch - 1) {
             __syncthreads();
@@ -542,6 +535,9 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
         for (int rr = 0; rr < E; ++rr) dst[base + (rr << ylog)] = x[rr];
         return;
     }
+    // the last forward chunk's element set is the first inverse chunk's: its twiddles load while
+    // the pointwise square runs
+    if (!TWF) inv_tw(w, lam0, hi, ne);
     if (MODE == ROW_SQUARE) {
 #pragma unroll
         for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], x[rr]);
@@ -551,12 +547,9 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
         for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], z[base + (rr << ylog)]);
     }
     for (int ci = ch.nch - 1; ci >= 0; --ci) {
-        const int lam0 = ch.lam0(ci, LOGE), ne = ch.ne(ci, LOGE);
-        ylog = g - lam0 - LOGE;
-        hi = s >> ylog;
-        const int lo = s & ((1 << ylog) - 1);
-        base = (hi << (g - lam0)) + lo;
+        geom(ci, lam0, ne, ylog, hi, base);
         if (ci != ch.nch - 1) {
+            if (!TWF) inv_tw(w, lam0, hi, ne);
             __syncthreads();
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) {
@@ -573,7 +566,7 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
                 return md.mont(rb[mu], tw[(1u << (lam0 + mu)) - 1u - (((uint32_t)hi << mu) + bi)]);
             }, ne);
         } else {
-            inv_levels<M, LOGE>(md, x, tw, ((1u << (a + lam0)) + (row << lam0) + hi), a + lam0, ne);
+            inv_levels_r<M, LOGE>(md, x, w, ne);
         }
         if (ci == 0) {
 #pragma unroll
@@ -766,7 +759,7 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) k_col_fwd(ColArgs A) {
     const int prime = blockIdx.y;
     const M md{A.pr.p[prime], A.pr.q[prime]};
     const size_t N = (size_t)1 << A.n;
-    col_fwd_body<M, LOGE>(md, A.src, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
+    col_fwd_body<M, LOGE>(md, A.src, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 template <class M, int LOGE>
@@ -775,7 +768,7 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) k_col_inv(ColArgs A) {
     const int prime = blockIdx.y;
     const M md{A.pr.p[prime], A.pr.q[prime]};
     const size_t N = (size_t)1 << A.n;
-    col_inv_body<M, LOGE>(md, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, blockIdx.x, lds);
+    col_inv_body<M, LOGE>(md, A.ws + prime * N, A.tw + prime * N, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 // Pipelined (persistent) column kernels: grid (workgroups per prime, rns).
@@ -813,14 +806,14 @@ struct ColAllArgs {
 template <int RNS, class M, int LOGE>
 __global__ void __launch_bounds__(512) k_col_fwd_all(ColAllArgs A) {
     extern __shared__ uint32_t lds[];
-    col_fwd_all_body<RNS, M, LOGE>(A.pr, A.src, A.ws, A.tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    col_fwd_all_body<RNS, M, LOGE>(A.pr, A.src, A.ws, A.tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 template <int RNS, class M, int LOGE>
 __global__ void __launch_bounds__(512) k_col_inv_norm(ColAllArgs A) {
     extern __shared__ uint32_t lds[];
     col_inv_norm_body<RNS, M, LOGE>(*A.crt, A.ws, A.tw, A.dst, A.seg, A.b, A.dup != 0, A.rounds, A.n, A.a, A.ng,
-                                    blockIdx.x, lds);
+                                    xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
 struct RowArgs {

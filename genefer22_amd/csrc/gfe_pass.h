@@ -1,0 +1,323 @@
+// gfe_pass.h -- row and column passes of the four-step NTT with compile-time geometry.
+//
+// Same transform as the runtime-geometry bodies of gfe_kernels.h (row_body, col_fwd_chunks,
+// col_inv_chunks): negacyclic Cooley-Tukey forward with Psi[m + j] = psi^bitrev(m + j), pointwise
+// square / product, Gentleman-Sande inverse; the residues these passes write are identical.  What
+// differs is how the work is laid out for gfx950:
+//
+//  * LOGG (levels of the pass) and LOGE (points per thread) are template parameters, so the chunk
+//    loop unrolls completely and every LDS and global element address is a per-thread base plus a
+//    compile-time offset (ds_read/ds_write immediate offsets, one VGPR address per chunk);
+//  * a chunk's twiddles are loaded into registers before its LDS exchange, so the loads overlap
+//    the barrier;
+//  * twiddle products use ModS::mulp (Montgomery quotient from a precomputed w q: no 64-bit
+//    multiply);
+//  * for the 31-bit primes (2p < 2^32) butterfly outputs that only feed the next level's multiply
+//    are left in [0, 2p) (forward), and the inverse's difference is formed as v - u + p in (0, 2p)
+//    before its multiply: mulp accepts any 32-bit operand and returns a reduced residue.
+//
+// Chunks: a LOGG-level pass runs a partial first chunk of R = LOGG mod LOGE levels (when R > 0),
+// then full chunks of LOGE levels.  In chunk ci the thread holds the E = 2^LOGE elements
+// k = base + rr 2^YL (rr < E) of one sub-transform, with base = (hi << (LOGG - lam0)) + lo,
+// hi = s >> YL, lo = s mod 2^YL (s = the thread's sub-transform index) and YL = LOGG - lam0 - LOGE.
+// base and rr 2^YL have disjoint bits, so the padded LDS index k + (k >> LOGE) of element rr is
+// (base + (base >> LOGE)) + (c + (c >> LOGE)) with c = rr 2^YL: a compile-time offset.
+#pragma once
+
+#include <type_traits>
+
+#include "gfe_kernels.h"
+
+namespace gfe {
+
+template <int I, int END, class F>
+__device__ __forceinline__ void static_for(const F &f) {
+    if constexpr (I < END) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, END>(f);
+    }
+}
+
+// END - 1, END - 2, ..., I
+template <int I, int END, class F>
+__device__ __forceinline__ void static_for_down(const F &f) {
+    if constexpr (I < END) {
+        f(std::integral_constant<int, END - 1>{});
+        static_for_down<I, END - 1>(f);
+    }
+}
+
+template <class M> struct LazyOk : std::false_type {};
+template <> struct LazyOk<ModS> : std::true_type {};
+
+template <int LOGG, int LOGE>
+struct Geo {
+    static_assert(LOGG >= LOGE, "a pass spans at least one chunk");
+    static constexpr int R = LOGG % LOGE;
+    static constexpr int NCH = (LOGG + LOGE - 1) / LOGE;
+    static constexpr int lam0(int ci) { return R == 0 ? ci * LOGE : (ci == 0 ? 0 : R + (ci - 1) * LOGE); }
+    static constexpr int ne(int ci) { return (R != 0 && ci == 0) ? R : LOGE; }
+    static constexpr int ylog(int ci) { return LOGG - lam0(ci) - LOGE; }
+};
+
+// Padded LDS offset of element rr of a chunk with stride 2^YL.
+template <int YL, int LOGE>
+__device__ __forceinline__ constexpr uint32_t chunk_off(int rr) {
+    return ((uint32_t)rr << YL) + (((uint32_t)rr << YL) >> LOGE);
+}
+
+// Forward CT levels 0..NE-1 of the chunk in x[], twiddles w[2^mu + bi] (load_fwd_tw).  Outputs of
+// every level but the last that are the next level's multiplied operands (index bit h/2 set) stay
+// lazy for the 31-bit primes: u + v and u - v + p, both in [0, 2p).
+template <class M, int LOGE, int NE>
+__device__ __forceinline__ void fwd_chunk(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t (&w)[1 << LOGE]) {
+    constexpr int E = 1 << LOGE;
+#pragma unroll
+    for (int mu = 0; mu < NE; ++mu) {
+        const int h = E >> (mu + 1);
+#pragma unroll
+        for (int bi = 0; bi < (1 << mu); ++bi) {
+            const uint32_t wv = w[(1 << mu) + bi], wq = wv * md.q;
+#pragma unroll
+            for (int i = 0; i < h; ++i) {
+                const int i0 = bi * 2 * h + i, i1 = i0 + h;
+                const uint32_t u = x[i0], v = md.mulp(x[i1], wv, wq);
+                if (LazyOk<M>::value && mu + 1 < NE && (i0 & (h >> 1)) != 0) {
+                    x[i0] = u + v;
+                    x[i1] = u - v + md.p;
+                } else {
+                    x[i0] = md.add(u, v);
+                    x[i1] = md.sub(u, v);
+                }
+            }
+        }
+    }
+}
+
+// Inverse GS levels NE-1..0 of the chunk, twiddles w[2^mu + j] (load_inv_tw; block bi uses entry
+// 2^mu - 1 - bi).  The difference feeding the multiply is lazy for the 31-bit primes.
+template <class M, int LOGE, int NE>
+__device__ __forceinline__ void inv_chunk(const M &md, uint32_t (&x)[1 << LOGE], const uint32_t (&w)[1 << LOGE]) {
+    constexpr int E = 1 << LOGE;
+#pragma unroll
+    for (int mu = NE - 1; mu >= 0; --mu) {
+        const int h = E >> (mu + 1);
+#pragma unroll
+        for (int bi = 0; bi < (1 << mu); ++bi) {
+            const uint32_t wv = w[(2 << mu) - 1 - bi], wq = wv * md.q;
+#pragma unroll
+            for (int i = 0; i < h; ++i) {
+                const int i0 = bi * 2 * h + i, i1 = i0 + h;
+                const uint32_t u = x[i0], v = x[i1];
+                x[i0] = md.add(u, v);
+                const uint32_t d = LazyOk<M>::value ? v - u + md.p : md.sub(v, u);
+                x[i1] = md.mulp(d, wv, wq);
+            }
+        }
+    }
+}
+
+// Byte offset -> element pointer (keeps the 32-bit per-thread offset separate from the uniform
+// base so that global accesses can use a scalar base address).
+template <class T>
+__device__ __forceinline__ T *at_bytes(T *base, uint32_t bytes) {
+    return reinterpret_cast<T *>(reinterpret_cast<typename std::conditional<std::is_const<T>::value, const char, char>::type *>(base) + bytes);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row pass: levels a..n-1 on NG rows of G = 2^LOGG contiguous residues, the pointwise square or
+// product, inverse levels n-1..a (ROW_FWD: forward only).  Thread t -> row q = t / (G/E) of the
+// block's rows (blk NG + q), sub-transform s = t mod (G/E).  LDS: row q at q (G + G/E), element k
+// at k + (k >> LOGE).  `data`, `zp` and `tw` point at one prime's N values.
+template <class M, int LOGE, int LOGG, int MODE>
+__device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ zp,
+                                         const uint32_t *__restrict__ tw, int a, int NG, unsigned blk, uint32_t *lds) {
+    using GM = Geo<LOGG, LOGE>;
+    constexpr int E = 1 << LOGE, G = 1 << LOGG, SPR = G >> LOGE;
+    const uint32_t t = threadIdx.x, q = t / SPR, s = t % SPR;
+    const uint32_t row = blk * (uint32_t)NG + q;
+    const uint32_t rowb = row * (uint32_t)G * 4u;  // byte offset of the row
+    uint32_t *L = lds + q * (G + SPR);
+    uint32_t x[E], w[E];
+    static_for<0, GM::NCH>([&](auto cc) {
+        constexpr int ci = decltype(cc)::value;
+        constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
+        const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+        load_fwd_tw<LOGE>(w, tw, (1u << (a + lam0)) + (row << lam0) + hi, NE);
+        if constexpr (ci == 0) {
+            const uint32_t *p = at_bytes(data, rowb + base * 4u);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = p[rr << YL];
+        } else {
+            __syncthreads();
+            const uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+        }
+        fwd_chunk<M, LOGE, NE>(md, x, w);
+        if constexpr (ci < GM::NCH - 1) {
+            __syncthreads();
+            uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+        }
+    });
+    // the last forward chunk (YL = 0: E consecutive elements from s E) is the first inverse chunk
+    constexpr int LAST = GM::NCH - 1;
+    const uint32_t base_last = s << LOGE;
+    if constexpr (MODE == ROW_FWD) {
+        uint32_t *p = at_bytes(data, rowb + base_last * 4u);
+#pragma unroll
+        for (int rr = 0; rr < E; ++rr) p[rr] = x[rr];
+        return;
+    } else {
+        load_inv_tw<LOGE>(w, tw, (1u << (a + GM::lam0(LAST))) + (row << GM::lam0(LAST)) + s, a + GM::lam0(LAST),
+                          GM::ne(LAST));
+        if constexpr (MODE == ROW_SQUARE) {
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], x[rr]);
+        } else {
+            const uint32_t *z = at_bytes(zp, rowb + base_last * 4u);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], z[rr]);
+        }
+        static_for_down<0, GM::NCH>([&](auto cc) {
+            constexpr int ci = decltype(cc)::value;
+            constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
+            const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+            if constexpr (ci != LAST) {
+                load_inv_tw<LOGE>(w, tw, (1u << (a + lam0)) + (row << lam0) + hi, a + lam0, NE);
+                __syncthreads();
+                const uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+                for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+            }
+            inv_chunk<M, LOGE, NE>(md, x, w);
+            if constexpr (ci == 0) {
+                uint32_t *p = at_bytes(data, rowb + base * 4u);
+#pragma unroll
+                for (int rr = 0; rr < E; ++rr) p[rr << YL] = x[rr];
+            } else {
+                __syncthreads();
+                uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+                for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+            }
+        });
+    }
+}
+
+// LDS words of a row pass workgroup.
+__host__ __device__ inline size_t row_pass_lds_words(int logg, int loge, int ng) {
+    return (size_t)ng * (((size_t)1 << logg) + ((size_t)1 << (logg - loge)));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Column passes: levels 0..a-1 (a = LOGG) down NG adjacent columns c = blk NG + q of the
+// 2^a x C matrix (element (k, c) at k C + c).  Thread t -> column q = t mod NG, sub-transform
+// s = t / NG.  LDS: column q at q SQ, element k at k + (k >> LOGE), with the column stride SQ
+// = 2^a + 2^a / E + pad = 64 / NG (mod 64): lanes of a wavefront (NG consecutive columns by
+// 64 / NG consecutive sub-transforms) then fall in distinct banks in every chunk.
+
+__host__ __device__ inline uint32_t col_pass_stride(int logg, int loge, int ng) {
+    const uint32_t w = (1u << logg) + (1u << (logg - loge));
+    const uint32_t want = (ng >= 64) ? 1u : 64u / (uint32_t)ng;
+    return w + ((want - (w & 63u)) & 63u);
+}
+
+__host__ __device__ inline size_t col_pass_lds_words(int logg, int loge, int ng) {
+    return (size_t)ng * col_pass_stride(logg, loge, ng);
+}
+
+// Forward levels 0..a-1 of one prime: signed digits `src` -> residues `dst` (both N values).
+template <class M, int LOGE, int LOGG>
+__device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                             const uint32_t *__restrict__ tw, uint32_t C, int NG, unsigned blk,
+                                             uint32_t *lds) {
+    using GM = Geo<LOGG, LOGE>;
+    constexpr int E = 1 << LOGE;
+    const uint32_t t = threadIdx.x, q = t & (uint32_t)(NG - 1), s = t / (uint32_t)NG;
+    const uint32_t c = blk * (uint32_t)NG + q;
+    uint32_t *L = lds + q * col_pass_stride(LOGG, LOGE, NG);
+    uint32_t x[E], w[E];
+    static_for<0, GM::NCH>([&](auto cc) {
+        constexpr int ci = decltype(cc)::value;
+        constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
+        const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+        load_fwd_tw<LOGE>(w, tw, (1u << lam0) + hi, NE);
+        if constexpr (ci == 0) {
+            const uint32_t off = (base * C + c) * 4u;
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = to_res(*at_bytes(src + (size_t)((uint32_t)rr << YL) * C, off), md.p);
+        } else {
+            __syncthreads();
+            const uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+        }
+        fwd_chunk<M, LOGE, NE>(md, x, w);
+        if constexpr (ci == GM::NCH - 1) {
+            const uint32_t off = (base * C + c) * 4u;
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) *at_bytes(dst + (size_t)((uint32_t)rr << YL) * C, off) = x[rr];
+        } else {
+            __syncthreads();
+            uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+        }
+    });
+}
+
+// Inverse levels a-1..0 of one prime, in place on the residues `data`.
+template <class M, int LOGE, int LOGG>
+__device__ __forceinline__ void col_inv_pass(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ tw,
+                                             uint32_t C, int NG, unsigned blk, uint32_t *lds) {
+    using GM = Geo<LOGG, LOGE>;
+    constexpr int E = 1 << LOGE;
+    const uint32_t t = threadIdx.x, q = t & (uint32_t)(NG - 1), s = t / (uint32_t)NG;
+    const uint32_t c = blk * (uint32_t)NG + q;
+    uint32_t *L = lds + q * col_pass_stride(LOGG, LOGE, NG);
+    uint32_t x[E], w[E];
+    static_for_down<0, GM::NCH>([&](auto cc) {
+        constexpr int ci = decltype(cc)::value;
+        constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
+        const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+        load_inv_tw<LOGE>(w, tw, (1u << lam0) + hi, lam0, NE);
+        if constexpr (ci == GM::NCH - 1) {
+            const uint32_t off = (base * C + c) * 4u;
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = *at_bytes(data + (size_t)((uint32_t)rr << YL) * C, off);
+        } else {
+            __syncthreads();
+            const uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+        }
+        inv_chunk<M, LOGE, NE>(md, x, w);
+        if constexpr (ci == 0) {
+            const uint32_t off = (base * C + c) * 4u;
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) *at_bytes(data + (size_t)((uint32_t)rr << YL) * C, off) = x[rr];
+        } else {
+            __syncthreads();
+            uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+        }
+    });
+}
+
+// Instantiated shapes of the compile-time passes (host and device agree on these ranges).
+constexpr int SPEC_ROW_MIN = 6, SPEC_ROW_MAX = 14;   // row levels
+constexpr int SPEC_COL_MIN = 5, SPEC_COL_MAX = 11;   // column levels
+
+__host__ __device__ inline bool spec_row_ok(int logg, int loge) {
+    return logg >= SPEC_ROW_MIN && logg <= SPEC_ROW_MAX && (loge == 4 || loge == 5) && logg >= loge;
+}
+__host__ __device__ inline bool spec_col_ok(int logg, int loge, int ng) {
+    return logg >= SPEC_COL_MIN && logg <= SPEC_COL_MAX && loge >= 3 && loge <= 5 && logg >= loge && ng >= 8 &&
+           ng <= 64;
+}
+
+}  // namespace gfe

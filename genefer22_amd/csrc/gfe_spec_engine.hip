@@ -1,0 +1,105 @@
+// gfe_spec_engine.hip -- single-candidate kernels of the compile-time passes (gfe_pass.h):
+// grid (blocks, primes), one prime's residues per workgroup.
+#include <mutex>
+
+#include "gfe_pass.h"
+#include "gfe_spec.h"
+
+namespace gfe {
+
+template <class M, int LOGE, int LOGG>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    col_fwd_pass<M, LOGE, LOGG>(md, A.src, A.ws + prime * N, A.tw + prime * N, (uint32_t)(N >> LOGG), A.ng,
+                                xcd_block(blockIdx.x, gridDim.x), lds);
+}
+
+template <class M, int LOGE, int LOGG>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_inv(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    col_inv_pass<M, LOGE, LOGG>(md, A.ws + prime * N, A.tw + prime * N, (uint32_t)(N >> LOGG), A.ng,
+                                xcd_block(blockIdx.x, gridDim.x), lds);
+}
+
+template <class M, int LOGE, int LOGG>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_row_sq(RowArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const M md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t N = (size_t)1 << A.n;
+    row_pass<M, LOGE, LOGG, ROW_SQUARE>(md, A.ws + prime * N, A.zp + prime * N, A.tw + prime * N, A.a, A.ng,
+                                        blockIdx.x, lds);
+}
+
+template <int KIND, class M, int LOGE, int LOGG = SPEC_COL_MIN>
+static SpecColFn col_lookup(int logg) {
+    if constexpr (LOGG > SPEC_COL_MAX) {
+        return nullptr;
+    } else {
+        if (logg == LOGG) {
+            if constexpr (LOGG >= LOGE) return KIND == 0 ? ks_col_fwd<M, LOGE, LOGG> : ks_col_inv<M, LOGE, LOGG>;
+            else return nullptr;
+        }
+        return col_lookup<KIND, M, LOGE, LOGG + 1>(logg);
+    }
+}
+
+template <class M, int LOGE, int LOGG = SPEC_ROW_MIN>
+static SpecRowFn row_lookup(int logg) {
+    if constexpr (LOGG > SPEC_ROW_MAX) {
+        return nullptr;
+    } else {
+        if (logg == LOGG) return ks_row_sq<M, LOGE, LOGG>;
+        return row_lookup<M, LOGE, LOGG + 1>(logg);
+    }
+}
+
+template <int KIND, class M>
+static SpecColFn col_fn(int loge, int logg) {
+    switch (loge) {
+        case 3: return col_lookup<KIND, M, 3>(logg);
+        case 4: return col_lookup<KIND, M, 4>(logg);
+        case 5: return col_lookup<KIND, M, 5>(logg);
+        default: return nullptr;
+    }
+}
+
+SpecColFn spec_col_fwd(int is32, int loge, int logg) {
+    return is32 ? col_fn<0, ModU>(loge, logg) : col_fn<0, ModS>(loge, logg);
+}
+SpecColFn spec_col_inv(int is32, int loge, int logg) {
+    return is32 ? col_fn<1, ModU>(loge, logg) : col_fn<1, ModS>(loge, logg);
+}
+SpecRowFn spec_row_square(int is32, int loge, int logg) {
+    if (loge == 4) return is32 ? row_lookup<ModU, 4>(logg) : row_lookup<ModS, 4>(logg);
+    if (loge == 5) return is32 ? row_lookup<ModU, 5>(logg) : row_lookup<ModS, 5>(logg);
+    return nullptr;
+}
+
+void spec_set_lds_attributes() {
+    static std::once_flag once;
+    std::call_once(once, []() {
+        const int lds = 160 * 1024;
+        for (int is32 = 0; is32 < 2; ++is32)
+            for (int loge = 3; loge <= 5; ++loge) {
+                for (int g = SPEC_COL_MIN; g <= SPEC_COL_MAX; ++g) {
+                    if (SpecColFn f = spec_col_fwd(is32, loge, g))
+                        (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                    if (SpecColFn f = spec_col_inv(is32, loge, g))
+                        (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                }
+                for (int g = SPEC_ROW_MIN; g <= SPEC_ROW_MAX; ++g)
+                    if (SpecRowFn f = spec_row_square(is32, loge, g))
+                        (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            }
+        (void)hipGetLastError();
+    });
+}
+
+}  // namespace gfe

@@ -65,7 +65,8 @@ def test_batch_api_submit_step_poll(gfe):
 
 
 @pytest.mark.parametrize("plan", [(7, 4, 8, 4, 8, 1), (6, 3, 16, 4, 4, 0), (4, 4, 32, 3, 4, 1), (7, 4, 8, 4, 8, 3),
-                                  (6, 3, 16, 4, 4, 2), (6, 3, 16, 4, 4, 4), (7, 4, 8, 4, 8, 5)])
+                                  (6, 3, 16, 4, 4, 2), (6, 3, 16, 4, 4, 4), (7, 4, 8, 4, 8, 5),
+                                  (6, 3, 16, 4, 8, 8), (7, 4, 8, 4, 16, 8)])
 def test_batch_plan_variants(gfe, plan):
     """Batched squarings under fixed plans (incl. factored row twiddles) equal single runs."""
     bases = [1000000000, 1534, 30000000, 1999999266]

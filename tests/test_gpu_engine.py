@@ -92,8 +92,9 @@ def _plans(n):
             for rl in (3, 4, 5):
                 for cng in (1, 8, 32):
                     for rng in (1, 4):
-                        # bit 0 factored row twiddles, bit 1 fused columns, bit 2 pipelined columns
-                        for flags in (0, 1, 2, 3, 4, 5):
+                        # bit 0 factored row twiddles, bit 1 fused columns, bit 2 pipelined columns,
+                        # bit 3 compile-time column and row passes
+                        for flags in (0, 1, 2, 3, 4, 5, 8):
                             yield (a, cl, cng, rl, rng, flags)
 
 
@@ -130,10 +131,10 @@ def test_every_plan_variant(gfe, oracle, b, n):
         t.set_plan((n, 4, 1, 4, 1, 0))  # no row levels left
 
 
-@pytest.mark.parametrize("flags", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("flags", [1, 2, 3, 4, 5, 8])
 def test_multiplicand_plan_variants(gfe, oracle, flags):
-    """initMultiplicand + mul (row pass forward-only and product modes) under factored twiddles
-    and fused column passes."""
+    """initMultiplicand + mul (row pass forward-only and product modes) under factored twiddles,
+    fused column passes and compile-time passes."""
     b, n = 1000000000, 14
     t = gfe.Transform(b, n)
     t.set_plan((7, 4, 8, 4, 8, flags))
