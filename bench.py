@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plan", default=None, help="fixed n=22 pass plan a,cl,cng,rl,rng,flags (default: the library's)")
-    ap.add_argument("--tune", action="store_true", help="autotune the n=22 plan before timing")
+    ap.add_argument("--no-tune", action="store_true", help="time the library's default n=22 plan instead of the autotuned one")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): candidates are independent and shard "
@@ -182,7 +182,7 @@ def main():
 
     import genefer22_amd as G
     G.lib()
-    res = bench_n22(G, args.steps, args.warmup, args.plan, args.tune)
+    res = bench_n22(G, args.steps, args.warmup, args.plan, not args.no_tune)
     log(f"[bench] n=22: {res['ns_per_squaring']:.0f} ns/squaring; kernels {res['kernel_us']}")
 
     batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64)
@@ -216,7 +216,7 @@ def main():
         "data": "synthetic (seeded b; residue 2^k mod b^N+1 from set(1) and a fixed dup pattern)",
         "config": {
             "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] (NTT3, 31-bit primes)",
-            "n": 22, "b": res["b"], "plan": res["plan"], "kernel_us": res["kernel_us"],
+            "n": 22, "b": res["b"], "plan": res["plan"], "plan_tuple": res["plan_tuple"], "kernel_us": res["kernel_us"],
             "batched_n15": batched, "parity": parity,
         },
         "roofline": res["roofline"],

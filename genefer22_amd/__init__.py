@@ -99,6 +99,7 @@ ABI = [
     ("gfe_batch_time_rounds", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     ("gfe_batch_run", ctypes.c_int, [ctypes.c_uint32, U32P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]),
     ("gfe_batch_backlog", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_batch_set_graphs", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("gfe_get_plan", ctypes.c_int, [ctypes.c_void_p, I32P]),
     ("gfe_set_plan", ctypes.c_int, [ctypes.c_void_p, I32P]),
     ("gfe_batch_get_plan", ctypes.c_int, [ctypes.c_void_p, I32P]),
@@ -348,6 +349,10 @@ class Batch:
     def set_plan(self, plan) -> None:
         a = np.ascontiguousarray(plan, dtype=np.int32)
         _ck(lib().gfe_batch_set_plan(self._h, _i32(a)))
+
+    def set_graphs(self, on: bool) -> None:
+        """Squaring blocks as captured HIP graphs (default) or launched kernel by kernel."""
+        _ck(lib().gfe_batch_set_graphs(self._h, 1 if on else 0))
 
     def submit(self, b: int, iters: int = 0, tag: int = 0) -> int | None:
         """Slot of the new candidate, or None when the batch is full."""

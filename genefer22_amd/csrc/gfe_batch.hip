@@ -311,6 +311,17 @@ struct Batch {
     unsigned resident_blocks(const void *fn, int threads, size_t lds);
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     PrimeArgs prS, prU;
+    // Squaring blocks as HIP graphs: the kernels of `rounds` consecutive rounds (per-round flag rows
+    // act_dev / dup_dev + r cap, which the block's flag upload fills before the launch) captured
+    // once per (rounds, grid depth zl, plan) and replayed.  The cache is dropped when the plan
+    // changes.  Captured kernels record no memset (fill kernels only).
+    struct BlockGraph { int rounds; uint32_t zl; uint64_t key; hipGraphExec_t exec; };
+    std::vector<BlockGraph> graphs;
+    bool use_graphs = true;
+    uint64_t plan_key() const;
+    hipGraphExec_t block_graph(int rounds);
+    void drop_graphs();
+    int run_block(int rounds);
     std::vector<Slot> slots;
     std::deque<gfe_batch_result> finished;
     double t0 = 0;
@@ -354,10 +365,64 @@ int check_hip(hipError_t e, const char *what) {
 
 int Batch::check(const char *what) { return check_hip(hipGetLastError(), what); }
 
+uint64_t Batch::plan_key() const {
+    int32_t a[6];
+    plan_to_array(plan, a);
+    uint64_t k = 0;
+    for (int i = 0; i < 6; ++i) k = k * 1000003u + (uint32_t)a[i];
+    return k;
+}
+
+void Batch::drop_graphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
+// The instantiated graph of a block of `rounds` squaring rounds at the current grid depth and plan
+// (captured on first use; null on failure).
+hipGraphExec_t Batch::block_graph(int rounds) {
+    const uint64_t key = plan_key();
+    for (auto &g : graphs)
+        if (g.rounds == rounds && g.zl == zdim() && g.key == key) return g.exec;
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
+    if (hipStreamEndCapture(stream, &graph) != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(graphs.front().exec);
+        graphs.erase(graphs.begin());
+    }
+    graphs.push_back({rounds, zdim(), key, exec});
+    return exec;
+}
+
+// Enqueues `rounds` squaring rounds (flag rows already uploaded): one graph launch, or the
+// kernels one by one when graphs are off or the plan uses persistent grids (their sizes come from
+// occupancy queries, which stay out of captures).
+int Batch::run_block(int rounds) {
+    if (use_graphs && !plan.pipe) {
+        if (hipGraphExec_t g = block_graph(rounds)) return check_hip(hipGraphLaunch(g, stream), "graph launch");
+        return set_error(GFE_EHIP, "squaring block graph capture failed");
+    }
+    for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
+    return check("square block");
+}
+
 Batch::~Batch() {
     if (!ok) return;
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
     (void)hipFree(regs); (void)hipFree(ws); (void)hipFree(zp); (void)hipFree(tw); (void)hipFree(carry);
     (void)hipFree(crt); (void)hipFree(slots_dev); (void)hipFree(act_dev); (void)hipFree(dup_dev);
     for (int i = 0; i < NSTAGE; ++i) {
@@ -927,8 +992,7 @@ int Batch::step(int32_t *active_out) {
         if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(dup_dev, dup, (size_t)rounds * cap, hipMemcpyHostToDevice, stream), "dup");
         stage_release();
         if (rc != GFE_OK) return rc;
-        for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
-        rc = check("square block");
+        rc = run_block(rounds);
         if (rc != GFE_OK) return rc;
         if ((int)blk_q.size() == NBLK) {
             rc = check_hip(hipEventSynchronize(blk_ev[blk_q.front()]), "block sync");
@@ -1062,6 +1126,16 @@ int gfe_batch_set_plan(gfe_batch *b, const int32_t plan[6]) {
     if (hipSetDevice(b->bt.device) != hipSuccess || hipStreamSynchronize(b->bt.stream) != hipSuccess)
         return set_error(GFE_EHIP, "set_plan sync");
     b->bt.plan = P;
+    b->bt.drop_graphs();
+    return GFE_OK;
+}
+
+int gfe_batch_set_graphs(gfe_batch *b, int on) {
+    if (!b) return set_error(GFE_EINVAL, "null batch");
+    if (hipSetDevice(b->bt.device) != hipSuccess || hipStreamSynchronize(b->bt.stream) != hipSuccess)
+        return set_error(GFE_EHIP, "set_graphs sync");
+    b->bt.use_graphs = on != 0;
+    if (!on) b->bt.drop_graphs();
     return GFE_OK;
 }
 
@@ -1090,15 +1164,28 @@ int gfe_batch_time_rounds(gfe_batch *b, uint32_t rounds, double *ms) {
     if (!b || !ms || rounds == 0) return set_error(GFE_EINVAL, "time_rounds");
     Batch &B = b->bt;
     if (hipSetDevice(B.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
-    std::vector<uint8_t> f(B.cap, 0);
-    for (uint32_t s = 0; s < B.cap; ++s) f[s] = B.slots[s].busy ? 1 : 0;
+    // every round: the busy slots square without doubling (64 flag rows, as a squaring block)
     uint8_t *st = B.stage_acquire();
-    memcpy(st, f.data(), B.cap);
-    int rc = gfe::check_hip(hipMemcpyAsync(B.act_dev, st, B.cap, hipMemcpyHostToDevice, B.stream), "flags");
+    memset(st, 0, (size_t)2 * 64 * B.cap);
+    for (uint32_t r = 0; r < 64; ++r)
+        for (uint32_t s = 0; s < B.cap; ++s) st[(size_t)r * B.cap + s] = B.slots[s].busy ? 1 : 0;
+    int rc = gfe::check_hip(hipMemcpyAsync(B.act_dev, st, (size_t)64 * B.cap, hipMemcpyHostToDevice, B.stream), "act");
+    if (rc == GFE_OK)
+        rc = gfe::check_hip(hipMemcpyAsync(B.dup_dev, st + (size_t)64 * B.cap, (size_t)64 * B.cap, hipMemcpyHostToDevice, B.stream), "dup");
     B.stage_release();
     if (rc != GFE_OK) return rc;
+    // captured blocks are built before the timed region
+    if (B.use_graphs && !B.plan.pipe) {
+        if (rounds >= 64 && !B.block_graph(64)) return set_error(GFE_EHIP, "squaring block graph capture failed");
+        if ((rounds % 64) && !B.block_graph((int)(rounds % 64))) return set_error(GFE_EHIP, "squaring block graph capture failed");
+    }
     (void)hipEventRecord(B.ev0, B.stream);
-    for (uint32_t r = 0; r < rounds; ++r) B.square(B.act_dev, nullptr);
+    for (uint32_t done = 0; done < rounds;) {
+        const int r = (int)std::min<uint32_t>(64, rounds - done);
+        rc = B.run_block(r);
+        if (rc != GFE_OK) return rc;
+        done += (uint32_t)r;
+    }
     (void)hipEventRecord(B.ev1, B.stream);
     rc = gfe::check_hip(hipEventSynchronize(B.ev1), "time sync");
     if (rc != GFE_OK) return rc;
@@ -1125,6 +1212,8 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     const gfe::Plan orig = B.plan;
     gfe::Plan best = orig;
     double best_ms = 1e30;
+    const bool graphs = B.use_graphs;
+    B.use_graphs = false;  // plans are compared on eager launches (no capture per candidate plan)
     for (const gfe::Plan &P : plans) {
         B.plan = P;
         double ms = 0;
@@ -1133,6 +1222,8 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
         if (ms < best_ms) { best_ms = ms; best = P; }
     }
     B.plan = best;
+    B.use_graphs = graphs;
+    B.drop_graphs();
     (void)hipMemcpyAsync(B.regs, save, bytes, hipMemcpyDeviceToDevice, B.stream);
     (void)hipStreamSynchronize(B.stream);
     (void)hipFree(save);

@@ -182,7 +182,41 @@ CrtConst make_crt(int rns, int is32, uint32_t n) {
 // ------------------------------------------------------------------------------------------------
 // Plans
 
+static Plan runtime_default_plan(uint32_t n);
+
+// Default plan: the first admissible compile-time plan of a preference list drawn from timings on
+// MI355X (n = 22: 2^8-point columns of 16 points per thread, 2^14-point rows of 32 points per
+// thread; n = 15: 2^6 x 2^9 with 8 and 16 points per thread), else the runtime-geometry default.
 Plan default_plan(uint32_t n) {
+    const int nn = (int)n;
+    std::vector<int> as;
+    auto add_a = [&](int a) {
+        if (a >= SPEC_COL_MIN && a <= SPEC_COL_MAX && std::find(as.begin(), as.end(), a) == as.end()) as.push_back(a);
+    };
+    if (nn >= 20) add_a(nn - 14);
+    add_a(nn / 2 - 1);
+    add_a(nn / 2);
+    add_a((nn + 1) / 2);
+    for (int a = SPEC_COL_MIN; a <= SPEC_COL_MAX; ++a) add_a(a);
+    const int cols[][2] = {{4, 16}, {3, 8}, {3, 16}, {4, 32}, {5, 16}, {3, 32}, {4, 8}};
+    const int rows[][2] = {{5, 1}, {4, 2}, {4, 1}, {4, 4}, {4, 8}, {4, 16}, {5, 2}, {4, 32}, {4, 64}};
+    for (int a : as) {
+        const bool big_rows = (nn - a) >= 13;
+        for (const auto &r : rows) {
+            if (!big_rows && r[0] == 5) continue;
+            for (const auto &c : cols) {
+                Plan P;
+                P.a = a; P.col_loge = c[0]; P.col_ng = c[1]; P.row_loge = r[0]; P.row_ng = r[1]; P.norm_s = 8;
+                P.spec = 1;
+                if (a <= 6 && c[0] == 4) continue;  // short columns: 8 points per thread
+                if (plan_valid(P, n)) return P;
+            }
+        }
+    }
+    return runtime_default_plan(n);
+}
+
+static Plan runtime_default_plan(uint32_t n) {
     Plan P;
     P.a = (int)n / 2;
     if (n >= 22) P.a = (int)n - 12;

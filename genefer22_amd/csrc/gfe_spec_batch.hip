@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kbs_col_inv(BArgs A, i
 }
 
 template <int LOGE, int LOGG>
-__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kbs_row_sq(BArgs A) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024, LOGE >= 5 ? 4 : 1) kbs_row_sq(BArgs A) {
     extern __shared__ uint32_t lds[];
     const int s = bslot(A);
     if (s < 0) return;

@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_inv(ColArgs A) 
 }
 
 template <class M, int LOGE, int LOGG>
-__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_row_sq(RowArgs A) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024, LOGE >= 5 ? 4 : 1) ks_row_sq(RowArgs A) {
     extern __shared__ uint32_t lds[];
     const int prime = blockIdx.y;
     const M md{A.pr.p[prime], A.pr.q[prime]};

@@ -72,8 +72,15 @@ size_t gfe_batch_device_bytes(uint32_t n, uint32_t capacity);
 int gfe_batch_get_digits(gfe_batch *bt, int32_t slot, uint32_t reg, int32_t *out);
 
 /* Benchmark: times `rounds` squaring rounds (one squaring of every busy slot per round) with HIP
- * events, without advancing the slots' programs (their register 0 is squared in place). */
+ * events, without advancing the slots' programs (their register 0 is squared in place).  Rounds
+ * are enqueued as gfe_batch_step enqueues them (graph launches of up to 64 rounds when on). */
 int gfe_batch_time_rounds(gfe_batch *bt, uint32_t rounds, double *ms);
+
+/* Squaring blocks as HIP graphs (on by default): each block of up to 64 rounds is captured once
+ * per (rounds, slot range, plan) and replayed with one launch; off, its kernels are launched one by
+ * one.  Results are identical either way.  Replaces the reference's per-kernel enqueue loop
+ * (src/transformGPU.h:665-693, src/ocl.h:626-651) for batches. */
+int gfe_batch_set_graphs(gfe_batch *bt, int on);
 
 /* Runs quick tests (or their first `iters` squarings) of `count` candidates of one n to completion
  * in one batch of min(count, capacity) slots, refilling slots as candidates finish.

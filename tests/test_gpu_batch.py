@@ -82,3 +82,27 @@ def test_batch_plan_variants(gfe, plan):
     bt.close()
     for r in done:
         assert r["res64"] == gfe.run_prp(r["b"], 13, iters=200, b_gl=-1)["res64"], (plan, r["b"])
+
+
+def test_batch_graphs_equal_eager(gfe):
+    """Squaring blocks replayed as captured HIP graphs (hundreds of replays, with the quick test's
+    register operations launched eagerly between them) give the results of kernel-by-kernel
+    launches, and the full quick test equals the plain driver's."""
+    cases = [(1534, 0), (1000000000, 700), (1999999266, 700), (30000000, 700)]
+    out = {}
+    for graphs in (True, False):
+        bt = gfe.Batch(12, len(cases))
+        bt.set_graphs(graphs)
+        for i, (b, it) in enumerate(cases):
+            bt.submit(b, iters=it, tag=i)
+        done = []
+        while len(done) < len(cases):
+            bt.step()
+            done += bt.poll()
+        bt.close()
+        out[graphs] = sorted((r["tag"], r["res64"], r["old64"], r["hash64"]) for r in done)
+    assert out[True] == out[False]
+    full = gfe.run_quick(1534, 12)
+    assert out[True][0][1] == full["res64"]
+    for tag, (b, it) in enumerate(cases[1:], start=1):
+        assert out[True][tag][1] == gfe.run_prp(b, 12, iters=it, b_gl=-1)["res64"], b
