@@ -11,6 +11,8 @@ One JSON line on stdout (progress on stderr):
   * valu       : the same kernel's butterfly rate and estimated VALU-slot use (integer-bound path);
   * batched_n15: candidates/hour at n = 15, 512 seeded candidates in one batch vs one candidate per
                  launch, from timed iteration blocks (full-test length = esize squarings);
+  * serving    : p50/p99 time-to-result of seeded Poisson requests of mixed n (12..20) through the
+                 serving loop, and requests/hour against the serial plain path;
   * parity     : RES64 after P PRP iterations on the GPU vs the reference's own CPU transform;
   * cpu_baseline: the reference's CPU transform (compiled from its sources into oracle/_ref) timed on
                  this host's cores on a bounded sample of the same workload.
@@ -30,6 +32,23 @@ from genefer22_amd import workloads as W  # noqa: E402
 
 METRIC = "ns per squaring at n=22; candidates/hour at n=15 batched vs plain, bit-exact"
 HBM_PEAK_GBS = 8000.0
+# VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz, one wave-instruction per CU per clock
+# (integer multiplies, min and add3 issue at this rate on gfx950; 32-bit adds somewhat faster).
+VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9
+
+# Per-launch measurements of the dominant kernel for plans measured with rocprofv3 on MI355X
+# (profiles/n22_spec_plan_kernels.md): HBM bytes = FETCH_SIZE x 1.998 + WRITE_SIZE x 1.000 (the copy
+# kernel's calibration), and VALU instructions per radix-2 butterfly = SQ_INSTS_VALU x 64 / butterflies.
+# Keyed by the plan fields the kernel depends on: the row pass by (a, row_loge, row_ng, flags).
+MEASURED = {
+    ("row", 8, 5, 1, 8): {"traffic": int((144.9 + 48.0) * 2**20), "valu_instr_per_butterfly": 11.9},
+}
+
+
+def measured(kernel: str, plan) -> dict:
+    a, cl, cng, rl, rng, fl = plan
+    key = (kernel, a, rl, rng, fl) if kernel == "row" else (kernel, a, cl, cng, fl)
+    return MEASURED.get(key, {})
 
 
 def log(*a):
@@ -76,20 +95,23 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     levels = {0: a, 1: 2 * (n - a), 2: a}.get(dom, 0)
     bfly = rns * levels * (N // 2)
     bfly_rate = bfly / (km[dom] * 1e-3) if levels else 0.0
-    slots = 14.0  # VALU issue slots of one butterfly (2 half-rate mul_lo/mul_hi + 1 mad_u64 + 8 adds/mins)
-    valu_peak_slots = 256 * 128 * 2.4e9
+    meas = measured(names[dom], t.get_plan())
+    ipb = meas.get("valu_instr_per_butterfly")
     res = {
         "b": b, "plan": plan, "plan_tuple": list(t.get_plan()), "ms_per_step": ms / steps, "ns_per_squaring": ms / steps * 1e6,
         "kernel_us": {names[i]: round(km[i] * 1e3, 2) for i in range(4)},
         "roofline": {
             "bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": meas.get("traffic"),
             "alg_bytes_per_launch": alg_bytes,
+            "note": "the kernel is bound by integer VALU issue (see valu), not by HBM",
         },
         "valu": {
-            "kernel": names[dom], "butterflies_per_s": bfly_rate, "slots_per_butterfly": slots,
-            "achieved_slots_per_s": bfly_rate * slots, "peak_slots_per_s": valu_peak_slots,
-            "frac": round(bfly_rate * slots / valu_peak_slots, 4),
+            "kernel": names[dom], "butterflies": bfly, "butterflies_per_s": round(bfly_rate),
+            "valu_instr_per_butterfly": ipb,
+            "achieved_lane_instr_per_s": round(bfly_rate * ipb) if ipb else None,
+            "peak_lane_instr_per_s": VALU_PEAK_LANE_INSTR,
+            "frac": round(bfly_rate * ipb / VALU_PEAK_LANE_INSTR, 4) if ipb else None,
         },
     }
     t.close()
@@ -135,6 +157,38 @@ def bench_batched_n15(G, count: int, block: int):
     }
 
 
+def bench_serving(G, count: int, rate: float, iters: int):
+    """Serving: `count` seeded requests (n uniform in 12..20, b up to the reference's limit per n,
+    Poisson arrivals at `rate`/s, the first `iters` PRP squarings each) replayed on the wall clock
+    through the native serving loop; p50/p99 time-to-result and requests/hour, against the same
+    requests run one after another on the plain one-candidate path (per-n ms/squaring measured on
+    the library's default plans)."""
+    from genefer22_amd import serve
+    reqs = W.serving_requests(count, rate, iters=iters)
+    t0 = time.perf_counter()
+    _, summary, stats = serve.run(reqs)
+    wall = time.perf_counter() - t0
+    ns = sorted({q.n for q in reqs})
+    b_of_n = {q.n: q.b for q in reqs}
+    ms_by_n = {}
+    for n in ns:
+        t = G.Transform(b_of_n[n], n)
+        t.set(1)
+        t.time_squarings(32)
+        ms_by_n[n] = t.time_squarings(256) / 256
+        t.close()
+    serial = serve.plain_serial_seconds(reqs, ms_by_n)
+    return {
+        "requests": count, "arrival_rate_per_s": rate, "squarings_per_request": iters,
+        "p50_s": summary["p50_s"], "p99_s": summary["p99_s"], "makespan_s": summary["makespan_s"],
+        "requests_per_hour": summary["requests_per_hour"],
+        "plain_serial_requests_per_hour": round(count / serial * 3600.0, 1),
+        "speedup_vs_plain_serial": round(serial / summary["makespan_s"], 2),
+        "plain_ms_per_squaring_by_n": {int(k): round(v, 5) for k, v in ms_by_n.items()},
+        "pools_created": stats["pools_created"], "bytes_peak": stats["bytes_peak"], "wall_s": round(wall, 2),
+    }
+
+
 def parity_n22(G, O, b: int, iters: int, threads: int):
     """RES64 after `iters` PRP iterations: GPU driver vs the reference CPU transform."""
     g = G.run_prp(b, 22, iters=iters, b_gl=-1)
@@ -171,6 +225,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the n=15 batched leg")
     ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plan", default=None, help="fixed n=22 pass plan a,cl,cng,rl,rng,flags (default: the library's)")
     ap.add_argument("--no-tune", action="store_true", help="time the library's default n=22 plan instead of the autotuned one")
@@ -188,6 +245,10 @@ def main():
     batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64)
     if batched:
         log(f"[bench] n=15: {batched}")
+
+    serving = None if args.no_serve else bench_serving(G, args.serve_requests, args.serve_rate, 4096)
+    if serving:
+        log(f"[bench] serving: {serving}")
 
     threads = min(16, os.cpu_count() or 1)
     parity = cpu = None
@@ -217,7 +278,7 @@ def main():
         "config": {
             "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] (NTT3, 31-bit primes)",
             "n": 22, "b": res["b"], "plan": res["plan"], "plan_tuple": res["plan_tuple"], "kernel_us": res["kernel_us"],
-            "batched_n15": batched, "parity": parity,
+            "batched_n15": batched, "serving": serving, "parity": parity,
         },
         "roofline": res["roofline"],
         "valu": res["valu"],
