@@ -29,6 +29,7 @@
 #include "gfe_batch_dev.h"
 #include "gfe_engine.h"
 #include "gfe_spec.h"
+#include "gfe_fused.h"
 #include "genefer.h"
 #include "gint.h"
 
@@ -410,6 +411,15 @@ hipGraphExec_t Batch::block_graph(int rounds) {
 // kernels one by one when graphs are off or the plan uses persistent grids (their sizes come from
 // occupancy queries, which stay out of captures).
 int Batch::run_block(int rounds) {
+    if (plan.fuse_sq) {  // one launch runs the whole block, one workgroup per slot
+        FusedArgs A;
+        memset(&A, 0, sizeof(A));
+        A.regs = regs; A.tw = tw; A.crt = crt; A.slots = slots_dev; A.act = act_dev; A.dupf = dup_dev;
+        A.reg_stride = (long long)(3 * N); A.stride = cap; A.rounds = rounds; A.prS = prS; A.prU = prU;
+        hipLaunchKernelGGL(fused_square_fn((int)n), dim3(zdim()), dim3(fused_threads((int)n)), fused_lds_bytes((int)n),
+                           stream, A);
+        return check("fused square block");
+    }
     if (use_graphs && !plan.pipe) {
         if (hipGraphExec_t g = block_graph(rounds)) return check_hip(hipGraphLaunch(g, stream), "graph launch");
         return set_error(GFE_EHIP, "squaring block graph capture failed");
@@ -465,7 +475,7 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     ok = true;
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
-    plan = default_plan(n);
+    plan = default_batch_plan(n, cap);
     if (hipMalloc(&regs, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)cap * 3 * N * 4) != hipSuccess ||
         hipMalloc(&zp, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&tw, (size_t)6 * N * 4) != hipSuccess ||
         hipMalloc(&carry, (size_t)cap * (N >> 3) * 8) != hipSuccess || hipMalloc(&crt, (size_t)cap * sizeof(CrtConst)) != hipSuccess ||
@@ -513,6 +523,7 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     set_row_attrs<0>();
     set_row_attrs<1>();
     spec_batch_set_lds_attributes();
+    fused_set_lds_attributes();
     set_lds_attr((const void *)kb_col_fwd_p<3>); set_lds_attr((const void *)kb_col_fwd_p<4>); set_lds_attr((const void *)kb_col_fwd_p<5>);
     set_lds_attr((const void *)kb_col_inv_p<3>); set_lds_attr((const void *)kb_col_inv_p<4>); set_lds_attr((const void *)kb_col_inv_p<5>);
     set_lds_attr((const void *)kb_col_fwd_all<3>); set_lds_attr((const void *)kb_col_fwd_all<4>);

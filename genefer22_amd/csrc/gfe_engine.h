@@ -12,6 +12,7 @@
 #include "gfe_kernels.h"
 #include "gfe_pass.h"
 #include "gfe_spec.h"
+#include "gfe_fused.h"
 
 namespace gfe {
 
@@ -24,9 +25,12 @@ struct Plan {
     int fused = 0;   // 1: all-prime column passes with the CRT and carry in the inverse one
     int pipe = 0;    // 1: persistent column and normalisation passes that prefetch the next block
     int spec = 0;    // 1: column passes and the squaring row pass with compile-time geometry (gfe_pass.h)
+    int fuse_sq = 0; // 1: the squaring in one workgroup per candidate (gfe_fused.hip, n <= 13); the
+                     //    other fields still shape the multiplicand passes
 };
 
 Plan default_plan(uint32_t n);
+Plan default_batch_plan(uint32_t n, uint32_t capacity);
 bool plan_valid(const Plan &P, uint32_t n);
 std::string plan_str(const Plan &P, uint32_t n);
 void plan_to_array(const Plan &P, int32_t out[6]);
@@ -71,6 +75,7 @@ struct Engine {
     void launch_norm(int dup);
     void launch_carry();
     void square(int dup);
+    void launch_fused(int dup);
     void mul();
     void init_multiplicand(uint32_t src);
     void mul_scalar(int32_t a);

@@ -216,6 +216,20 @@ Plan default_plan(uint32_t n) {
     return runtime_default_plan(n);
 }
 
+// Batches of small candidates square each candidate in one workgroup (one launch per block of
+// rounds): on MI355X that is 1.4x-1.9x the four-step passes' throughput at n = 12-13 once the
+// batch holds 64 or more candidates, while a lone candidate is faster spread over the four-step
+// passes' many workgroups.
+Plan default_batch_plan(uint32_t n, uint32_t capacity) {
+    Plan P = default_plan(n);
+    if (capacity >= 32) {
+        Plan F = P;
+        F.fuse_sq = 1;
+        if (plan_valid(F, n)) return F;
+    }
+    return P;
+}
+
 static Plan runtime_default_plan(uint32_t n) {
     Plan P;
     P.a = (int)n / 2;
@@ -234,6 +248,12 @@ static Plan runtime_default_plan(uint32_t n) {
 }
 
 bool plan_valid(const Plan &P, uint32_t n) {
+    if (P.fuse_sq) {
+        if ((int)n < FUSED_MIN_N || (int)n > FUSED_MAX_N || P.fused || P.pipe) return false;
+        Plan Q = P;
+        Q.fuse_sq = 0;
+        return plan_valid(Q, n);
+    }
     const int gc = P.a, gr = (int)n - P.a;
     if (P.spec) {
         if (P.fused || P.pipe || P.row_tw) return false;
@@ -292,20 +312,27 @@ std::vector<Plan> candidate_plans(uint32_t n) {
                                 const int Tc = cng * (1 << a) / (1 << cl), Tr = rng * (1 << (n - a)) / (1 << rl);
                                 if (Tc < 64 || Tr < 64 || Tc > 1024 || Tr > 1024) continue;
                                 if (plan_valid(P, n)) plans.push_back(P);
+                                if (P.spec && (int)n >= FUSED_MIN_N && (int)n <= FUSED_MAX_N) {
+                                    Plan F = P;
+                                    F.fuse_sq = 1;
+                                    if (plan_valid(F, n)) plans.push_back(F);
+                                }
                             }
     return plans;
 }
 
 void plan_to_array(const Plan &P, int32_t out[6]) {
     out[0] = P.a; out[1] = P.col_loge; out[2] = P.col_ng; out[3] = P.row_loge; out[4] = P.row_ng;
-    out[5] = P.row_tw | (P.fused << 1) | (P.pipe << 2) | (P.spec << 3);
+    out[5] = P.row_tw | (P.fused << 1) | (P.pipe << 2) | (P.spec << 3) | (P.fuse_sq << 4);
 }
 
 bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out) {
     Plan P;
-    if (in[5] < 0 || in[5] > 15 || (in[5] & 6) == 6 || ((in[5] & 8) && in[5] != 8)) return false;
+    const int fl = in[5] & 15;
+    if (in[5] < 0 || in[5] > 31 || (fl & 6) == 6 || ((fl & 8) && fl != 8)) return false;
     P.a = in[0]; P.col_loge = in[1]; P.col_ng = in[2]; P.row_loge = in[3]; P.row_ng = in[4];
-    P.row_tw = in[5] & 1; P.fused = (in[5] >> 1) & 1; P.pipe = (in[5] >> 2) & 1; P.spec = (in[5] >> 3) & 1;
+    P.row_tw = fl & 1; P.fused = (fl >> 1) & 1; P.pipe = (fl >> 2) & 1; P.spec = (fl >> 3) & 1;
+    P.fuse_sq = (in[5] >> 4) & 1;
     P.norm_s = 8;
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
     if (P.a < 1 || P.a >= (int)n || P.col_loge < 3 || P.col_loge > 5 || P.row_loge < 3 || P.row_loge > 5) return false;
@@ -320,7 +347,8 @@ std::string plan_str(const Plan &P, uint32_t n) {
     ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (n - P.a)
        << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg" << (P.row_tw ? ", factored twiddles" : "")
        << "), " << (P.fused ? "all-prime columns with CRT + carry" : "norm 8 digits/thread")
-       << (P.pipe ? ", pipelined columns and norm" : "") << (P.spec ? ", compile-time passes" : "");
+       << (P.pipe ? ", pipelined columns and norm" : "") << (P.spec ? ", compile-time passes" : "")
+       << (P.fuse_sq ? ", squaring in one workgroup" : "");
     return ss.str();
 }
 
@@ -464,6 +492,7 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     if (hipSetDevice(device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice failed");
     set_lds_attributes();
     spec_set_lds_attributes();
+    fused_set_lds_attributes();
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
     plan = default_plan(n);
     rounds = carry_rounds(b, n);
@@ -642,7 +671,19 @@ void Engine::launch_carry() {
     hipLaunchKernelGGL(k_carry_wg, dim3((A.nwg + 255) / 256, 1, 1), dim3(256), 0, stream, A);
 }
 
+void Engine::launch_fused(int dup) {
+    FusedArgs A;
+    memset(&A, 0, sizeof(A));
+    A.regs = regs; A.tw = tw; A.crt = crt; A.slots = nullptr; A.act = nullptr; A.dupf = nullptr;
+    A.rounds = 1; A.dup0 = dup; A.b = b; A.rns = rns; A.is32 = is32; A.prS = pr; A.prU = pr;
+    hipLaunchKernelGGL(fused_square_fn((int)n), dim3(1), dim3(fused_threads((int)n)), fused_lds_bytes((int)n), stream, A);
+}
+
 void Engine::square(int dup) {
+    if (plan.fuse_sq) {
+        launch_fused(dup);
+        return;
+    }
     launch_col_fwd(0, ws);
     launch_row(ROW_SQUARE, ws);
     launch_col_inv(dup);
@@ -893,7 +934,19 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
     float t = 0;
     (void)hipEventElapsedTime(&t, e0, e1);
     *ms = t;
-    if (kernel_ms) {
+    if (kernel_ms && g.plan.fuse_sq) {  // one kernel per squaring
+        double acc = 0;
+        for (uint32_t i = 0; i < count; ++i) {
+            (void)hipEventRecord(g.events[2], g.stream);
+            g.launch_fused((dup_bits >> (i % 64)) & 1);
+            (void)hipEventRecord(g.events[3], g.stream);
+            if (hipEventSynchronize(g.events[3]) != hipSuccess) return set_error(GFE_EHIP, "event sync");
+            float d = 0;
+            (void)hipEventElapsedTime(&d, g.events[2], g.events[3]);
+            acc += d;
+        }
+        for (int k = 0; k < 5; ++k) kernel_ms[k] = (k == 0) ? acc / count : 0.0;
+    } else if (kernel_ms) {
         double acc[5] = {0, 0, 0, 0, 0};
         const int nk = 4;
         for (uint32_t i = 0; i < count; ++i) {

@@ -66,7 +66,7 @@ def test_batch_api_submit_step_poll(gfe):
 
 @pytest.mark.parametrize("plan", [(7, 4, 8, 4, 8, 1), (6, 3, 16, 4, 4, 0), (4, 4, 32, 3, 4, 1), (7, 4, 8, 4, 8, 3),
                                   (6, 3, 16, 4, 4, 2), (6, 3, 16, 4, 4, 4), (7, 4, 8, 4, 8, 5),
-                                  (6, 3, 16, 4, 8, 8), (7, 4, 8, 4, 16, 8)])
+                                  (6, 3, 16, 4, 8, 8), (7, 4, 8, 4, 16, 8), (6, 3, 16, 4, 8, 24)])
 def test_batch_plan_variants(gfe, plan):
     """Batched squarings under fixed plans (incl. factored row twiddles) equal single runs."""
     bases = [1000000000, 1534, 30000000, 1999999266]
@@ -106,3 +106,24 @@ def test_batch_graphs_equal_eager(gfe):
     assert out[True][0][1] == full["res64"]
     for tag, (b, it) in enumerate(cases[1:], start=1):
         assert out[True][tag][1] == gfe.run_prp(b, 12, iters=it, b_gl=-1)["res64"], b
+
+
+def test_batch_default_small_n_squares_in_one_workgroup(gfe):
+    """A batch of 40 candidates at n = 12 defaults to the single-workgroup squaring (plan bit 16:
+    one launch per block of rounds, one workgroup per slot); its residues equal single runs on the
+    four-step engine."""
+    from genefer22_amd import workloads
+    bases = workloads.seeded_bases(1212, 1024, 2_000_000_000, 40)
+    bt = gfe.Batch(12, 40)
+    assert bt.get_plan()[5] & 16
+    for i, b in enumerate(bases):
+        bt.submit(b, iters=300, tag=i)
+    done = []
+    while len(done) < len(bases):
+        bt.step()
+        done += bt.poll()
+    bt.close()
+    by_tag = {r["tag"]: r for r in done}
+    for i in (0, 7, 19, 33, 39):
+        alone = gfe.run_prp(bases[i], 12, iters=300, b_gl=-1)
+        assert (by_tag[i]["res64"], by_tag[i]["old64"]) == (alone["res64"], alone["old64"]), bases[i]

@@ -93,8 +93,8 @@ def _plans(n):
                 for cng in (1, 8, 32):
                     for rng in (1, 4):
                         # bit 0 factored row twiddles, bit 1 fused columns, bit 2 pipelined columns,
-                        # bit 3 compile-time column and row passes
-                        for flags in (0, 1, 2, 3, 4, 5, 8):
+                        # bit 3 compile-time column and row passes, bit 4 squaring in one workgroup
+                        for flags in (0, 1, 2, 3, 4, 5, 8, 24):
                             yield (a, cl, cng, rl, rng, flags)
 
 
@@ -131,13 +131,14 @@ def test_every_plan_variant(gfe, oracle, b, n):
         t.set_plan((n, 4, 1, 4, 1, 0))  # no row levels left
 
 
-@pytest.mark.parametrize("flags", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("flags", [1, 2, 3, 4, 5, 8, 24])
 def test_multiplicand_plan_variants(gfe, oracle, flags):
     """initMultiplicand + mul (row pass forward-only and product modes) under factored twiddles,
     fused column passes and compile-time passes."""
-    b, n = 1000000000, 14
+    # the single-workgroup squaring (bit 4) is for n <= 13
+    b, n, shape = (1000000000, 13, (6, 3, 16, 4, 8)) if flags & 16 else (1000000000, 14, (7, 4, 8, 4, 8))
     t = gfe.Transform(b, n)
-    t.set_plan((7, 4, 8, 4, 8, flags))
+    t.set_plan(shape + (flags,))
     o = oracle.Engine(b, n)
     t.set(3); o.set(3)
     for _ in range(5):
