@@ -820,7 +820,7 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
     // slot table and CRT constants; r0 = 1, r1 = 1 (prp: set(1); copy(1, 0))
     SlotInfo &si = slot_host[s];
     si.b = b; si.rns = S.rns; si.is32 = S.is32; si.rounds = carry_rounds(b, n);
-    crt_host[s] = make_crt(S.rns, S.is32, n);
+    crt_host[s] = make_crt(S.rns, S.is32, n, b);
     zl = std::max(zl, (uint32_t)s + 1);
     int rc = check_hip(hipMemcpyAsync(slots_dev + s, &si, sizeof(si), hipMemcpyHostToDevice, stream), "slot");
     if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(crt + s, crt_host + s, sizeof(CrtConst), hipMemcpyHostToDevice, stream), "crt");

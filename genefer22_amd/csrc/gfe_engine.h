@@ -40,7 +40,7 @@ bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out);
 void select_primes(uint32_t b, uint32_t n, int *rns, int *is32);
 uint32_t prime_of(int is32, int k);
 uint32_t mulw_const(uint32_t w, uint32_t p);
-CrtConst make_crt(int rns, int is32, uint32_t n);
+CrtConst make_crt(int rns, int is32, uint32_t n, uint32_t b = 0);
 int carry_rounds(uint32_t b, uint32_t n);
 int set_error(int code, const std::string &msg);
 extern thread_local std::string g_last_error;

@@ -158,7 +158,7 @@ int carry_rounds(uint32_t b, uint32_t n) {
     return r;
 }
 
-CrtConst make_crt(int rns, int is32, uint32_t n) {
+CrtConst make_crt(int rns, int is32, uint32_t n, uint32_t b) {
     CrtConst c;
     memset(&c, 0, sizeof(c));
     for (int k = 0; k < 3; ++k) {
@@ -175,6 +175,14 @@ CrtConst make_crt(int rns, int is32, uint32_t n) {
     c.inv_p12_m3 = mulw_const(hinv(hmul(p1 % p3, p2 % p3, p3), p3), p3);
     c.p12 = (uint64_t)p1 * p2;
     c.p12_d = (double)c.p12;
+    for (int k = 0; k < 3; ++k) c.scale_q[k] = c.scale[k] * c.q[k];
+    c.inv_p1_m2_q = c.inv_p1_m2 * c.q[1];
+    c.p1_m3_q = c.p1_m3 * c.q[2];
+    c.inv_p12_m3_q = c.inv_p12_m3 * c.q[2];
+    c.p12_lo = (uint32_t)c.p12;
+    c.p1_d = (double)p1;
+    // one rounded quotient is within 1 of the true one when N b < 2^49 (crt_digit_fast)
+    c.fast = (b != 0 && ((uint64_t)b << n) < (1ull << 49)) ? 1 : 0;
     (void)rns;
     return c;
 }
@@ -512,7 +520,7 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
         if (hipMemcpy(tw + (size_t)k * N, htw.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess)
             return set_error(GFE_EHIP, "twiddle upload");
     }
-    const CrtConst c = make_crt(rns, is32, n);
+    const CrtConst c = make_crt(rns, is32, n, b);
     if (hipMemcpy(crt, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "crt upload");
     if (hipMemcpy(bdev, &b, 4, hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "b upload");
     for (int k = 0; k < 3; ++k) { pr.p[k] = prime_of(is32, k); pr.q[k] = pinv32(pr.p[k]); }

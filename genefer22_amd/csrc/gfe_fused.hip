@@ -121,21 +121,36 @@ __device__ __forceinline__ void fused_crt(const CrtConst &c, const uint32_t *res
     const uint32_t o = t * (FE + 1);
     int64_t cr = 0;
     // digits in runs of 8 (a rolled loop keeps one run's CRT values live at a time)
+    if (c.fast) {
+        double cd = 0.0;
+        uint32_t cl = 0;
 #pragma unroll 1
-    for (int h = 0; h < FE; h += 8) {
+        for (int h = 0; h < FE; h += 8) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const uint32_t k = o + h + j;
-            const uint32_t r1 = m1.mulw(res[k], c.scale[0]);
-            const uint32_t r2 = m2.mulw(res[RS + k], c.scale[1]);
-            const uint32_t r3 = (RNS == 3) ? m3.mulw(res[2 * RS + k], c.scale[2]) : 0u;
-            uint64_t x64;
-            double xd;
-            garner<RNS>(c, m1, m2, m3, r1, r2, r3, &x64, &xd);
-            if (dup) { x64 <<= 1; xd *= 2.0; }
-            x64 += (uint64_t)cr;
-            xd += (double)cr;
-            dig[k] = div_bal(x64, xd, b, binv, &cr);
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t k = o + h + j;
+                dig[k] = crt_digit_fast<RNS>(c, m1, m2, m3, res[k], res[RS + k], (RNS == 3) ? res[2 * RS + k] : 0u, dup,
+                                             b, binv, cd, cl);
+            }
+        }
+        cr = carry_of(cd);
+    } else {
+#pragma unroll 1
+        for (int h = 0; h < FE; h += 8) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t k = o + h + j;
+                const uint32_t r1 = m1.mulw(res[k], c.scale[0]);
+                const uint32_t r2 = m2.mulw(res[RS + k], c.scale[1]);
+                const uint32_t r3 = (RNS == 3) ? m3.mulw(res[2 * RS + k], c.scale[2]) : 0u;
+                uint64_t x64;
+                double xd;
+                garner<RNS>(c, m1, m2, m3, r1, r2, r3, &x64, &xd);
+                if (dup) { x64 <<= 1; xd *= 2.0; }
+                x64 += (uint64_t)cr;
+                xd += (double)cr;
+                dig[k] = div_bal(x64, xd, b, binv, &cr);
+            }
         }
     }
     cs[t] = cr;

@@ -641,17 +641,27 @@ __device__ __forceinline__ void norm_body(const CrtConst &c, const uint32_t *__r
     }
     int32_t d[S];
     int64_t cr = 0;
+    if (c.fast) {
+        double cd = 0.0;
+        uint32_t cl = 0;
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-        const uint32_t r1 = m1.mulw(a[0][j], c.scale[0]);
-        const uint32_t r2 = m2.mulw(a[1][j], c.scale[1]);
-        const uint32_t r3 = (RNS == 3) ? m3.mulw(a[2][j], c.scale[2]) : 0u;
-        uint64_t x64; double xd;
-        garner<RNS>(c, m1, m2, m3, r1, r2, r3, &x64, &xd);
-        if (dup) { x64 <<= 1; xd *= 2.0; }
-        x64 += (uint64_t)cr;
-        xd += (double)cr;
-        d[j] = div_bal(x64, xd, b, binv, &cr);
+        for (int j = 0; j < S; ++j)
+            d[j] = crt_digit_fast<RNS>(c, m1, m2, m3, a[0][j], a[1][j], (RNS == 3) ? a[RNS - 1][j] : 0u, dup, b, binv,
+                                       cd, cl);
+        cr = carry_of(cd);
+    } else {
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const uint32_t r1 = m1.mulw(a[0][j], c.scale[0]);
+            const uint32_t r2 = m2.mulw(a[1][j], c.scale[1]);
+            const uint32_t r3 = (RNS == 3) ? m3.mulw(a[RNS - 1][j], c.scale[2]) : 0u;
+            uint64_t x64; double xd;
+            garner<RNS>(c, m1, m2, m3, r1, r2, r3, &x64, &xd);
+            if (dup) { x64 <<= 1; xd *= 2.0; }
+            x64 += (uint64_t)cr;
+            xd += (double)cr;
+            d[j] = div_bal(x64, xd, b, binv, &cr);
+        }
     }
     carry_tail<S>(d, cr, dst, i0, wc, blk, b, binv);
 }
