@@ -86,6 +86,8 @@ ABI = [
     ("gfe_run_prp_trace", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, U64P, U64P, ctypes.POINTER(Result)]),
     ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),
     ("gfe_residue", ctypes.c_int, [I32P, ctypes.c_uint32, ctypes.c_uint32, I32P, ctypes.POINTER(Result)]),
+    ("gfe_write_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                       ctypes.c_int, U64P, ctypes.POINTER(Result)]),
     ("gfe_batch_create", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("gfe_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("gfe_batch_slot_bytes", ctypes.c_size_t, [ctypes.c_uint32]),
@@ -306,6 +308,38 @@ def run_proof(b: int, n: int, depth: int, device: int = 0) -> dict:
     out = r.as_dict()
     out["mu_hash64"] = list(mu)
     return out
+
+
+def write_proof(b: int, n: int, depth: int, main_path: str, checkpoints: bool = False, device: int = 0) -> dict:
+    """proof() writing the reference's files: `<main_path>.proof` and, with `checkpoints`, the
+    checkpoint files `<main_path>_<i>.ckpt` (src/genefer.h:536-546, 620-734)."""
+    r = Result()
+    mu = (ctypes.c_uint64 * (depth + 1))()
+    _ck(lib().gfe_write_proof(b, n, device, depth, main_path.encode(), 1 if checkpoints else 0, mu, ctypes.byref(r)))
+    out = r.as_dict()
+    out["mu_hash64"] = list(mu)
+    return out
+
+
+def read_proof_file(path: str) -> dict:
+    """Parses a proof file of the reference's layout (src/genefer.h:629-717): version, depth, the
+    mu_k residue records (uint32 size, uint32 base, size int32 digits) and the CRC-32 trailer,
+    which is checked (~crc32 ^ 0xa23777ac over the preceding bytes, src/file.h:150-161)."""
+    import zlib
+    data = open(path, "rb").read()
+    body, trailer = data[:-4], int.from_bytes(data[-4:], "little")
+    if trailer != ((~zlib.crc32(body)) & 0xFFFFFFFF) ^ 0xA23777AC:
+        raise GfeError(f"{path}: bad file (crc32)")
+    version, depth = np.frombuffer(body[:8], dtype=np.int32)
+    mus, off = [], 8
+    for _ in range(int(depth) + 1):
+        size, base = np.frombuffer(body[off:off + 8], dtype=np.uint32)
+        off += 8
+        mus.append((int(base), np.frombuffer(body[off:off + 4 * int(size)], dtype=np.int32).copy()))
+        off += 4 * int(size)
+    if off != len(body):
+        raise GfeError(f"{path}: trailing bytes")
+    return {"version": int(version), "depth": int(depth), "mus": mus}
 
 
 class Batch:

@@ -186,6 +186,14 @@ public:
         return _g.gethash64();
     }
 
+    // Canonical digits of Pietrzak-Li checkpoint i (for the checkpoint files of the reference's
+    // proof mode, src/genefer.h:536-546).
+    std::vector<int32_t> ckpt_digits(size_t i, bool in_regs) {
+        load_ckpt(i, in_regs);
+        _g.unbalance();
+        return std::vector<int32_t>(_g.data(), _g.data() + _g.getSize());
+    }
+
 private:
     std::unique_ptr<transformHIP> _t;
     gint _g;

@@ -139,6 +139,65 @@ int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_ha
     }
 }
 
+// A residue record of the reference's files (gint::write, src/gint.h:160-167): uint32 size, uint32
+// base, size int32 canonical digits.
+static bool write_gint_record(gfe::file &f, const std::vector<int32_t> &canon, uint32_t b) {
+    const uint32_t size = (uint32_t)canon.size();
+    return f.write(reinterpret_cast<const char *>(&size), sizeof(size)) &&
+           f.write(reinterpret_cast<const char *>(&b), sizeof(b)) &&
+           f.write(reinterpret_cast<const char *>(canon.data()), canon.size() * sizeof(int32_t));
+}
+
+int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int write_ckpts,
+                    uint64_t *mu_hash64, gfe_result *out) {
+    if (!out || !main_path || depth < 1 || depth > 12) return gfe::set_error(GFE_EINVAL, "proof arguments");
+    memset(out, 0, sizeof(*out));
+    try {
+        const double t0 = now_s();
+        const bool fast = (n <= 17);  // src/genefer.h:1377-1385
+        gfe::genefer G(b, n, device, fast ? 3 + (size_t(1) << depth) : 3);
+        const size_t es = G.esize();
+        const int B_GL = gfe::genefer::B_GerbiczLi(es), B_PL = gfe::genefer::B_PietrzakLi(es, depth);
+        G.prp(B_GL, B_PL, fast);
+        G.t().getInt(G.gi());
+        fill_residue(G.gi(), out);
+        out->esize = es;
+        out->iters = es;
+        out->gl_ok = G.GL(B_GL) ? 1 : 0;
+        const std::string main(main_path);
+        if (write_ckpts) {  // <main>_<i>.ckpt: one residue record and the CRC (src/genefer.h:140-144, 536-546)
+            for (size_t i = 0; i <= (es - 1) / (size_t)B_PL; ++i) {
+                gfe::file f(main + "_" + std::to_string(i) + ".ckpt", "wb");
+                if (!f.exists() || !write_gint_record(f, G.ckpt_digits(i, fast), b))
+                    return gfe::set_error(GFE_EINVAL, "cannot write checkpoint file");
+                f.write_crc32();
+            }
+        }
+        std::vector<std::vector<int32_t>> mus;
+        out->pkey = G.PL(depth, fast, mus);
+        // <main>.proof: int version = 1, int depth, mu_0 .. mu_depth, CRC (src/genefer.h:629-717)
+        gfe::file pf(main + ".proof", "wb");
+        const int version = 1;
+        if (!pf.exists() || !pf.write(reinterpret_cast<const char *>(&version), sizeof(version)) ||
+            !pf.write(reinterpret_cast<const char *>(&depth), sizeof(depth)))
+            return gfe::set_error(GFE_EINVAL, "cannot write proof file");
+        gfe::gint g(size_t(1) << n, b);
+        for (size_t k = 0; k < mus.size(); ++k) {
+            if (!write_gint_record(pf, mus[k], b)) return gfe::set_error(GFE_EINVAL, "cannot write proof file");
+            if (mu_hash64) {
+                std::copy(mus[k].begin(), mus[k].end(), g.data());
+                g.reset();
+                mu_hash64[k] = g.gethash64();
+            }
+        }
+        pf.write_crc32();
+        out->seconds = now_s() - t0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EHIP, ex.what());
+    }
+}
+
 int gfe_residue(const int32_t *digits, uint32_t n, uint32_t b, int32_t *canon, gfe_result *out) {
     if (!digits || !out || n > 24) return gfe::set_error(GFE_EINVAL, "residue arguments");
     memset(out, 0, sizeof(*out));

@@ -44,6 +44,15 @@ int gfe_run_prp_trace(uint32_t b, uint32_t n, int device, uint64_t iters, uint64
  * mu_hash64 (may be null) receives gethash64 of mu_0..mu_depth (depth + 1 values). */
 int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_hash64, gfe_result *out);
 
+/* As gfe_run_proof, writing the reference's files: `<main_path>.proof` (int version = 1, int depth,
+ * then mu_0..mu_depth as residue records -- uint32 size, uint32 base, size int32 canonical digits --
+ * and the CRC-32 trailer ~crc ^ 0xa23777ac; src/genefer.h:620-734, src/gint.h:160-167,
+ * src/file.h:150-161), and with write_ckpts the checkpoint files `<main_path>_<i>.ckpt` (one residue
+ * record and the trailer each; src/genefer.h:140-144, 536-546), which the reference's proof and
+ * server modes read. */
+int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int write_ckpts,
+                    uint64_t *mu_hash64, gfe_result *out);
+
 /* Canonical residue of a digit vector with the product's gint (src/gint.h:40-225):
  * canon (may be null) receives the canonical digits. */
 int gfe_residue(const int32_t *digits, uint32_t n, uint32_t b, int32_t *canon, gfe_result *out);

@@ -236,3 +236,28 @@ def test_pietrzak_li_proof_identity(oracle):
         q = oracle.hash32(mu, n, b)
         v1 = v1 * pow(oracle.digits_to_int(mu, b), q, F) % F
     assert pkey == oracle.hash64(oracle.int_to_digits(v1, n, b), n, b)
+
+
+def test_pietrzak_li_proof_against_integer_fixture(oracle):
+    """The oracle engine's proof at b = 1026, n = 12 equals tests/golden/pl_n12.json, which holds the
+    proof computed with Python integers alone (tests/golden/make_pl_golden.py): pins the mu_k and
+    the proof key of the restated protocol (src/genefer.h:620-734) against mathematics."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pl_n12.json")
+    if not os.path.exists(path):
+        pytest.skip("pl_n12.json not generated")
+    fx = json.load(open(path))
+    b, n = fx["b"], fx["n"]
+    E = oracle.exponent(b, n)
+    es = E.bit_length()
+    assert es == fx["esize"]
+    want = fx["proofs"]["3"]
+    B_PL = oracle.b_pietrzak_li(es, 3)
+    assert B_PL == want["B_PL"]
+    e = oracle.Engine(b, n)
+    ck = {}
+    oracle.prp(e, E, B_GL=oracle.b_gerbicz_li(es), B_PL=B_PL, checkpoints=ck)
+    mus, pkey = oracle.pietrzak_li(e, ck, 3)
+    assert [f"{oracle.hash64(m, n, b):016X}" for m in mus] == want["mu_hash64"]
+    assert f"{pkey:016X}" == want["pkey"]
