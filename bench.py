@@ -118,7 +118,7 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     return res
 
 
-def bench_batched_n15(G, count: int, block: int):
+def bench_batched_n15(G, count: int, block: int, tune: bool = True):
     """Candidates/hour at n = 15: `count` seeded candidates in one batch vs one candidate per launch.
 
     Both legs time squaring iterations with HIP events on resident state; candidates/hour =
@@ -132,7 +132,7 @@ def bench_batched_n15(G, count: int, block: int):
         if bt.submit(b, 0, i) is None:
             raise RuntimeError("batch refused a candidate")
     bt.step()  # the first 64-iteration block of every candidate's program
-    plan_b = bt.tune(2)
+    plan_b = bt.tune(2) if tune else bt.info()
     log(f"[bench] batch plan: {plan_b}")
     bt.time_rounds(block)
     per_iter_b = bt.time_rounds(4 * block) / (4 * block)
@@ -140,7 +140,8 @@ def bench_batched_n15(G, count: int, block: int):
     plain_ms = []
     for b in bases[:4]:
         t = G.Transform(b, n)
-        t.tune(4)
+        if tune:
+            t.tune(4)
         t.set(1)
         t.time_squarings(block)
         plain_ms.append(t.time_squarings(4 * block) / (4 * block))
@@ -230,7 +231,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plan", default=None, help="fixed n=22 pass plan a,cl,cng,rl,rng,flags (default: the library's)")
-    ap.add_argument("--no-tune", action="store_true", help="time the library's default n=22 plan instead of the autotuned one")
+    ap.add_argument("--no-tune", action="store_true", help="time the library's default plans instead of the autotuned ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): candidates are independent and shard "
@@ -242,7 +243,7 @@ def main():
     res = bench_n22(G, args.steps, args.warmup, args.plan, not args.no_tune)
     log(f"[bench] n=22: {res['ns_per_squaring']:.0f} ns/squaring; kernels {res['kernel_us']}")
 
-    batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64)
+    batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64, not args.no_tune)
     if batched:
         log(f"[bench] n=15: {batched}")
 

@@ -310,12 +310,15 @@ def run_proof(b: int, n: int, depth: int, device: int = 0) -> dict:
     return out
 
 
-def write_proof(b: int, n: int, depth: int, main_path: str, checkpoints: bool = False, device: int = 0) -> dict:
+def write_proof(b: int, n: int, depth: int, main_path: str, checkpoints: bool = False, host_checkpoints: bool = False,
+                device: int = 0) -> dict:
     """proof() writing the reference's files: `<main_path>.proof` and, with `checkpoints`, the
-    checkpoint files `<main_path>_<i>.ckpt` (src/genefer.h:536-546, 620-734)."""
+    checkpoint files `<main_path>_<i>.ckpt` (src/genefer.h:536-546, 620-734).  `host_checkpoints`
+    keeps the checkpoints in host memory instead of HBM registers."""
     r = Result()
     mu = (ctypes.c_uint64 * (depth + 1))()
-    _ck(lib().gfe_write_proof(b, n, device, depth, main_path.encode(), 1 if checkpoints else 0, mu, ctypes.byref(r)))
+    flags = (1 if checkpoints else 0) | (2 if host_checkpoints else 0)
+    _ck(lib().gfe_write_proof(b, n, device, depth, main_path.encode(), flags, mu, ctypes.byref(r)))
     out = r.as_dict()
     out["mu_hash64"] = list(mu)
     return out

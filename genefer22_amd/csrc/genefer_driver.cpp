@@ -13,6 +13,14 @@ int set_error(int code, const std::string &msg);
 
 namespace {
 
+// Pietrzak-Li checkpoints stay in device registers (3 + i) whenever the 3 + 2^depth registers fit in
+// 64 GiB of HBM: n = 22 at proof power 8 takes 4.1 GiB.  The reference keeps them on the GPU only
+// for n <= 17 and writes files otherwise (src/genefer.h:1377-1385).
+bool checkpoints_in_hbm(uint32_t n, int depth) {
+    const double bytes = (3.0 + (double)(size_t(1) << depth)) * (double)(size_t(1) << n) * 4.0;
+    return depth <= 11 && bytes <= 64.0 * 1024 * 1024 * 1024;  // gfe_create takes at most 4096 registers
+}
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -112,7 +120,7 @@ int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_ha
     memset(out, 0, sizeof(*out));
     try {
         const double t0 = now_s();
-        const bool fast = (n <= 17);  // src/genefer.h:1377-1385
+        const bool fast = checkpoints_in_hbm(n, depth);
         gfe::genefer G(b, n, device, fast ? 3 + (size_t(1) << depth) : 3);
         const size_t es = G.esize();
         const int B_GL = gfe::genefer::B_GerbiczLi(es), B_PL = gfe::genefer::B_PietrzakLi(es, depth);
@@ -148,13 +156,14 @@ static bool write_gint_record(gfe::file &f, const std::vector<int32_t> &canon, u
            f.write(reinterpret_cast<const char *>(canon.data()), canon.size() * sizeof(int32_t));
 }
 
-int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int write_ckpts,
+int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int flags,
                     uint64_t *mu_hash64, gfe_result *out) {
-    if (!out || !main_path || depth < 1 || depth > 12) return gfe::set_error(GFE_EINVAL, "proof arguments");
+    if (!out || !main_path || depth < 1 || depth > 12 || (flags & ~3)) return gfe::set_error(GFE_EINVAL, "proof arguments");
     memset(out, 0, sizeof(*out));
+    const bool write_ckpts = (flags & GFE_PROOF_CKPT_FILES) != 0;
     try {
         const double t0 = now_s();
-        const bool fast = (n <= 17);  // src/genefer.h:1377-1385
+        const bool fast = checkpoints_in_hbm(n, depth) && !(flags & GFE_PROOF_HOST_CKPTS);
         gfe::genefer G(b, n, device, fast ? 3 + (size_t(1) << depth) : 3);
         const size_t es = G.esize();
         const int B_GL = gfe::genefer::B_GerbiczLi(es), B_PL = gfe::genefer::B_PietrzakLi(es, depth);

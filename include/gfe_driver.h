@@ -39,18 +39,22 @@ int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl
 int gfe_run_prp_trace(uint32_t b, uint32_t n, int device, uint64_t iters, uint64_t every, uint64_t *res64,
                       uint64_t *old64, gfe_result *out);
 
-/* proof(), src/genefer.h:750-766: PRP with Pietrzak-Li checkpoints (in registers for n <= 17 as the
- * reference's fast_checkpoints, host copies otherwise), the Gerbicz-Li check, then the proof.
+/* proof(), src/genefer.h:750-766: PRP with Pietrzak-Li checkpoints, the Gerbicz-Li check, then the
+ * proof.  The checkpoints stay in device registers (the reference's fast_checkpoints) whenever the
+ * 3 + 2^depth registers fit in 64 GiB of HBM (n = 22 at depth 8: 4.1 GiB), host copies otherwise.
  * mu_hash64 (may be null) receives gethash64 of mu_0..mu_depth (depth + 1 values). */
 int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_hash64, gfe_result *out);
 
 /* As gfe_run_proof, writing the reference's files: `<main_path>.proof` (int version = 1, int depth,
  * then mu_0..mu_depth as residue records -- uint32 size, uint32 base, size int32 canonical digits --
  * and the CRC-32 trailer ~crc ^ 0xa23777ac; src/genefer.h:620-734, src/gint.h:160-167,
- * src/file.h:150-161), and with write_ckpts the checkpoint files `<main_path>_<i>.ckpt` (one residue
- * record and the trailer each; src/genefer.h:140-144, 536-546), which the reference's proof and
- * server modes read. */
-int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int write_ckpts,
+ * src/file.h:150-161).  flags: GFE_PROOF_CKPT_FILES also writes the checkpoint files
+ * `<main_path>_<i>.ckpt` (one residue record and the trailer each; src/genefer.h:140-144, 536-546),
+ * which the reference's proof and server modes read; GFE_PROOF_HOST_CKPTS keeps the checkpoints in
+ * host memory even when they fit in HBM. */
+#define GFE_PROOF_CKPT_FILES 1
+#define GFE_PROOF_HOST_CKPTS 2
+int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int flags,
                     uint64_t *mu_hash64, gfe_result *out);
 
 /* Canonical residue of a digit vector with the product's gint (src/gint.h:40-225):

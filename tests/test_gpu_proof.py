@@ -51,6 +51,14 @@ def test_proof_and_files_match_integer_arithmetic(gfe, tmp_path, depth):
     assert not os.path.exists(f"{main}_{count}.ckpt")
 
 
+def test_proof_with_host_checkpoints(gfe, tmp_path):
+    """Checkpoints in host memory (the reference's file mode) give the same proof as in HBM."""
+    fx = _fixture()
+    r = gfe.write_proof(fx["b"], fx["n"], 3, str(tmp_path / "host"), host_checkpoints=True)
+    assert [f"{h:016X}" for h in r["mu_hash64"]] == fx["proofs"]["3"]["mu_hash64"]
+    assert f"{r['pkey']:016X}" == fx["proofs"]["3"]["pkey"]
+
+
 def test_proof_without_files_equals_run_proof(gfe):
     fx = _fixture()
     r = gfe.run_proof(fx["b"], fx["n"], 3)
