@@ -12,7 +12,8 @@ One JSON line on stdout (progress on stderr):
   * batched_n15: candidates/hour at n = 15, 512 seeded candidates in one batch vs one candidate per
                  launch, from timed iteration blocks (full-test length = esize squarings);
   * serving    : p50/p99 time-to-result of seeded Poisson requests of mixed n (12..20) through the
-                 serving loop, and requests/hour against the serial plain path;
+                 serving loop, and requests/hour against the serial plain path, at a saturating
+                 arrival rate (serving) and at a light one (serving_light: latency near service time);
   * parity     : RES64 after P PRP iterations on the GPU vs the reference's own CPU transform;
   * cpu_baseline: the reference's CPU transform (compiled from its sources into oracle/_ref) timed on
                  this host's cores on a bounded sample of the same workload.
@@ -158,30 +159,43 @@ def bench_batched_n15(G, count: int, block: int, tune: bool = True):
     }
 
 
-def bench_serving(G, count: int, rate: float, iters: int):
+def plain_ms_by_n(G, reqs) -> dict:
+    """ms per squaring of one candidate on the plain path with the library's default plan, per n."""
+    b_of_n = {q.n: q.b for q in reqs}
+    out = {}
+    for n in sorted(b_of_n):
+        t = G.Transform(b_of_n[n], n)
+        t.set(1)
+        t.time_squarings(32)
+        out[n] = t.time_squarings(256) / 256
+        t.close()
+    return out
+
+
+def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
     """Serving: `count` seeded requests (n uniform in 12..20, b up to the reference's limit per n,
     Poisson arrivals at `rate`/s, the first `iters` PRP squarings each) replayed on the wall clock
     through the native serving loop; p50/p99 time-to-result and requests/hour, against the same
     requests run one after another on the plain one-candidate path (per-n ms/squaring measured on
-    the library's default plans)."""
+    the library's default plans) and against each request's own plain run time (p50 of
+    iters x ms/squaring of its n: the latency floor of one request alone on the plain path)."""
     from genefer22_amd import serve
     reqs = W.serving_requests(count, rate, iters=iters)
     t0 = time.perf_counter()
     _, summary, stats = serve.run(reqs)
     wall = time.perf_counter() - t0
-    ns = sorted({q.n for q in reqs})
-    b_of_n = {q.n: q.b for q in reqs}
-    ms_by_n = {}
-    for n in ns:
-        t = G.Transform(b_of_n[n], n)
-        t.set(1)
-        t.time_squarings(32)
-        ms_by_n[n] = t.time_squarings(256) / 256
-        t.close()
+    missing = sorted({q.n for q in reqs} - set(ms_cache))
+    if missing:
+        ms_cache.update(plain_ms_by_n(G, [q for q in reqs if q.n in missing]))
+    ms_by_n = {n: ms_cache[n] for n in sorted({q.n for q in reqs})}
     serial = serve.plain_serial_seconds(reqs, ms_by_n)
+    alone = sorted(q.iters * ms_by_n[q.n] / 1e3 for q in reqs)
     return {
         "requests": count, "arrival_rate_per_s": rate, "squarings_per_request": iters,
         "p50_s": summary["p50_s"], "p99_s": summary["p99_s"], "makespan_s": summary["makespan_s"],
+        "p50_admit_wait_s": summary["p50_admit_wait_s"],
+        "plain_alone_p50_s": round(serve.percentile(alone, 50), 5),
+        "plain_alone_p99_s": round(serve.percentile(alone, 99), 5),
         "requests_per_hour": summary["requests_per_hour"],
         "plain_serial_requests_per_hour": round(count / serial * 3600.0, 1),
         "speedup_vs_plain_serial": round(serial / summary["makespan_s"], 2),
@@ -229,6 +243,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--light-requests", type=int, default=100, help="requests of the light-load serving replay")
+    ap.add_argument("--light-rate", type=float, default=8.0, help="arrivals/s of the light-load serving replay")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plan", default=None, help="fixed n=22 pass plan a,cl,cng,rl,rng,flags (default: the library's)")
     ap.add_argument("--no-tune", action="store_true", help="time the library's default plans instead of the autotuned ones")
@@ -247,9 +263,13 @@ def main():
     if batched:
         log(f"[bench] n=15: {batched}")
 
-    serving = None if args.no_serve else bench_serving(G, args.serve_requests, args.serve_rate, 4096)
-    if serving:
-        log(f"[bench] serving: {serving}")
+    serving = serving_light = None
+    if not args.no_serve:
+        ms_cache = {}
+        serving = bench_serving(G, args.serve_requests, args.serve_rate, 4096, ms_cache)
+        log(f"[bench] serving (saturated): {serving}")
+        serving_light = bench_serving(G, args.light_requests, args.light_rate, 4096, ms_cache)
+        log(f"[bench] serving (light load): {serving_light}")
 
     threads = min(16, os.cpu_count() or 1)
     parity = cpu = None
@@ -279,7 +299,7 @@ def main():
         "config": {
             "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] (NTT3, 31-bit primes)",
             "n": 22, "b": res["b"], "plan": res["plan"], "plan_tuple": res["plan_tuple"], "kernel_us": res["kernel_us"],
-            "batched_n15": batched, "serving": serving, "parity": parity,
+            "batched_n15": batched, "serving": serving, "serving_light": serving_light, "parity": parity,
         },
         "roofline": res["roofline"],
         "valu": res["valu"],

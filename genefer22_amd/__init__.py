@@ -84,6 +84,8 @@ ABI = [
     ("gfe_run_quick", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(Result)]),
     ("gfe_run_prp", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_int32, I32P, ctypes.POINTER(Result)]),
     ("gfe_run_prp_trace", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, U64P, U64P, ctypes.POINTER(Result)]),
+    ("gfe_run_prp_resumable", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
+                                             ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Result)]),
     ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),
     ("gfe_residue", ctypes.c_int, [I32P, ctypes.c_uint32, ctypes.c_uint32, I32P, ctypes.POINTER(Result)]),
     ("gfe_write_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
@@ -287,6 +289,21 @@ def run_prp(b: int, n: int, iters: int = 0, b_gl: int = 0, device: int = 0, digi
     out = r.as_dict()
     if digits:
         out["digits"] = d
+    return out
+
+
+def run_prp_resumable(b: int, n: int, ctx_path: str, save_every: int = 0, stop_after: int = 0,
+                      device: int = 0) -> dict:
+    """The PRP loop with the reference's context file (src/genefer.h:198-276, 479-552): resumes from
+    `ctx_path` when it holds a context of this kind, saves every `save_every` squarings and stops
+    (saving) after `stop_after` squarings of this call; 'done' tells whether the loop has ended, in
+    which case the Gerbicz-Li check has run ('gl_ok')."""
+    r = Result()
+    done = ctypes.c_int32(0)
+    _ck(lib().gfe_run_prp_resumable(b, n, device, os.fsencode(ctx_path), save_every, stop_after,
+                                    ctypes.byref(done), ctypes.byref(r)))
+    out = r.as_dict()
+    out["done"] = bool(done.value)
     return out
 
 

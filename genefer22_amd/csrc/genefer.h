@@ -16,6 +16,8 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "transform_hip.h"
@@ -118,6 +120,106 @@ public:
         }
     }
 
+    // Context files, src/genefer.h:198-276: int version (-1, the reference's GPU builds), int where
+    // (0: the PRP loop), int i (the loop index to run next, -1 once the loop has ended), double
+    // elapsed seconds, the transform payload of registers 0..1 (0..2 for where != 0; all of them
+    // with fast checkpoints), the CRC trailer.  A save writes <path>.new, then moves <path> to
+    // <path>.old and <path>.new to <path>; a read that finds <path> missing or invalid tries
+    // <path>.old.
+    bool saveContext(const std::string &path, int where, bool fast, int i, double elapsed) const {
+        const std::string old_path = path + ".old", new_path = path + ".new";
+        {
+            file f(new_path, "wb");
+            const int version = -1;
+            if (!f.write(reinterpret_cast<const char *>(&version), sizeof(version))) return false;
+            if (!f.write(reinterpret_cast<const char *>(&where), sizeof(where))) return false;
+            if (!f.write(reinterpret_cast<const char *>(&i), sizeof(i))) return false;
+            if (!f.write(reinterpret_cast<const char *>(&elapsed), sizeof(elapsed))) return false;
+            _t->saveContext(f, fast ? 0 : context_regs(where));
+            f.write_crc32();
+            if (!f.close()) return false;
+        }
+        std::remove(old_path.c_str());
+        if (file_exists(path) && std::rename(path.c_str(), old_path.c_str()) != 0) return false;
+        return std::rename(new_path.c_str(), path.c_str()) == 0;
+    }
+
+    // 0 on success, -1 when the file is missing, -2 when it is not a context of this kind
+    // (src/genefer.h:198-218).
+    int readContextFile(const std::string &path, int where, bool fast, int &i, double &elapsed) {
+        file f(path, "rb");
+        if (!f.exists()) return -1;
+        int version = 0, rwhere = 0;
+        if (!f.read(reinterpret_cast<char *>(&version), sizeof(version)) || -version != 1) return -2;
+        if (!f.read(reinterpret_cast<char *>(&rwhere), sizeof(rwhere)) || rwhere != where) return -2;
+        if (!f.read(reinterpret_cast<char *>(&i), sizeof(i))) return -2;
+        if (!f.read(reinterpret_cast<char *>(&elapsed), sizeof(elapsed))) return -2;
+        if (!_t->readContext(f, fast ? 0 : context_regs(where))) return -2;
+        if (!f.check_crc32()) return -2;
+        return 0;
+    }
+
+    bool readContext(const std::string &path, int where, bool fast, int &i, double &elapsed) {
+        if (readContextFile(path, where, fast, i, elapsed) == 0) return true;
+        return readContextFile(path + ".old", where, fast, i, elapsed) == 0;
+    }
+
+    // The PRP loop of src/genefer.h:479-552 with its context file at `path`: resumes from a context
+    // of this kind when there is one, else starts from r0 = 1.  Saves a context every `save_every`
+    // squarings of this call (the reference saves every 600 s of wall time; 0: none) and, once
+    // `stop_after` squarings of this call have run (0: no limit), saves one and stops, as the
+    // reference does when asked to quit.  Returns true when the loop has run to its end (the
+    // context then holds i = -1).  `resumed_at` receives the loop index of the resumed context
+    // (-2 for a fresh start), `i_next` the index a later call starts at, `elapsed` the accumulated
+    // wall time.
+    bool prp_ctx(int B_GL, const std::string &path, long save_every, long stop_after, int &resumed_at, int &i_next,
+                 double &elapsed) {
+        transformHIP &T = *_t;
+        int ri = 0;
+        double rt = 0;
+        const bool found = readContext(path, 0, false, ri, rt);
+        if (!found) {
+            ri = 0;
+            rt = 0;
+            T.set(1);
+            T.copy(1, 0);
+        }
+        resumed_at = found ? ri : -2;
+        const auto t0 = std::chrono::steady_clock::now();
+        auto clock = [&] { return rt + std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+        auto save = [&](int i) {
+            if (!saveContext(path, 0, false, i, clock())) throw std::runtime_error("cannot save context " + path);
+        };
+        const int i0 = (int)esize() - 1, i_start = found ? ri : i0;
+        if (found && ri == -1) {
+            elapsed = rt;
+            i_next = -1;
+            return true;
+        }
+        long done = 0;
+        for (int i = i_start; i >= 0; --i) {
+            if (stop_after > 0 && done == stop_after) {
+                save(i);
+                elapsed = clock();
+                i_next = i;
+                return false;
+            }
+            if (save_every > 0 && done > 0 && done % save_every == 0) save(i);
+            T.squareDup(mpz_tstbit(_E, (mp_bitcnt_t)i) != 0);
+            if (B_GL > 0 && (i % B_GL == 0) && (i / B_GL != 0)) {
+                T.copy(2, 0);
+                T.mul(1);
+                T.copy(1, 0);
+                T.copy(0, 2);
+            }
+            ++done;
+        }
+        save(-1);
+        elapsed = clock();
+        i_next = -1;
+        return true;
+    }
+
     // The Gerbicz-Li check, src/genefer.h:557-615.  In: r0 = 2^E, r1 = d(t).
     bool GL(int B_GL) {
         transformHIP &T = *_t;
@@ -196,6 +298,15 @@ public:
 
 private:
     std::unique_ptr<transformHIP> _t;
+
+    static size_t context_regs(int where) { return (where == 0) ? 2 : 3; }
+    static bool file_exists(const std::string &path) {
+        std::FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) return false;
+        std::fclose(f);
+        return true;
+    }
+
     gint _g;
     mpz_t _E;
     std::map<size_t, std::vector<int32_t>> _ckpts;

@@ -83,6 +83,36 @@ int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl
     }
 }
 
+int gfe_run_prp_resumable(uint32_t b, uint32_t n, int device, const char *ctx_path, uint64_t save_every,
+                          uint64_t stop_after, int32_t *done, gfe_result *out) {
+    if (!out || !ctx_path || !*ctx_path) return gfe::set_error(GFE_EINVAL, "resumable prp arguments");
+    memset(out, 0, sizeof(*out));
+    if (done) *done = 0;
+    try {
+        gfe::genefer G(b, n, device, 3);
+        const size_t es = G.esize();
+        const int B_GL = gfe::genefer::B_GerbiczLi(es);
+        int resumed_at = -2, i_next = 0;
+        double elapsed = 0;
+        const bool ended = G.prp_ctx(B_GL, ctx_path, (long)save_every, (long)stop_after, resumed_at, i_next, elapsed);
+        G.t().getInt(G.gi());
+        fill_residue(G.gi(), out);
+        out->esize = es;
+        out->iters = (uint64_t)((long)es - 1 - (long)i_next);
+        out->seconds = elapsed;
+        if (ended) {
+            out->gl_ok = G.GL(B_GL) ? 1 : 0;
+        } else {
+            out->gl_ok = -1;
+            out->is_prp = -1;
+        }
+        if (done) *done = ended ? 1 : 0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EHIP, ex.what());
+    }
+}
+
 int gfe_run_prp_trace(uint32_t b, uint32_t n, int device, uint64_t iters, uint64_t every, uint64_t *res64,
                       uint64_t *old64, gfe_result *out) {
     if (!out || !res64 || !old64 || every == 0) return gfe::set_error(GFE_EINVAL, "trace arguments");

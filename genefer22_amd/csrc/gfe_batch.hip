@@ -315,9 +315,13 @@ struct Batch {
     // Squaring blocks as HIP graphs: the kernels of `rounds` consecutive rounds (per-round flag rows
     // act_dev / dup_dev + r cap, which the block's flag upload fills before the launch) captured
     // once per (rounds, grid depth zl, plan) and replayed.  The cache is dropped when the plan
-    // changes.  Captured kernels record no memset (fill kernels only).
-    struct BlockGraph { int rounds; uint32_t zl; uint64_t key; hipGraphExec_t exec; };
+    // changes and keeps the MAX_GRAPHS most recently launched otherwise.  An executable graph is
+    // destroyed only after the stream has drained: a launch of it may still be queued.  Captured
+    // kernels record no memset (fill kernels only).
+    static constexpr size_t MAX_GRAPHS = 16;
+    struct BlockGraph { int rounds; uint32_t zl; uint64_t key; hipGraphExec_t exec; uint64_t last_use; };
     std::vector<BlockGraph> graphs;
+    uint64_t graph_uses = 0;
     bool use_graphs = true;
     uint64_t plan_key() const;
     hipGraphExec_t block_graph(int rounds);
@@ -375,6 +379,8 @@ uint64_t Batch::plan_key() const {
 }
 
 void Batch::drop_graphs() {
+    if (graphs.empty()) return;
+    (void)hipStreamSynchronize(stream);
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
 }
@@ -384,7 +390,10 @@ void Batch::drop_graphs() {
 hipGraphExec_t Batch::block_graph(int rounds) {
     const uint64_t key = plan_key();
     for (auto &g : graphs)
-        if (g.rounds == rounds && g.zl == zdim() && g.key == key) return g.exec;
+        if (g.rounds == rounds && g.zl == zdim() && g.key == key) {
+            g.last_use = ++graph_uses;
+            return g.exec;
+        }
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
     for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
@@ -399,11 +408,14 @@ hipGraphExec_t Batch::block_graph(int rounds) {
         (void)hipGetLastError();
         return nullptr;
     }
-    if (graphs.size() >= 8) {
-        (void)hipGraphExecDestroy(graphs.front().exec);
-        graphs.erase(graphs.begin());
+    if (graphs.size() >= MAX_GRAPHS) {
+        auto lru = std::min_element(graphs.begin(), graphs.end(),
+                                    [](const BlockGraph &x, const BlockGraph &y) { return x.last_use < y.last_use; });
+        (void)hipStreamSynchronize(stream);
+        (void)hipGraphExecDestroy(lru->exec);
+        graphs.erase(lru);
     }
-    graphs.push_back({rounds, zdim(), key, exec});
+    graphs.push_back({rounds, zdim(), key, exec, ++graph_uses});
     return exec;
 }
 

@@ -874,28 +874,63 @@ size_t gfe_cache_size(const gfe_engine *e) {
     return (size_t)g.rns * g.N * 4 + g.N * 4;
 }
 
-static int engine_kind(const Engine &g) { return 0x47460000 | (g.rns << 4) | g.is32; }
+// transform::EKind of the reference's GPU transform (src/transform.h:45, src/transformGPU.h:947):
+// the first int of a context payload.
+static int engine_kind(const Engine &g) { return g.rns == 2 ? 6 : 7; }
 
 size_t gfe_context_size(const gfe_engine *e, uint32_t num_regs) {
     if (!e) return 0;
-    return sizeof(int) + (size_t)num_regs * e->eng.N * 4;
+    return sizeof(int) + (size_t)num_regs * e->eng.rns * e->eng.N * 4;
 }
 
+// The registers hold signed digits; the payload holds their plain residues prime by prime, as the
+// reference's device registers do between squarings (set_int: d < 0 -> d + p).
 int gfe_save(gfe_engine *e, uint32_t num_regs, void *buf, size_t size) {
     GFE_GUARD(e);
     Engine &g = e->eng;
     if (num_regs == 0) num_regs = g.num_regs;
     if (num_regs > g.num_regs) return set_error(GFE_EINVAL, "num_regs");
     if (!buf || size < gfe_context_size(e, num_regs)) return set_error(GFE_EINVAL, "buffer too small");
-    const int kind = engine_kind(g);
-    memcpy(buf, &kind, sizeof(kind));
-    if (hipMemcpyAsync((char *)buf + sizeof(int), g.regs, (size_t)num_regs * g.N * 4, hipMemcpyDeviceToHost, g.stream) !=
-        hipSuccess)
+    std::vector<int32_t> d((size_t)num_regs * g.N);
+    if (hipMemcpyAsync(d.data(), g.regs, d.size() * 4, hipMemcpyDeviceToHost, g.stream) != hipSuccess)
         return set_error(GFE_EHIP, "save");
     if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "save sync");
+    const int kind = engine_kind(g);
+    memcpy(buf, &kind, sizeof(kind));
+    uint32_t *z = reinterpret_cast<uint32_t *>((char *)buf + sizeof(int));
+    for (uint32_t r = 0; r < num_regs; ++r)
+        for (int k = 0; k < g.rns; ++k) {
+            const uint32_t p = gfe::prime_of(g.is32, k);
+            const int32_t *dr = d.data() + (size_t)r * g.N;
+            uint32_t *zr = z + ((size_t)r * g.rns + k) * g.N;
+            for (size_t i = 0; i < g.N; ++i) zr[i] = dr[i] < 0 ? (uint32_t)dr[i] + p : (uint32_t)dr[i];
+        }
     return GFE_OK;
 }
 
+// Balanced digits |d| <= b/2 (+1 for digit 0) of the same value mod b^N + 1: a carry pass with the
+// wrap x^N = -1, then at most two more passes for the carry that wraps; what is left of it stays in
+// digit 0.
+static void balance_digits(int32_t *d, size_t N, uint32_t b) {
+    const int64_t bb = b, half = b / 2;
+    int64_t c = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+        for (size_t i = 0; i < N; ++i) {
+            const int64_t v = (int64_t)d[i] + c, t = v + half;
+            const int64_t q = t >= 0 ? t / bb : -((-t + bb - 1) / bb);
+            d[i] = (int32_t)(v - q * bb);
+            c = q;
+        }
+        c = -c;
+        if (c == 0) return;
+    }
+    d[0] = (int32_t)((int64_t)d[0] + c);
+}
+
+// Digits come back from the first prime's residue (ZPT::get_int, src/transformGPU.h:146); the
+// other primes must hold the residues of the same digit.  The reference's registers hold digits in
+// (-b, b) (its normalisation keeps the sign of the carry, ocl/kernel.cl:2286-2300); they are
+// balanced here, as the engine keeps them.
 int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size) {
     GFE_GUARD(e);
     Engine &g = e->eng;
@@ -906,8 +941,28 @@ int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size) {
     memcpy(&kind, buf, sizeof(kind));
     if (kind != engine_kind(g)) return set_error(GFE_EKIND, "context of another engine kind");
     if (size < gfe_context_size(e, num_regs)) return set_error(GFE_EINVAL, "short context");
-    if (hipMemcpyAsync(g.regs, (const char *)buf + sizeof(int), (size_t)num_regs * g.N * 4, hipMemcpyHostToDevice,
-                       g.stream) != hipSuccess)
+    const uint32_t *z = reinterpret_cast<const uint32_t *>((const char *)buf + sizeof(int));
+    const uint32_t p0 = gfe::prime_of(g.is32, 0);
+    std::vector<int32_t> d((size_t)num_regs * g.N);
+    for (uint32_t r = 0; r < num_regs; ++r) {
+        const uint32_t *z0 = z + (size_t)r * g.rns * g.N;
+        int32_t *dr = d.data() + (size_t)r * g.N;
+        for (size_t i = 0; i < g.N; ++i) {
+            const uint32_t v = z0[i];
+            if (v >= p0) return set_error(GFE_EINVAL, "context residue not reduced");
+            dr[i] = (v >= p0 / 2) ? (int32_t)(int64_t(v) - p0) : (int32_t)v;
+        }
+        for (int k = 1; k < g.rns; ++k) {
+            const uint32_t p = gfe::prime_of(g.is32, k);
+            const uint32_t *zk = z0 + (size_t)k * g.N;
+            for (size_t i = 0; i < g.N; ++i) {
+                const uint32_t want = dr[i] < 0 ? (uint32_t)dr[i] + p : (uint32_t)dr[i];
+                if (zk[i] != want) return set_error(GFE_EINVAL, "context residues disagree between primes");
+            }
+        }
+        balance_digits(dr, g.N, g.b);
+    }
+    if (hipMemcpyAsync(g.regs, d.data(), d.size() * 4, hipMemcpyHostToDevice, g.stream) != hipSuccess)
         return set_error(GFE_EHIP, "load");
     if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "load sync");
     return GFE_OK;

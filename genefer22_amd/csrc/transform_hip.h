@@ -29,6 +29,13 @@ public:
     file(const file &) = delete;
     file &operator=(const file &) = delete;
     bool exists() const { return _f != nullptr; }
+    // Flushes and closes; false when a write did not reach the file.
+    bool close() {
+        if (!_f) return false;
+        const bool ok = std::fclose(_f) == 0;
+        _f = nullptr;
+        return ok;
+    }
     uint32_t crc32() const { return _crc; }
     bool read(char *p, size_t n) {
         if (!_f) return false;

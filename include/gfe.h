@@ -59,10 +59,16 @@ int gfe_set_digits(gfe_engine *e, uint32_t reg, const int32_t *in);
 /* transform::getMemSize / getCacheSize, src/transform.h:65-66. */
 size_t gfe_mem_size(const gfe_engine *e);
 size_t gfe_cache_size(const gfe_engine *e);
-/* transform::saveContext / readContext, src/transform.h:68-69 (payload of the context file after
- * the driver's header: int kind, then the num_regs registers).  gfe_context_size gives the bytes
+/* transform::saveContext / readContext, src/transform.h:68-69: the payload of the context file
+ * after the driver's header, in the layout of the reference's GPU transform
+ * (src/transformGPU.h:1127-1150): int kind (transform::EKind, src/transform.h:45: 6 = NTT2,
+ * 7 = NTT3), then for each of the num_regs registers and each prime of the set, N uint32 plain
+ * residues of the signed digits (ZPT::set_int, src/transformGPU.h:147).  A context written by the
+ * reference's GPU build loads here and the other way round.  gfe_context_size gives the bytes
  * gfe_save writes.  gfe_load returns GFE_EKIND for a payload of another kind (the reference's
- * readContext returns false, src/transformGPU.h:1127-1139). */
+ * readContext returns false) and GFE_EINVAL when the residues of one digit disagree between primes.
+ * Loaded digits of any size (the reference's registers hold digits in (-b, b)) are balanced to
+ * |d| <= b/2 on the host, which keeps the value. */
 size_t gfe_context_size(const gfe_engine *e, uint32_t num_regs);
 int gfe_save(gfe_engine *e, uint32_t num_regs, void *buf, size_t size);
 int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size);

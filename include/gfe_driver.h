@@ -57,6 +57,21 @@ int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_ha
 int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int flags,
                     uint64_t *mu_hash64, gfe_result *out);
 
+/* The PRP loop with the reference's context file at ctx_path (src/genefer.h:198-276, 479-552):
+ * resumes from it (or from ctx_path.old) when it holds a PRP-loop context of this engine kind,
+ * else starts from r0 = 1.  Writes a context every `save_every` squarings of this call (0: none)
+ * and, once `stop_after` squarings of this call have run (0: no limit), writes one and returns, as
+ * the reference does when asked to quit.  The file is the reference GPU build's: int -1, int 0
+ * (the PRP loop), int i (next loop index, -1 at the end), double elapsed seconds, the transform
+ * payload of gfe_save for registers 0 and 1, the CRC-32 trailer; each save goes through
+ * ctx_path.new and keeps the previous file as ctx_path.old.  out->iters counts the squarings of
+ * the loop done over all calls; once the loop has ended the Gerbicz-Li check runs at the
+ * reference's cadence (out->gl_ok 1/0) and out->is_prp is set; before that out->gl_ok = -1 and
+ * out->is_prp = -1.  out->seconds is the elapsed time the context records.  *done (may be null)
+ * is 1 when the loop has ended. */
+int gfe_run_prp_resumable(uint32_t b, uint32_t n, int device, const char *ctx_path, uint64_t save_every,
+                          uint64_t stop_after, int32_t *done, gfe_result *out);
+
 /* Canonical residue of a digit vector with the product's gint (src/gint.h:40-225):
  * canon (may be null) receives the canonical digits. */
 int gfe_residue(const int32_t *digits, uint32_t n, uint32_t b, int32_t *canon, gfe_result *out);

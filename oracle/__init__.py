@@ -265,20 +265,26 @@ def closed_form_residue(b: int, n: int, iters: int) -> int:
 
 
 def prp(eng, E: int, iters: int | None = None, B_GL: int | None = None, B_PL: int = 0,
-        checkpoints: dict | None = None):
+        checkpoints: dict | None = None, resume_at: int | None = None):
     """The Fermat PRP loop, src/genefer.h:479-552, on any engine with the transform interface.
 
     Registers: 0 = 2^(partial exponent), 1 = d(t) (Gerbicz-Li product), 2 = scratch.
     Runs the first `iters` exponent bits (all when None).  With B_PL, register 0 is recorded into
-    `checkpoints[i // B_PL]` at every i % B_PL == 0 (the proof checkpoints, 536-546).
+    `checkpoints[i // B_PL]` at every i % B_PL == 0 (the proof checkpoints, 536-546).  With
+    `resume_at`, registers 0 and 1 already hold a resumed context (484-503) and the loop runs from
+    index resume_at for `iters` bits (to the end when None).
     Returns the index i of the last processed bit."""
     esize = E.bit_length()
     if B_GL is None:
         B_GL = b_gerbicz_li(esize)
-    eng.set(1)
-    eng.copy(1, 0)
-    i_end = 0 if iters is None or iters >= esize else esize - iters
-    for i in range(esize - 1, i_end - 1, -1):
+    i_start = esize - 1
+    if resume_at is None:
+        eng.set(1)
+        eng.copy(1, 0)
+    else:
+        i_start = resume_at
+    i_end = 0 if iters is None or iters > i_start else i_start + 1 - iters
+    for i in range(i_start, i_end - 1, -1):
         eng.square_dup(((E >> i) & 1) != 0)
         if B_GL > 0 and i % B_GL == 0 and i // B_GL != 0:
             eng.copy(2, 0)

@@ -108,6 +108,30 @@ def test_batch_graphs_equal_eager(gfe):
         assert out[True][tag][1] == gfe.run_prp(b, 12, iters=it, b_gl=-1)["res64"], b
 
 
+def test_batch_graph_cache_turnover(gfe):
+    """Candidates that finish highest slot first shrink the grid depth block after block, so each
+    block needs a graph of its own depth and the cache of executable graphs turns over many times
+    while earlier launches are still queued; every residue equals its single run."""
+    from genefer22_amd import workloads
+    n, cap = 14, 24
+    bases = workloads.seeded_bases(1414, 1024, 2_000_000_000, cap)
+    iters = [64 * (cap - i) + 5 for i in range(cap)]
+    bt = gfe.Batch(n, cap)
+    assert not bt.get_plan()[5] & 16
+    bt.set_graphs(True)
+    for i, b in enumerate(bases):
+        assert bt.submit(b, iters=iters[i], tag=i) == i
+    done = []
+    while len(done) < cap:
+        bt.step()
+        done += bt.poll()
+    bt.close()
+    by_tag = {r["tag"]: r for r in done}
+    for i, b in enumerate(bases):
+        alone = gfe.run_prp(b, n, iters=iters[i], b_gl=-1)
+        assert (by_tag[i]["res64"], by_tag[i]["old64"]) == (alone["res64"], alone["old64"]), (i, b)
+
+
 def test_batch_default_small_n_squares_in_one_workgroup(gfe):
     """A batch of 40 candidates at n = 12 defaults to the single-workgroup squaring (plan bit 16:
     one launch per block of rounds, one workgroup per slot); its residues equal single runs on the

@@ -121,6 +121,27 @@ def test_prp_closed_form(oracle, b, n, iters):
         assert v == oracle.closed_form_residue(b, n, iters)
 
 
+def test_prp_resume_equals_one_run(oracle):
+    """A loop stopped after 70 iterations and resumed from its registers 0 and 1 (a context,
+    src/genefer.h:484-503) reaches the registers of one uninterrupted run of 150, and r0 the closed
+    form; the Gerbicz-Li product crosses the stop (B_GL = 7)."""
+    b, n, B_GL = 1_999_999_266, 6, 7
+    E = oracle.exponent(b, n)
+    esize = E.bit_length()
+    one = oracle.Engine(b, n)
+    oracle.prp(one, E, iters=150, B_GL=B_GL)
+    first = oracle.Engine(b, n)
+    oracle.prp(first, E, iters=70, B_GL=B_GL)
+    second = oracle.Engine(b, n)
+    second.set_digits(first.get_digits(0), 0)
+    second.set_digits(first.get_digits(1), 1)
+    oracle.prp(second, E, iters=80, B_GL=B_GL, resume_at=esize - 1 - 70)
+    for reg in (0, 1):
+        assert np.array_equal(oracle.unbalance(second.get_digits(reg), n, b), oracle.unbalance(one.get_digits(reg), n, b))
+    v = oracle.digits_to_int(second.get_digits(0), b) % (b ** (1 << n) + 1)
+    assert v == oracle.closed_form_residue(b, n, 150)
+
+
 def test_square_mul(oracle):
     b, n = 777_778, 6
     rs = np.random.RandomState(3)
