@@ -326,7 +326,12 @@ struct Batch {
     uint64_t plan_key() const;
     hipGraphExec_t block_graph(int rounds);
     void drop_graphs();
-    int run_block(int rounds);
+    int run_block(int rounds, uint32_t nact);
+    // The single-workgroup squaring at n = 13 pays off once 128 or more slots are active: with
+    // fewer, one workgroup per candidate leaves most CUs idle and the four-step passes finish a
+    // round sooner (one candidate: 76 ms against 166 ms for 4096 squarings on MI355X).  At n = 12
+    // it is the faster path at any occupancy.
+    bool fused_block(uint32_t nact) const { return plan.fuse_sq && (n <= 12 || nact >= 128u); }
     std::vector<Slot> slots;
     std::deque<gfe_batch_result> finished;
     double t0 = 0;
@@ -422,8 +427,8 @@ hipGraphExec_t Batch::block_graph(int rounds) {
 // Enqueues `rounds` squaring rounds (flag rows already uploaded): one graph launch, or the
 // kernels one by one when graphs are off or the plan uses persistent grids (their sizes come from
 // occupancy queries, which stay out of captures).
-int Batch::run_block(int rounds) {
-    if (plan.fuse_sq) {  // one launch runs the whole block, one workgroup per slot
+int Batch::run_block(int rounds, uint32_t nact) {
+    if (fused_block(nact)) {  // one launch runs the whole block, one workgroup per slot
         FusedArgs A;
         memset(&A, 0, sizeof(A));
         A.regs = regs; A.tw = tw; A.crt = crt; A.slots = slots_dev; A.act = act_dev; A.dupf = dup_dev;
@@ -987,12 +992,14 @@ int Batch::step(int32_t *active_out) {
     }
     // 2. one block of up to 64 squaring rounds
     int rounds = 0;
+    uint32_t nact = 0;
     std::vector<uint64_t> len(cap, 0);
     for (uint32_t s = 0; s < cap; ++s) {
         Slot &S = slots[s];
         if (!S.busy || S.done || S.q.empty() || S.q.front().kind != IT_RUN) continue;
         len[s] = std::min<uint64_t>(64, S.q.front().len - S.pos);
         rounds = std::max(rounds, (int)len[s]);
+        ++nact;
     }
     if (rounds > 0) {
         uint8_t *st = stage_acquire();
@@ -1015,7 +1022,7 @@ int Batch::step(int32_t *active_out) {
         if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(dup_dev, dup, (size_t)rounds * cap, hipMemcpyHostToDevice, stream), "dup");
         stage_release();
         if (rc != GFE_OK) return rc;
-        rc = run_block(rounds);
+        rc = run_block(rounds, nact);
         if (rc != GFE_OK) return rc;
         if ((int)blk_q.size() == NBLK) {
             rc = check_hip(hipEventSynchronize(blk_ev[blk_q.front()]), "block sync");
@@ -1190,6 +1197,8 @@ int gfe_batch_time_rounds(gfe_batch *b, uint32_t rounds, double *ms) {
     // every round: the busy slots square without doubling (64 flag rows, as a squaring block)
     uint8_t *st = B.stage_acquire();
     memset(st, 0, (size_t)2 * 64 * B.cap);
+    uint32_t nact = 0;
+    for (uint32_t s = 0; s < B.cap; ++s) nact += B.slots[s].busy ? 1 : 0;
     for (uint32_t r = 0; r < 64; ++r)
         for (uint32_t s = 0; s < B.cap; ++s) st[(size_t)r * B.cap + s] = B.slots[s].busy ? 1 : 0;
     int rc = gfe::check_hip(hipMemcpyAsync(B.act_dev, st, (size_t)64 * B.cap, hipMemcpyHostToDevice, B.stream), "act");
@@ -1198,14 +1207,14 @@ int gfe_batch_time_rounds(gfe_batch *b, uint32_t rounds, double *ms) {
     B.stage_release();
     if (rc != GFE_OK) return rc;
     // captured blocks are built before the timed region
-    if (B.use_graphs && !B.plan.pipe) {
+    if (B.use_graphs && !B.plan.pipe && !B.fused_block(nact)) {
         if (rounds >= 64 && !B.block_graph(64)) return set_error(GFE_EHIP, "squaring block graph capture failed");
         if ((rounds % 64) && !B.block_graph((int)(rounds % 64))) return set_error(GFE_EHIP, "squaring block graph capture failed");
     }
     (void)hipEventRecord(B.ev0, B.stream);
     for (uint32_t done = 0; done < rounds;) {
         const int r = (int)std::min<uint32_t>(64, rounds - done);
-        rc = B.run_block(r);
+        rc = B.run_block(r, nact);
         if (rc != GFE_OK) return rc;
         done += (uint32_t)r;
     }

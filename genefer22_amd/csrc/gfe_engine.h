@@ -64,6 +64,12 @@ struct Engine {
     unsigned resident_blocks(const void *fn, int threads, size_t lds);
     std::vector<hipEvent_t> events;
     bool dev_ok = false;
+    // The carries of the last normalisation are not yet in register 0: the next squaring's column
+    // forward pass adds them on the way in (plans with compile-time passes); every other use of the
+    // registers runs the carry kernel first (flush_carry).
+    bool carry_pending = false;
+    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused && !plan.fuse_sq; }
+    void flush_carry();
 
     ~Engine();
     int init(uint32_t b, uint32_t n, uint32_t num_regs, int device);

@@ -553,10 +553,16 @@ unsigned Engine::resident_blocks(const void *fn, int threads, size_t lds) {
 
 // Forward column pass from register `src` into `dst` (ws or zp).
 void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
+    const bool absorb = carry_pending && src == 0 && defer_carry();
+    if (carry_pending && !absorb) flush_carry();
     if (plan.spec) {
         ColArgs A;
         A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
         A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
+        if (absorb) {  // the caller overwrites register 0 (square, mul)
+            A.wc = carry; A.b = b;
+            carry_pending = false;
+        }
         const uint32_t C = 1u << (n - plan.a);
         const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
         hipLaunchKernelGGL(spec_col_fwd(is32, plan.col_loge, plan.a), dim3(C / plan.col_ng, rns, 1), dim3(T),
@@ -669,6 +675,13 @@ void Engine::launch_norm(int dup) {
     } else {
         hipLaunchKernelGGL(K.norm, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
     }
+    if (defer_carry()) carry_pending = true;
+    else launch_carry();
+}
+
+void Engine::flush_carry() {
+    if (!carry_pending) return;
+    carry_pending = false;
     launch_carry();
 }
 
@@ -689,6 +702,7 @@ void Engine::launch_fused(int dup) {
 
 void Engine::square(int dup) {
     if (plan.fuse_sq) {
+        flush_carry();
         launch_fused(dup);
         return;
     }
@@ -706,11 +720,13 @@ void Engine::mul() {
 }
 
 void Engine::init_multiplicand(uint32_t src) {
+    flush_carry();  // register 0 is read but not rewritten
     launch_col_fwd(src, zp);
     launch_row(ROW_FWD, zp);
 }
 
 void Engine::mul_scalar(int32_t a) {
+    flush_carry();
     MulScalarArgs A;
     A.dst = regs; A.carry = carry; A.b = b; A.a = a;
     const unsigned nwg = (unsigned)(N / (8 * 256));
@@ -791,6 +807,7 @@ int gfe_destroy(gfe_engine *e) {
 
 int gfe_set(gfe_engine *e, uint32_t a) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     e->eng.set(a);
     return e->eng.check("set");
 }
@@ -828,6 +845,7 @@ int gfe_mul(gfe_engine *e) {
 
 int gfe_copy(gfe_engine *e, uint32_t dst, uint32_t src) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     Engine &g = e->eng;
     if (dst >= g.num_regs || src >= g.num_regs) return set_error(GFE_EINVAL, "register index out of range");
     if (dst == src) return GFE_OK;
@@ -837,6 +855,7 @@ int gfe_copy(gfe_engine *e, uint32_t dst, uint32_t src) {
 
 int gfe_get_digits(gfe_engine *e, uint32_t reg, int32_t *out) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     Engine &g = e->eng;
     if (!out) return set_error(GFE_EINVAL, "null output");
     if (reg >= g.num_regs) return set_error(GFE_EINVAL, "register index out of range");
@@ -848,6 +867,7 @@ int gfe_get_digits(gfe_engine *e, uint32_t reg, int32_t *out) {
 
 int gfe_set_digits(gfe_engine *e, uint32_t reg, const int32_t *in) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     Engine &g = e->eng;
     if (!in) return set_error(GFE_EINVAL, "null input");
     if (reg >= g.num_regs) return set_error(GFE_EINVAL, "register index out of range");
@@ -887,6 +907,7 @@ size_t gfe_context_size(const gfe_engine *e, uint32_t num_regs) {
 // reference's device registers do between squarings (set_int: d < 0 -> d + p).
 int gfe_save(gfe_engine *e, uint32_t num_regs, void *buf, size_t size) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     Engine &g = e->eng;
     if (num_regs == 0) num_regs = g.num_regs;
     if (num_regs > g.num_regs) return set_error(GFE_EINVAL, "num_regs");
@@ -933,6 +954,7 @@ static void balance_digits(int32_t *d, size_t N, uint32_t b) {
 // balanced here, as the engine keeps them.
 int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     Engine &g = e->eng;
     if (num_regs == 0) num_regs = g.num_regs;
     if (num_regs > g.num_regs) return set_error(GFE_EINVAL, "num_regs");
@@ -981,6 +1003,7 @@ int gfe_info(const gfe_engine *e, char *buf, size_t len) {
 
 int gfe_sync(gfe_engine *e) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     if (hipStreamSynchronize(e->eng.stream) != hipSuccess) return set_error(GFE_EHIP, "sync");
     return e->eng.check("sync");
 }
@@ -1039,6 +1062,7 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
 // residue (register 0 is saved and restored); the fastest is kept.
 int gfe_tune(gfe_engine *e, uint32_t reps) {
     GFE_GUARD(e);
+    e->eng.flush_carry();
     Engine &g = e->eng;
     if (reps == 0) reps = 8;
     std::vector<int32_t> saved(g.N), trial(g.N);
@@ -1059,6 +1083,7 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
         g.plan = P;
         if (hipMemcpyAsync(g.regs, trial.data(), g.N * 4, hipMemcpyHostToDevice, g.stream) != hipSuccess)
             return set_error(GFE_EHIP, "tune upload");
+        g.carry_pending = false;  // register 0 was just rewritten
         g.square(0);
         (void)hipEventRecord(g.events[0], g.stream);
         for (uint32_t i = 0; i < reps; ++i) g.square(0);
@@ -1073,6 +1098,7 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
         if (t < best_ms) { best_ms = t; best = P; }
     }
     g.plan = best;
+    g.carry_pending = false;
     if (hipMemcpyAsync(g.regs, saved.data(), g.N * 4, hipMemcpyHostToDevice, g.stream) != hipSuccess ||
         hipStreamSynchronize(g.stream) != hipSuccess)
         return set_error(GFE_EHIP, "tune restore");
@@ -1090,6 +1116,7 @@ int gfe_set_plan(gfe_engine *e, const int32_t plan[6]) {
     if (!plan) return set_error(GFE_EINVAL, "set_plan");
     gfe::Plan P;
     if (!gfe::plan_from_array(plan, e->eng.n, &P)) return set_error(GFE_EINVAL, "plan not admissible for this n");
+    e->eng.flush_carry();
     if (hipStreamSynchronize(e->eng.stream) != hipSuccess) return set_error(GFE_EHIP, "set_plan sync");
     e->eng.plan = P;
     return GFE_OK;

@@ -759,6 +759,8 @@ struct ColArgs {
     const uint32_t *tw;   // [prime][N], Montgomery form
     PrimeArgs pr;
     int n, a, ng;
+    const int64_t *wc = nullptr;  // forward: workgroup carries still to add into src (carry_wg_body), or null
+    uint32_t b = 0;
 };
 
 // 32 points per thread need more than the 128 registers a 1024-thread workgroup leaves: such plans

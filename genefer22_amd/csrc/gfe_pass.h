@@ -229,11 +229,58 @@ __host__ __device__ inline size_t col_pass_lds_words(int logg, int loge, int ng)
     return (size_t)ng * col_pass_stride(logg, loge, ng);
 }
 
-// Forward levels 0..a-1 of one prime: signed digits `src` -> residues `dst` (both N values).
+// Digits per workgroup of the normalisation (256 threads x 8): its carry-outs wc[] enter the next
+// range's first 8 digits.
+constexpr uint32_t NORM_RANGE = 2048;
+
+// Adds the pending normalisation carries (carry_wg_body's work) into the digits a column workgroup
+// has just loaded, so that the separate carry kernel can be skipped before a squaring.  Range
+// g = i / 2048 starts at digit i = 2048 g, row i / C, column i mod C: with C >= 2048 at the columns
+// k 2048 of every row, with C < 2048 at column 0 of every (2048 / C)-th row; its first 8 digits are
+// columns c0..c0+7 of that row, held by lanes q < 8 of the workgroup whose first column c0 is the
+// range start.  The workgroup's threads take one such row each (rows r = thread, thread + T, ...):
+// re-read its 8 digits, add the carry of range g - 1 (-carry of the last range for g = 0:
+// x^N = -1) exactly as carry_wg_body does, and leave them in LDS for lanes q < 8.  The patch area
+// (2^LOGG x 8 words) is released by the barrier that precedes the pass's first LDS exchange.
+template <int LOGE, int LOGG>
+__device__ __forceinline__ void col_absorb_carries(int32_t (&dg)[1 << LOGE], const int32_t *__restrict__ src,
+                                                   const int64_t *__restrict__ wc, uint32_t b, uint32_t C,
+                                                   uint32_t c0, uint32_t q, uint32_t base, int YL, uint32_t nrange,
+                                                   uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    const bool starts = (C >= NORM_RANGE) ? (c0 % NORM_RANGE) == 0 : c0 == 0;
+    if (!starts) return;  // uniform over the workgroup
+    const uint32_t every = (C >= NORM_RANGE) ? 1u : NORM_RANGE / C;
+    int32_t *P = reinterpret_cast<int32_t *>(lds);
+    const double binv = 1.0 / (double)b;
+    for (uint32_t r = threadIdx.x; r < (1u << LOGG); r += blockDim.x) {
+        if (r % every) continue;
+        const uint32_t g = (uint32_t)(((size_t)r * C + c0) / NORM_RANGE);
+        const int64_t cin = (g == 0) ? -wc[nrange - 1] : wc[g - 1];
+        const int32_t *p = src + (size_t)r * C + c0;
+        const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);
+        int32_t e8[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        if (cin != 0) absorb_carry<8>(e8, cin, b, binv);
+        *reinterpret_cast<int4 *>(P + r * 8) = make_int4(e8[0], e8[1], e8[2], e8[3]);
+        *reinterpret_cast<int4 *>(P + r * 8 + 4) = make_int4(e8[4], e8[5], e8[6], e8[7]);
+    }
+    __syncthreads();
+    if (q < 8) {
+#pragma unroll
+        for (int rr = 0; rr < E; ++rr) {
+            const uint32_t r = base + ((uint32_t)rr << YL);
+            if (r % every == 0) dg[rr] = P[r * 8 + q];
+        }
+    }
+}
+
+// Forward levels 0..a-1 of one prime: signed digits `src` -> residues `dst` (both N values).  With
+// `wc`, the carries of the last normalisation are added on the way in (col_absorb_carries).
 template <class M, int LOGE, int LOGG>
 __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                              const uint32_t *__restrict__ tw, uint32_t C, int NG, unsigned blk,
-                                             uint32_t *lds) {
+                                             uint32_t *lds, const int64_t *__restrict__ wc = nullptr,
+                                             uint32_t b = 0) {
     using GM = Geo<LOGG, LOGE>;
     constexpr int E = 1 << LOGE;
     const uint32_t t = threadIdx.x, q = t & (uint32_t)(NG - 1), s = t / (uint32_t)NG;
@@ -247,8 +294,14 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
         load_fwd_tw<LOGE>(w, tw, (1u << lam0) + hi, NE);
         if constexpr (ci == 0) {
             const uint32_t off = (base * C + c) * 4u;
+            int32_t dg[E];
 #pragma unroll
-            for (int rr = 0; rr < E; ++rr) x[rr] = to_res(*at_bytes(src + (size_t)((uint32_t)rr << YL) * C, off), md.p);
+            for (int rr = 0; rr < E; ++rr) dg[rr] = *at_bytes(src + (size_t)((uint32_t)rr << YL) * C, off);
+            if (wc)
+                col_absorb_carries<LOGE, LOGG>(dg, src, wc, b, C, blk * (uint32_t)NG, q, base, YL,
+                                               (uint32_t)(((size_t)C << LOGG) / NORM_RANGE), lds);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = to_res(dg[rr], md.p);
         } else {
             __syncthreads();
             const uint32_t *p = L + base + (base >> LOGE);

@@ -14,7 +14,7 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd(ColArgs A) 
     const M md{A.pr.p[prime], A.pr.q[prime]};
     const size_t N = (size_t)1 << A.n;
     col_fwd_pass<M, LOGE, LOGG>(md, A.src, A.ws + prime * N, A.tw + prime * N, (uint32_t)(N >> LOGG), A.ng,
-                                xcd_block(blockIdx.x, gridDim.x), lds);
+                                xcd_block(blockIdx.x, gridDim.x), lds, A.wc, A.b);
 }
 
 template <class M, int LOGE, int LOGG>

@@ -108,6 +108,29 @@ def test_batch_graphs_equal_eager(gfe):
         assert out[True][tag][1] == gfe.run_prp(b, 12, iters=it, b_gl=-1)["res64"], b
 
 
+@pytest.mark.parametrize("n,count", [(12, 40), (13, 160)])
+def test_batch_fused_squaring_by_occupancy(gfe, n, count):
+    """With the single-workgroup squaring in the plan: at n = 13 blocks of 128 or more active slots
+    run it and blocks of fewer run the four-step passes (the tail of the run, as candidates finish
+    one by one); at n = 12 every block runs it.  Every residue equals its single run."""
+    from genefer22_amd import workloads
+    bases = workloads.seeded_bases(1300 + n, 1024, 2_000_000_000, count)
+    bt = gfe.Batch(n, count)
+    assert bt.get_plan()[5] & 16
+    iters = [200 + 3 * i for i in range(count)]
+    for i, b in enumerate(bases):
+        bt.submit(b, iters=iters[i], tag=i)
+    done = []
+    while len(done) < count:
+        bt.step()
+        done += bt.poll()
+    bt.close()
+    by_tag = {r["tag"]: r for r in done}
+    for i in (0, 1, count // 2, count - 2, count - 1):
+        alone = gfe.run_prp(bases[i], n, iters=iters[i], b_gl=-1)
+        assert (by_tag[i]["res64"], by_tag[i]["old64"]) == (alone["res64"], alone["old64"]), (i, bases[i])
+
+
 def test_batch_graph_cache_turnover(gfe):
     """Candidates that finish highest slot first shrink the grid depth block after block, so each
     block needs a graph of its own depth and the cache of executable graphs turns over many times
