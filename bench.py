@@ -8,6 +8,7 @@ One JSON line on stdout (progress on stderr):
                  timed with HIP events on the engine's stream;
   * roofline   : the dominant kernel's algorithmic bytes (one read + one write of the rns x N uint32
                  residues, 8 rns N bytes per launch) over its average launch duration, against 8 TB/s;
+  * passes     : every pass's algorithmic HBM rate (and the whole squaring's) against 8 TB/s;
   * valu       : the same kernel's butterfly rate and estimated VALU-slot use (integer-bound path);
   * batched_n15: candidates/hour at n = 15, 512 seeded candidates in one batch vs one candidate per
                  launch, from timed iteration blocks (full-test length = esize squarings);
@@ -86,11 +87,19 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     dom = max(range(4), key=lambda i: km[i])
     if fused:  # digits in, residues out | row | residues in, digits out | segment carries
         ng = t.get_plan()[2]
-        alg_bytes = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 4 * rns * N + 4 * N + 8 * N // ng,
-                     3: 16 * N // ng}[dom]
+        alg = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 4 * rns * N + 4 * N + 8 * N // ng, 3: 16 * N // ng}
     else:
-        alg_bytes = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 8 * rns * N, 3: 4 * rns * N + 4 * N}[dom]
+        alg = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 8 * rns * N, 3: 4 * rns * N + 4 * N}
+    alg_bytes = alg[dom]
     achieved = alg_bytes / (km[dom] * 1e-3) / 1e9
+    # achieved algorithmic HBM rate of every pass (one read + one write of what the pass must move)
+    passes = {names[i]: {"us": round(km[i] * 1e3, 2), "alg_bytes": alg[i],
+                         "GBps": round(alg[i] / (km[i] * 1e-3) / 1e9, 1),
+                         "frac": round(alg[i] / (km[i] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for i in range(4)}
+    sq_bytes = sum(alg.values())
+    passes["squaring"] = {"us": round(ms / steps * 1e3, 2), "alg_bytes": sq_bytes,
+                          "GBps": round(sq_bytes / (ms / steps * 1e-3) / 1e9, 1),
+                          "frac": round(sq_bytes / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     # butterflies of the dominant pass (row: (n - a) levels fwd + inv, N/2 per level per prime)
     a = t.get_plan()[0]
     levels = {0: a, 1: 2 * (n - a), 2: a}.get(dom, 0)
@@ -101,6 +110,7 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     res = {
         "b": b, "plan": plan, "plan_tuple": list(t.get_plan()), "ms_per_step": ms / steps, "ns_per_squaring": ms / steps * 1e6,
         "kernel_us": {names[i]: round(km[i] * 1e3, 2) for i in range(4)},
+        "passes": passes,
         "roofline": {
             "bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": meas.get("traffic"),
@@ -302,6 +312,7 @@ def main():
             "batched_n15": batched, "serving": serving, "serving_light": serving_light, "parity": parity,
         },
         "roofline": res["roofline"],
+        "passes": res["passes"],
         "valu": res["valu"],
         "cpu_baseline": cpu,
     }

@@ -622,25 +622,13 @@ __device__ __forceinline__ void carry_tail(int32_t (&d)[S], int64_t cr, int32_t 
 
 // Scale, Garner CRT and balanced carry of one candidate: `ws` = its rns x N residues, `dst` = its
 // register 0, `wc` = its workgroup carries; workgroup blk covers digits [blk 256 S, (blk+1) 256 S).
+// Scale, Garner CRT, x2 (dup) and the balanced carry across one thread's S consecutive digits of
+// residues a[k][j]: d receives the digits (carry-in 0), cr the thread's carry-out.
 template <int RNS, class M, int S>
-__device__ __forceinline__ void norm_body(const CrtConst &c, const uint32_t *__restrict__ ws, int32_t *dst, int64_t *wc,
-                                          uint32_t b, bool dup, int n, unsigned blk) {
-    const size_t N = (size_t)1 << n;
-    const size_t i0 = ((size_t)blk * 256 + threadIdx.x) * S;
-    const double binv = 1.0 / (double)b;
-    const M m1{c.p[0], c.q[0]}, m2{c.p[1], c.q[1]}, m3{c.p[2], c.q[2]};
-    const uint32_t *w0 = ws + i0;
-    uint32_t a[MAX_RNS][S];
-#pragma unroll
-    for (int k = 0; k < RNS; ++k) {
-#pragma unroll
-        for (int j = 0; j < S; j += 4) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(w0 + (size_t)k * N + j);
-            a[k][j] = v.x; a[k][j + 1] = v.y; a[k][j + 2] = v.z; a[k][j + 3] = v.w;
-        }
-    }
-    int32_t d[S];
-    int64_t cr = 0;
+__device__ __forceinline__ void norm_digits(const CrtConst &c, const M &m1, const M &m2, const M &m3,
+                                            const uint32_t (&a)[MAX_RNS][S], bool dup, uint32_t b, double binv,
+                                            int32_t (&d)[S], int64_t &cr) {
+    cr = 0;
     if (c.fast) {
         double cd = 0.0;
         uint32_t cl = 0;
@@ -663,11 +651,35 @@ __device__ __forceinline__ void norm_body(const CrtConst &c, const uint32_t *__r
             d[j] = div_bal(x64, xd, b, binv, &cr);
         }
     }
+}
+
+// Scale, Garner CRT and balanced carry of one candidate: `ws` = its rns x N residues, `dst` = its
+// register 0, `wc` = its workgroup carries; workgroup blk covers digits [blk 256 S, (blk+1) 256 S).
+template <int RNS, class M, int S>
+__device__ __forceinline__ void norm_body(const CrtConst &c, const uint32_t *__restrict__ ws, int32_t *dst, int64_t *wc,
+                                          uint32_t b, bool dup, int n, unsigned blk) {
+    const size_t N = (size_t)1 << n;
+    const size_t i0 = ((size_t)blk * 256 + threadIdx.x) * S;
+    const double binv = 1.0 / (double)b;
+    const M m1{c.p[0], c.q[0]}, m2{c.p[1], c.q[1]}, m3{c.p[2], c.q[2]};
+    const uint32_t *w0 = ws + i0;
+    uint32_t a[MAX_RNS][S];
+#pragma unroll
+    for (int k = 0; k < RNS; ++k) {
+#pragma unroll
+        for (int j = 0; j < S; j += 4) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(w0 + (size_t)k * N + j);
+            a[k][j] = v.x; a[k][j + 1] = v.y; a[k][j + 2] = v.z; a[k][j + 3] = v.w;
+        }
+    }
+    int32_t d[S];
+    int64_t cr;
+    norm_digits<RNS, M, S>(c, m1, m2, m3, a, dup, b, binv, d, cr);
     carry_tail<S>(d, cr, dst, i0, wc, blk, b, binv);
 }
 
-// Persistent normalisation: workgroup blocks blk = first, first + stride, ... (< N / (256 S)),
-// the next block's residues loaded while the current block is converted.
+// Normalisation of several ranges per workgroup: blocks blk = first, first + stride, ...
+// (< N / (256 S)), the next block's residues loaded while the current block is converted.
 template <int RNS, class M, int S>
 __device__ __forceinline__ void norm_loop(const CrtConst &c, const uint32_t *__restrict__ ws, int32_t *dst, int64_t *wc,
                                           uint32_t b, bool dup, int n, unsigned first, unsigned stride) {
@@ -693,19 +705,8 @@ __device__ __forceinline__ void norm_loop(const CrtConst &c, const uint32_t *__r
         const bool more = blk + stride < nblk;
         if (more) load(an, blk + stride);
         int32_t d[S];
-        int64_t cr = 0;
-#pragma unroll
-        for (int j = 0; j < S; ++j) {
-            const uint32_t r1 = m1.mulw(a[0][j], c.scale[0]);
-            const uint32_t r2 = m2.mulw(a[1][j], c.scale[1]);
-            const uint32_t r3 = (RNS == 3) ? m3.mulw(a[RNS - 1][j], c.scale[2]) : 0u;
-            uint64_t x64; double xd;
-            garner<RNS>(c, m1, m2, m3, r1, r2, r3, &x64, &xd);
-            if (dup) { x64 <<= 1; xd *= 2.0; }
-            x64 += (uint64_t)cr;
-            xd += (double)cr;
-            d[j] = div_bal(x64, xd, b, binv, &cr);
-        }
+        int64_t cr;
+        norm_digits<RNS, M, S>(c, m1, m2, m3, a, dup, b, binv, d, cr);
         carry_tail<S>(d, cr, dst, ((size_t)blk * 256 + threadIdx.x) * S, wc, blk, b, binv);
         __syncthreads();  // carry_tail's LDS is rewritten by the next block
         if (more) {

@@ -216,13 +216,16 @@ __host__ __device__ inline size_t row_pass_lds_words(int logg, int loge, int ng)
 // Column passes: levels 0..a-1 (a = LOGG) down NG adjacent columns c = blk NG + q of the
 // 2^a x C matrix (element (k, c) at k C + c).  Thread t -> column q = t mod NG, sub-transform
 // s = t / NG.  LDS: column q at q SQ, element k at k + (k >> LOGE), with the column stride SQ
-// = 2^a + 2^a / E + pad = 64 / NG (mod 64): lanes of a wavefront (NG consecutive columns by
-// 64 / NG consecutive sub-transforms) then fall in distinct banks in every chunk.
+// = 2^a + 2^a / E + pad = 32 / NG (mod 32; odd for NG >= 32).  gfx950 serves a wave's 32-bit LDS
+// accesses half a wave at a time over 32 banks (the row pass's pattern, conflict-free in that
+// model and not in a 64-bank one, measures no conflicts; a stride of 64 / NG mod 64 measured
+// conflicts in the column passes): the 32 lanes of each half (NG consecutive columns by 32 / NG
+// consecutive sub-transforms) then fall in distinct banks in every chunk.
 
 __host__ __device__ inline uint32_t col_pass_stride(int logg, int loge, int ng) {
     const uint32_t w = (1u << logg) + (1u << (logg - loge));
-    const uint32_t want = (ng >= 64) ? 1u : 64u / (uint32_t)ng;
-    return w + ((want - (w & 63u)) & 63u);
+    const uint32_t want = (ng >= 32) ? 1u : 32u / (uint32_t)ng;
+    return w + ((want - (w & 31u)) & 31u);
 }
 
 __host__ __device__ inline size_t col_pass_lds_words(int logg, int loge, int ng) {
