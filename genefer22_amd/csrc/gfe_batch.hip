@@ -332,6 +332,15 @@ struct Batch {
     // round sooner (one candidate: 76 ms against 166 ms for 4096 squarings on MI355X).  At n = 12
     // it is the faster path at any occupancy.
     bool fused_block(uint32_t nact) const { return plan.fuse_sq && (n <= 12 || nact >= 128u); }
+    // Squaring rounds leave the normalisation's workgroup carries in `carry` and the next round's
+    // column forward pass adds them (compile-time plans); carry_wg_body clears what it applies, so
+    // `carry` always holds carries not yet in register 0, or zeros, and a squaring round may absorb
+    // it unconditionally.  Register operations, the single-workgroup squaring and plan changes
+    // apply the pending carries first (flush_carries).
+    bool carries_pending = false;
+    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused; }
+    void flush_carries();
+    int zero_carries();
     std::vector<Slot> slots;
     std::deque<gfe_batch_result> finished;
     double t0 = 0;
@@ -429,6 +438,7 @@ hipGraphExec_t Batch::block_graph(int rounds) {
 // occupancy queries, which stay out of captures).
 int Batch::run_block(int rounds, uint32_t nact) {
     if (fused_block(nact)) {  // one launch runs the whole block, one workgroup per slot
+        flush_carries();
         FusedArgs A;
         memset(&A, 0, sizeof(A));
         A.regs = regs; A.tw = tw; A.crt = crt; A.slots = slots_dev; A.act = act_dev; A.dupf = dup_dev;
@@ -437,6 +447,7 @@ int Batch::run_block(int rounds, uint32_t nact) {
                            stream, A);
         return check("fused square block");
     }
+    if (defer_carry()) carries_pending = true;
     if (use_graphs && !plan.pipe) {
         if (hipGraphExec_t g = block_graph(rounds)) return check_hip(hipGraphLaunch(g, stream), "graph launch");
         return set_error(GFE_EHIP, "squaring block graph capture failed");
@@ -532,6 +543,7 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     if (hipMemcpy(slots_dev, si.data(), cap * sizeof(SlotInfo), hipMemcpyHostToDevice) != hipSuccess)
         return set_error(GFE_EHIP, "slot upload");
     if (hipMemsetAsync(regs, 0, (size_t)cap * 3 * N * 4, stream) != hipSuccess ||
+        hipMemsetAsync(carry, 0, (size_t)cap * (N >> 3) * 8, stream) != hipSuccess ||
         hipMemsetAsync(act_dev, 0, (size_t)64 * cap, stream) != hipSuccess ||
         hipMemsetAsync(dup_dev, 0, (size_t)64 * cap, stream) != hipSuccess)
         return set_error(GFE_EHIP, "memset");
@@ -681,12 +693,27 @@ void Batch::launch_norm_carry(const BArgs &A0) {
     } else {
         hipLaunchKernelGGL(kb_norm, dim3(nwg, 1, zdim()), dim3(256), 0, stream, A);
     }
+    if (!A.absorb) hipLaunchKernelGGL(kb_carry_wg, dim3((nwg + 255) / 256, 1, zdim()), dim3(256), 0, stream, A);
+}
+
+void Batch::flush_carries() {
+    if (!carries_pending) return;
+    carries_pending = false;
+    const BArgs A = args(nullptr, nullptr);
+    const unsigned nwg = (unsigned)(N >> 11);
     hipLaunchKernelGGL(kb_carry_wg, dim3((nwg + 255) / 256, 1, zdim()), dim3(256), 0, stream, A);
+}
+
+// Clears every slot's carries (after a plan whose passes leave other values in `carry`).
+int Batch::zero_carries() {
+    carries_pending = false;
+    return check_hip(hipMemsetAsync(carry, 0, (size_t)cap * (N >> 3) * 8, stream), "carry clear");
 }
 
 void Batch::square(const uint8_t *active, const uint8_t *dup) {
     BArgs A = args(active, dup);
     A.src_reg = 0;
+    A.absorb = defer_carry() ? 1 : 0;
     launch_col_fwd(A, 0);
     launch_row(A, ROW_SQUARE);
     launch_col_inv(A);
@@ -960,6 +987,7 @@ int Batch::step(int32_t *active_out) {
             groups[std::make_tuple(it.kind, it.a, it.b)].push_back((int)s);
         }
         if (groups.empty()) break;
+        flush_carries();
         for (auto &kv : groups) {
             const int kind = std::get<0>(kv.first);
             const uint32_t a = std::get<1>(kv.first), b2 = std::get<2>(kv.first);
@@ -1153,7 +1181,9 @@ int gfe_batch_set_plan(gfe_batch *b, const int32_t plan[6]) {
     if (!b || !plan) return set_error(GFE_EINVAL, "set_plan");
     gfe::Plan P;
     if (!gfe::plan_from_array(plan, b->bt.n, &P)) return set_error(GFE_EINVAL, "plan not admissible for this n");
-    if (hipSetDevice(b->bt.device) != hipSuccess || hipStreamSynchronize(b->bt.stream) != hipSuccess)
+    if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
+    b->bt.flush_carries();
+    if (b->bt.zero_carries() != GFE_OK || hipStreamSynchronize(b->bt.stream) != hipSuccess)
         return set_error(GFE_EHIP, "set_plan sync");
     b->bt.plan = P;
     b->bt.drop_graphs();
@@ -1236,6 +1266,7 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     for (auto &S : B.slots) any |= S.busy;
     if (!any) return set_error(GFE_EINVAL, "tune needs busy slots");
     const std::vector<gfe::Plan> plans = gfe::candidate_plans(B.n);
+    B.flush_carries();
     // squarings on the busy slots' r0: save and restore register 0 of every slot
     const size_t bytes = (size_t)B.cap * 3 * B.N * 4;
     int32_t *save = nullptr;
@@ -1248,6 +1279,7 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     B.use_graphs = false;  // plans are compared on eager launches (no capture per candidate plan)
     for (const gfe::Plan &P : plans) {
         B.plan = P;
+        if (B.zero_carries() != GFE_OK) break;
         double ms = 0;
         if (gfe_batch_time_rounds(b, 1, &ms) != GFE_OK) continue;
         if (gfe_batch_time_rounds(b, reps, &ms) != GFE_OK) continue;
@@ -1256,6 +1288,7 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     B.plan = best;
     B.use_graphs = graphs;
     B.drop_graphs();
+    (void)B.zero_carries();
     (void)hipMemcpyAsync(B.regs, save, bytes, hipMemcpyDeviceToDevice, B.stream);
     (void)hipStreamSynchronize(B.stream);
     (void)hipFree(save);

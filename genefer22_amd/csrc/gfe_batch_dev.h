@@ -25,6 +25,7 @@ struct BArgs {
     int n, a, ng;
     uint32_t src_reg, dst_reg;
     int32_t value;
+    int32_t absorb;          // squaring rounds: the column forward pass adds the pending carries
 };
 
 __device__ __forceinline__ int bslot(const BArgs &A) {

@@ -37,7 +37,7 @@ int set_error(int code, const std::string &msg) {
 
 struct CarryArgs {
     int32_t *dst;
-    const int64_t *carry;
+    int64_t *carry;
     uint32_t b;
     unsigned nwg;   // workgroups of the carry pass (each 256 x 8 digits)
 };

@@ -737,11 +737,16 @@ __device__ __forceinline__ void mulscalar_body(int32_t *dc, int64_t *wc, uint32_
 }
 
 // Adds the carry-out of workgroup w - 1 (of the last one, negated, for w = 0: x^N = -1) into the
-// first 8 digits of workgroup w's range (256 x 8 digits per workgroup).
-__device__ __forceinline__ void carry_wg_body(int32_t *dc, const int64_t *wc, unsigned nwg, uint32_t b, unsigned w) {
+// first 8 digits of workgroup w's range (256 x 8 digits per workgroup), and clears it: a carry
+// array holds only carries not yet in the digits (or zeros), so a later absorption of it is exact.
+// Each entry is read by one thread only.
+__device__ __forceinline__ void carry_wg_body(int32_t *dc, int64_t *wc, unsigned nwg, uint32_t b, unsigned w) {
     constexpr int S = 8;
-    const int64_t cin = (w == 0) ? -wc[nwg - 1] : wc[w - 1];
-    if (cin == 0) return;
+    const unsigned src = (w == 0) ? nwg - 1 : w - 1;
+    const int64_t c = wc[src];
+    if (c == 0) return;
+    wc[src] = 0;
+    const int64_t cin = (w == 0) ? -c : c;
     int32_t *p = dc + (size_t)w * 256 * S;
     int32_t e[S];
     const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);

@@ -22,8 +22,11 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kbs_col_fwd(BArgs A, i
     const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
     const uint32_t C = (uint32_t)(N >> LOGG);
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-    if (si.is32) col_fwd_pass<ModU, LOGE, LOGG>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, C, A.ng, blk, lds);
-    else col_fwd_pass<ModS, LOGE, LOGG>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, C, A.ng, blk, lds);
+    const int64_t *wc = A.absorb ? A.carry + (size_t)s * (N >> 3) : nullptr;
+    if (si.is32)
+        col_fwd_pass<ModU, LOGE, LOGG>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, C, A.ng, blk, lds, wc, si.b);
+    else
+        col_fwd_pass<ModS, LOGE, LOGG>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, C, A.ng, blk, lds, wc, si.b);
 }
 
 template <int LOGE, int LOGG>
