@@ -129,7 +129,7 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     return res
 
 
-def bench_batched_n15(G, count: int, block: int, tune: bool = True):
+def bench_batched_n15(G, count: int, block: int, tune: bool = True, plan=None):
     """Candidates/hour at n = 15: `count` seeded candidates in one batch vs one candidate per launch.
 
     Both legs time squaring iterations with HIP events on resident state; candidates/hour =
@@ -143,10 +143,15 @@ def bench_batched_n15(G, count: int, block: int, tune: bool = True):
         if bt.submit(b, 0, i) is None:
             raise RuntimeError("batch refused a candidate")
     bt.step()  # the first 64-iteration block of every candidate's program
-    plan_b = bt.tune(2) if tune else bt.info()
+    if plan:
+        bt.set_plan(plan)
+        plan_b = bt.info()
+    else:
+        plan_b = bt.tune(2) if tune else bt.info()
     log(f"[bench] batch plan: {plan_b}")
     bt.time_rounds(block)
     per_iter_b = bt.time_rounds(4 * block) / (4 * block)
+    plan_tuple_b = list(bt.get_plan())
     bt.close()
     plain_ms = []
     for b in bases[:4]:
@@ -161,7 +166,7 @@ def bench_batched_n15(G, count: int, block: int, tune: bool = True):
     cand_h_batched = 3600.0 / (per_iter_b * 1e-3 * mean_es) * count
     cand_h_plain = 3600.0 / (per_sq_plain * 1e-3 * mean_es)
     return {
-        "candidates": count, "mean_esize": round(mean_es), "block": block, "batch_plan": plan_b,
+        "candidates": count, "mean_esize": round(mean_es), "block": block, "batch_plan": plan_b, "batch_plan_tuple": plan_tuple_b,
         "batched_ms_per_iteration": round(per_iter_b, 4), "plain_ms_per_squaring": round(per_sq_plain, 5),
         "candidates_per_hour_batched": round(cand_h_batched, 1),
         "candidates_per_hour_plain": round(cand_h_plain, 1),
@@ -250,6 +255,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the n=15 batched leg")
     ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--batch-plan", default=None, help="fixed n=15 batch plan a,cl,cng,rl,rng,flags (default: autotuned)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -257,7 +263,10 @@ def main():
     ap.add_argument("--light-rate", type=float, default=8.0, help="arrivals/s of the light-load serving replay")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plan", default=None, help="fixed n=22 pass plan a,cl,cng,rl,rng,flags (default: the library's)")
-    ap.add_argument("--no-tune", action="store_true", help="time the library's default plans instead of the autotuned ones")
+    ap.add_argument("--no-tune", action="store_true", help="time the library's default batch plan instead of the autotuned one")
+    ap.add_argument("--tune-n22", action="store_true",
+                    help="autotune the n=22 plan (default: the library's default plan, which the tuner picks within "
+                         "run-to-run noise; a fixed plan keeps the line comparable with the committed profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): candidates are independent and shard "
@@ -266,10 +275,11 @@ def main():
 
     import genefer22_amd as G
     G.lib()
-    res = bench_n22(G, args.steps, args.warmup, args.plan, not args.no_tune)
+    res = bench_n22(G, args.steps, args.warmup, args.plan, args.tune_n22)
     log(f"[bench] n=22: {res['ns_per_squaring']:.0f} ns/squaring; kernels {res['kernel_us']}")
 
-    batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64, not args.no_tune)
+    bplan = [int(v) for v in args.batch_plan.split(",")] if args.batch_plan else None
+    batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64, not args.no_tune, bplan)
     if batched:
         log(f"[bench] n=15: {batched}")
 

@@ -336,9 +336,13 @@ struct Batch {
     // column forward pass adds them (compile-time plans); carry_wg_body clears what it applies, so
     // `carry` always holds carries not yet in register 0, or zeros, and a squaring round may absorb
     // it unconditionally.  Register operations, the single-workgroup squaring and plan changes
-    // apply the pending carries first (flush_carries).
+    // apply the pending carries first (flush_carries).  Only for rows of 2^11 or more digits: with
+    // shorter rows every (2^11 / row)-th row starts a carry range, an eighth or more of the column
+    // workgroups take the extra step and the column pass loses what the carry kernel cost (n = 15,
+    // 512 slots, measured on MI355X: 0.367 ms per round either way; n = 20, one slot: 52.9 us against
+    // 54.8 us with the carry kernel).
     bool carries_pending = false;
-    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused; }
+    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused && (int)n - plan.a >= 11; }
     void flush_carries();
     int zero_carries();
     std::vector<Slot> slots;

@@ -127,22 +127,39 @@ __device__ __forceinline__ T *at_bytes(T *base, uint32_t bytes) {
 // ---------------------------------------------------------------------------------------------
 // Row pass: levels a..n-1 on NG rows of G = 2^LOGG contiguous residues, the pointwise square or
 // product, inverse levels n-1..a (ROW_FWD: forward only).  Thread t -> row q = t / (G/E) of the
-// block's rows (blk NG + q), sub-transform s = t mod (G/E).  LDS: row q at q (G + G/E), element k
-// at k + (k >> LOGE).  `data`, `zp` and `tw` point at one prime's N values.
+// block's rows (blk NG + q), sub-transform s = t mod (G/E).  LDS: row q at q (G + G / 2^PAD),
+// element k at k + (k >> PAD), PAD = row_pad(LOGG, LOGE).  `data`, `zp` and `tw` point at one
+// prime's N values.
+//
+// Bank conflicts (32 banks, half a wave at a time): with 16 points per thread and rows of 2^10+
+// points, no padding k + (k >> P) serves both sides of the exchange into the chunk of stride 2^4
+// (lanes 32 consecutive elements apart on one side, two groups of 16 2^(LOGG-lam0) apart on the
+// other).  Padding every 32 elements and letting that chunk take the sub-transforms in the order
+// of s with bits 4 and 5 swapped (the two 16-lane groups of a half wave then sit two blocks apart,
+// 16 banks apart) makes every exchange conflict-free; any per-chunk order of the sub-transforms
+// computes the same pass.
+__host__ __device__ constexpr int row_pad(int logg, int loge) { return (loge == 4 && logg >= 10) ? 5 : loge; }
+__host__ __device__ constexpr bool row_swap45(int logg, int loge, int yl) { return loge == 4 && logg >= 10 && yl == 4; }
+__device__ __forceinline__ uint32_t swap_bits45(uint32_t s) {
+    const uint32_t d = ((s >> 4) ^ (s >> 5)) & 1u;
+    return s ^ (d * 0x30u);
+}
+
 template <class M, int LOGE, int LOGG, int MODE>
 __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ zp,
                                          const uint32_t *__restrict__ tw, int a, int NG, unsigned blk, uint32_t *lds) {
     using GM = Geo<LOGG, LOGE>;
-    constexpr int E = 1 << LOGE, G = 1 << LOGG, SPR = G >> LOGE;
+    constexpr int E = 1 << LOGE, G = 1 << LOGG, SPR = G >> LOGE, PAD = row_pad(LOGG, LOGE);
     const uint32_t t = threadIdx.x, q = t / SPR, s = t % SPR;
     const uint32_t row = blk * (uint32_t)NG + q;
     const uint32_t rowb = row * (uint32_t)G * 4u;  // byte offset of the row
-    uint32_t *L = lds + q * (G + SPR);
+    uint32_t *L = lds + q * (G + (G >> PAD));
     uint32_t x[E], w[E];
     static_for<0, GM::NCH>([&](auto cc) {
         constexpr int ci = decltype(cc)::value;
         constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
-        const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+        const uint32_t sc = row_swap45(LOGG, LOGE, YL) ? swap_bits45(s) : s;
+        const uint32_t hi = sc >> YL, base = (hi << (LOGG - lam0)) + (sc & ((1u << YL) - 1u));
         load_fwd_tw<LOGE>(w, tw, (1u << (a + lam0)) + (row << lam0) + hi, NE);
         if constexpr (ci == 0) {
             const uint32_t *p = at_bytes(data, rowb + base * 4u);
@@ -150,16 +167,16 @@ __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ dat
             for (int rr = 0; rr < E; ++rr) x[rr] = p[rr << YL];
         } else {
             __syncthreads();
-            const uint32_t *p = L + base + (base >> LOGE);
+            const uint32_t *p = L + base + (base >> PAD);
 #pragma unroll
-            for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+            for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, PAD>(rr)];
         }
         fwd_chunk<M, LOGE, NE>(md, x, w);
         if constexpr (ci < GM::NCH - 1) {
             __syncthreads();
-            uint32_t *p = L + base + (base >> LOGE);
+            uint32_t *p = L + base + (base >> PAD);
 #pragma unroll
-            for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+            for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, PAD>(rr)] = x[rr];
         }
     });
     // the last forward chunk (YL = 0: E consecutive elements from s E) is the first inverse chunk
@@ -184,13 +201,14 @@ __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ dat
         static_for_down<0, GM::NCH>([&](auto cc) {
             constexpr int ci = decltype(cc)::value;
             constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
-            const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+            const uint32_t sc = row_swap45(LOGG, LOGE, YL) ? swap_bits45(s) : s;
+            const uint32_t hi = sc >> YL, base = (hi << (LOGG - lam0)) + (sc & ((1u << YL) - 1u));
             if constexpr (ci != LAST) {
                 load_inv_tw<LOGE>(w, tw, (1u << (a + lam0)) + (row << lam0) + hi, a + lam0, NE);
                 __syncthreads();
-                const uint32_t *p = L + base + (base >> LOGE);
+                const uint32_t *p = L + base + (base >> PAD);
 #pragma unroll
-                for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+                for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, PAD>(rr)];
             }
             inv_chunk<M, LOGE, NE>(md, x, w);
             if constexpr (ci == 0) {
@@ -199,9 +217,9 @@ __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ dat
                 for (int rr = 0; rr < E; ++rr) p[rr << YL] = x[rr];
             } else {
                 __syncthreads();
-                uint32_t *p = L + base + (base >> LOGE);
+                uint32_t *p = L + base + (base >> PAD);
 #pragma unroll
-                for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+                for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, PAD>(rr)] = x[rr];
             }
         });
     }
@@ -209,7 +227,7 @@ __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ dat
 
 // LDS words of a row pass workgroup.
 __host__ __device__ inline size_t row_pass_lds_words(int logg, int loge, int ng) {
-    return (size_t)ng * (((size_t)1 << logg) + ((size_t)1 << (logg - loge)));
+    return (size_t)ng * (((size_t)1 << logg) + ((size_t)1 << (logg - row_pad(logg, loge))));
 }
 
 // ---------------------------------------------------------------------------------------------
