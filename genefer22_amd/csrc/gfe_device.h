@@ -52,28 +52,25 @@ struct ModS {  // p < 2^31
 
 struct ModU {  // 2^31 < p < 2^32
     uint32_t p, q;
-    __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const {
-        const uint32_t s = a + b;
-        return ((s < a) | (s >= p)) ? s - p : s;
+    // x - y mod p for x, y < p: one subtraction whose borrow selects the corrected value
+    // (v_sub_co + v_add + v_cndmask; a + b may not fit 32 bits, so the sum is formed as a - (p - b)).
+    __device__ __forceinline__ uint32_t subr(uint32_t x, uint32_t y) const {
+        uint32_t d;
+        const bool borrow = __builtin_sub_overflow(x, y, &d);
+        return borrow ? d + p : d;
     }
-    __device__ __forceinline__ uint32_t sub(uint32_t a, uint32_t b) const {
-        const uint32_t d = a - b;
-        return (a < b) ? d + p : d;
-    }
+    __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) const { return subr(a, p - b); }
+    __device__ __forceinline__ uint32_t sub(uint32_t a, uint32_t b) const { return subr(a, b); }
     __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b) const {
         const uint64_t t = (uint64_t)a * b;
         const uint32_t m = (uint32_t)t * q;
-        const uint32_t hi = (uint32_t)(t >> 32), mh = __umulhi(m, p);
-        const uint32_t r = hi - mh;
-        return (hi < mh) ? r + p : r;
+        return subr((uint32_t)(t >> 32), __umulhi(m, p));
     }
     // a * w mod p with wm = w 2^32 mod p (Montgomery form of w).
     __device__ __forceinline__ uint32_t mulw(uint32_t a, uint32_t wm) const { return mont(a, wm); }
     // As ModS::mulp (wq = wm q mod 2^32).
     __device__ __forceinline__ uint32_t mulp(uint32_t a, uint32_t wm, uint32_t wq) const {
-        const uint32_t hi = __umulhi(a, wm), mh = __umulhi(a * wq, p);
-        const uint32_t r = hi - mh;
-        return (hi < mh) ? r + p : r;
+        return subr(__umulhi(a, wm), __umulhi(a * wq, p));
     }
 };
 
