@@ -46,8 +46,11 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kbs_col_inv(BArgs A, i
     else col_inv_pass<ModS, LOGE, LOGG>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, tw, C, A.ng, blk, lds);
 }
 
+// 32 points per thread with both prime sets' arithmetic in one kernel need up to 145 registers:
+// bounded at 128 (4 waves per SIMD) some shapes spill to scratch, so these run at 3 waves per SIMD
+// (n = 20, one slot: 52.3 us per round against 52.9 us with the spills).
 template <int LOGE, int LOGG>
-__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024, LOGE >= 5 ? 4 : 1) kbs_row_sq(BArgs A) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024, LOGE >= 5 ? 3 : 1) kbs_row_sq(BArgs A) {
     extern __shared__ uint32_t lds[];
     const int s = bslot(A);
     if (s < 0) return;
