@@ -27,8 +27,11 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_inv(ColArgs A) 
                                 xcd_block(blockIdx.x, gridDim.x), lds);
 }
 
+// 32 points per thread: rows of 2^11 and 2^13 points need 130-147 registers and spill when held to
+// 128 (4 waves per SIMD), so the bound is 3 waves; the other shapes stay within 128 registers
+// (2^14 points: 119) and keep 4 waves per SIMD.
 template <class M, int LOGE, int LOGG>
-__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024, LOGE >= 5 ? 4 : 1) ks_row_sq(RowArgs A) {
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024, LOGE >= 5 ? 3 : 1) ks_row_sq(RowArgs A) {
     extern __shared__ uint32_t lds[];
     const int prime = blockIdx.y;
     const M md{A.pr.p[prime], A.pr.q[prime]};
