@@ -174,3 +174,46 @@ def test_batch_default_small_n_squares_in_one_workgroup(gfe):
     for i in (0, 7, 19, 33, 39):
         alone = gfe.run_prp(bases[i], 12, iters=300, b_gl=-1)
         assert (by_tag[i]["res64"], by_tag[i]["old64"]) == (alone["res64"], alone["old64"]), bases[i]
+
+
+def test_batch_plan_changes_with_pending_carries(gfe):
+    """n = 19 (rows of 2^11 points and more: squaring rounds leave their carries for the next round's
+    column pass): plan changes between blocks — to another compile-time plan, to the all-prime
+    fused plan whose passes use the carry array for segment carries, and back — apply or clear
+    the pending carries, and every residue equals its single run."""
+    n = 19
+    bases = [1000000000, 1534, 30000000, 1999999266]
+    iters = 5 * 64 + 17
+    bt = gfe.Batch(n, len(bases))
+    base_plan = tuple(bt.get_plan())
+    assert base_plan[5] & 8 and n - base_plan[0] >= 11
+
+    def first_ok(cands):
+        for p in cands:
+            try:
+                bt.set_plan(p)
+                return p
+            except gfe.GfeError:
+                continue
+        raise AssertionError("no admissible plan")
+
+    for i, b in enumerate(bases):
+        bt.submit(b, iters=iters, tag=i)
+    schedule = [
+        [(a, cl, cng, rl, rng, 8) for a in (7, 6) for cl in (4, 3) for cng in (16, 8) for rl in (4, 5) for rng in (1, 2)],
+        [(a, cl, cng, rl, rng, 2) for a in (9, 8, 10) for cl in (4, 3) for cng in (16, 8) for rl in (4,) for rng in (1, 2, 4)],
+        [base_plan],
+    ]
+    done = []
+    bt.step()
+    for cands in schedule:
+        first_ok(cands)
+        bt.step()
+        done += bt.poll()
+    while len(done) < len(bases):
+        bt.step()
+        done += bt.poll()
+    bt.close()
+    for r in done:
+        alone = gfe.run_prp(r["b"], n, iters=iters, b_gl=-1)
+        assert (r["res64"], r["old64"]) == (alone["res64"], alone["old64"]), r["b"]

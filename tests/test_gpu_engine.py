@@ -147,3 +147,57 @@ def test_multiplicand_plan_variants(gfe, oracle, flags):
     t.square_dup(True); o.square_dup(True)
     t.mul(1); o.mul(1)
     assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b))
+
+
+@pytest.mark.parametrize("b,n", [(20204692, 16), (1999999266, 15)])
+def test_pending_carries_reach_every_use(gfe, oracle, b, n):
+    """With compile-time passes a squaring leaves its workgroup carries for the next squaring's
+    column pass; every other use of the registers applies them first: copy, read-out, context
+    save, initMultiplicand of register 0, a plan change, and the timed squarings of
+    time_squarings (per-kernel leg included).  Each state is checked against the oracle."""
+    t = gfe.Transform(b, n)
+    plan = t.get_plan()
+    assert plan[5] & 8
+    o = oracle.Engine(b, n)
+    want = lambda reg=0: canon(oracle, o.get_digits(reg), n, b)
+    got = lambda reg=0: canon(oracle, t.get_digits(reg), n, b)
+    t.set(3); o.set(3)
+    for k in range(4):
+        t.square_dup(k & 1); o.square_dup(k & 1)
+    t.copy(1, 0); o.copy(1, 0)                       # copy while pending
+    t.square_dup(False); o.square_dup(False)
+    data = t.save_context(2)                          # context while pending
+    u = gfe.Transform(b, n)
+    assert u.read_context(data, 2)
+    assert np.array_equal(canon(oracle, u.get_digits(0), n, b), want(0))
+    u.close()
+    t.square_dup(True); o.square_dup(True)
+    t.init_multiplicand(0); o.init_multiplicand(0)    # forward transform of r0 while pending
+    t.square_dup(False); o.square_dup(False)
+    t.mul(); o.mul()
+    assert np.array_equal(got(0), want(0)) and np.array_equal(got(1), want(1))
+    t.square_dup(False); o.square_dup(False)
+    alt = next(p for p in _plans(n) if p[5] == 0 and p != plan and _admissible(gfe, b, n, p))
+    t.set_plan(alt)                                   # plan change while pending
+    t.square_dup(True); o.square_dup(True)
+    t.set_plan(plan)
+    t.square_dup(False); o.square_dup(False)
+    pattern = 0b1011
+    t.time_squarings(6, pattern)                      # timed squarings, then the per-kernel leg
+    t.time_squarings(3, pattern, per_kernel=True)    # 3 timed squarings, then 3 timed kernel by kernel
+    for i in list(range(6)) + list(range(3)) + list(range(3)):
+        o.square_dup((pattern >> (i % 64)) & 1)
+    assert np.array_equal(got(0), want(0))
+    assert np.abs(t.get_digits(0)).max() <= b // 2 + 4
+    t.close()
+
+
+def _admissible(gfe, b, n, plan):
+    t = gfe.Transform(b, n)
+    try:
+        t.set_plan(plan)
+        return True
+    except gfe.GfeError:
+        return False
+    finally:
+        t.close()
