@@ -201,3 +201,23 @@ def _admissible(gfe, b, n, plan):
         return False
     finally:
         t.close()
+
+
+@pytest.mark.parametrize("b,n", [(2000000, 21), (150000, 23), (30000000, 23)])
+def test_largest_n_from_full_size_digits(gfe, oracle, b, n):
+    """The largest transforms the reference's command line admits (n = 21, 23; NTT2 with 32-bit and
+    31-bit primes, NTT3) from a seeded full-size balanced residue (every digit nonzero, carries
+    across every workgroup and the negacyclic wrap), two squarings, one doubled: equal to the
+    oracle."""
+    rs = np.random.RandomState(n * 1000 + b % 997)
+    d = oracle.balance(rs.randint(0, b, 1 << n).astype(np.int32), n, b)
+    t = gfe.Transform(b, n)
+    o = oracle.Engine(b, n)
+    assert (t.get_plan()[5] & 8) and o.rns == gfe.select(b, n)[0]
+    t.set_digits(d, 0); o.set_digits(d, 0)
+    for dup in (True, False):
+        t.square_dup(dup); o.square_dup(dup)
+    got = t.get_digits(0)
+    assert np.abs(got.astype(np.int64)).max() <= b // 2 + 4
+    assert np.array_equal(canon(oracle, got, n, b), canon(oracle, o.get_digits(0), n, b))
+    t.close()
