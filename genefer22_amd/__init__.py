@@ -84,6 +84,9 @@ ABI = [
     ("gfe_run_quick", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(Result)]),
     ("gfe_run_prp", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_int32, I32P, ctypes.POINTER(Result)]),
     ("gfe_run_prp_trace", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, U64P, U64P, ctypes.POINTER(Result)]),
+    ("gfe_default_plan", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, I32P]),
+    ("gfe_plan_admissible", ctypes.c_int, [ctypes.c_uint32, I32P]),
+    ("gfe_candidate_plans", ctypes.c_int, [ctypes.c_uint32, I32P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("gfe_run_prp_resumable", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
                                              ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Result)]),
     ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),
@@ -290,6 +293,30 @@ def run_prp(b: int, n: int, iters: int = 0, b_gl: int = 0, device: int = 0, digi
     if digits:
         out["digits"] = d
     return out
+
+
+def default_plan(n: int, capacity: int = 0) -> tuple:
+    """The plan an engine (capacity 0) or a batch of `capacity` slots of n starts with (no device)."""
+    p = np.zeros(6, dtype=np.int32)
+    _ck(lib().gfe_default_plan(n, capacity, _i32(p)))
+    return tuple(int(x) for x in p)
+
+
+def plan_admissible(n: int, plan) -> bool:
+    """Whether the kernels accept `plan` at n (no device)."""
+    p = np.ascontiguousarray(plan, dtype=np.int32)
+    if p.shape != (6,):
+        raise ValueError("a plan has 6 fields")
+    return lib().gfe_plan_admissible(n, _i32(p)) == 0
+
+
+def candidate_plans(n: int) -> list[tuple]:
+    """The plans the autotuner times at n (no device)."""
+    cnt = ctypes.c_uint32(0)
+    _ck(lib().gfe_candidate_plans(n, None, 0, ctypes.byref(cnt)))
+    buf = np.zeros(6 * cnt.value, dtype=np.int32)
+    _ck(lib().gfe_candidate_plans(n, _i32(buf), cnt.value, ctypes.byref(cnt)))
+    return [tuple(int(x) for x in buf[6 * i:6 * i + 6]) for i in range(cnt.value)]
 
 
 def run_prp_resumable(b: int, n: int, ctx_path: str, save_every: int = 0, stop_after: int = 0,

@@ -1111,6 +1111,28 @@ int gfe_get_plan(const gfe_engine *e, int32_t plan[6]) {
     return GFE_OK;
 }
 
+int gfe_default_plan(uint32_t n, uint32_t capacity, int32_t plan[6]) {
+    if (!plan || n < 12 || n > 23) return set_error(GFE_EINVAL, "default_plan");
+    gfe::plan_to_array(capacity ? gfe::default_batch_plan(n, capacity) : gfe::default_plan(n), plan);
+    return GFE_OK;
+}
+
+int gfe_plan_admissible(uint32_t n, const int32_t plan[6]) {
+    gfe::Plan P;
+    if (!plan || n < 12 || n > 23 || !gfe::plan_from_array(plan, n, &P))
+        return set_error(GFE_EINVAL, "plan not admissible for this n");
+    return GFE_OK;
+}
+
+int gfe_candidate_plans(uint32_t n, int32_t *plans, uint32_t max, uint32_t *count) {
+    if (!count || n < 12 || n > 23) return set_error(GFE_EINVAL, "candidate_plans");
+    const std::vector<gfe::Plan> c = gfe::candidate_plans(n);
+    *count = (uint32_t)c.size();
+    if (plans)
+        for (uint32_t i = 0; i < std::min<uint32_t>(max, *count); ++i) gfe::plan_to_array(c[i], plans + 6 * (size_t)i);
+    return GFE_OK;
+}
+
 int gfe_set_plan(gfe_engine *e, const int32_t plan[6]) {
     GFE_GUARD(e);
     if (!plan) return set_error(GFE_EINVAL, "set_plan");

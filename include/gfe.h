@@ -101,6 +101,15 @@ int gfe_tune(gfe_engine *e, uint32_t reps);
 int gfe_get_plan(const gfe_engine *e, int32_t plan[6]);
 int gfe_set_plan(gfe_engine *e, const int32_t plan[6]);
 
+/* Plan queries that need no device (no reference counterpart, for tooling and tests):
+ * gfe_default_plan writes the plan an engine of n starts with (capacity 0) or a batch of `capacity`
+ * slots starts with; gfe_plan_admissible returns GFE_OK for a plan the kernels accept at n, else
+ * GFE_EINVAL; gfe_candidate_plans writes how many plans the autotuner times at n to *count and,
+ * when plans is not null, the first min(max, count) of them (6 int32 each). */
+int gfe_default_plan(uint32_t n, uint32_t capacity, int32_t plan[6]);
+int gfe_plan_admissible(uint32_t n, const int32_t plan[6]);
+int gfe_candidate_plans(uint32_t n, int32_t *plans, uint32_t max, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""Host logic of libgfe.so that needs no device: pass-plan defaults, admissibility and the
+autotuner's candidate list, and the prime-set selection of create_ocl (src/transform_ocl.cpp:35-50)
+against the oracle's restatement, including the bounds between prime sets."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def G():
+    import genefer22_amd
+    if not os.path.exists(genefer22_amd.LIB_PATH):
+        from genefer22_amd import build
+        build.build()
+    genefer22_amd.lib()
+    return genefer22_amd
+
+
+def threads(n, plan):
+    a, cl, cng, rl, rng, _ = plan
+    return cng * (1 << a) // (1 << cl), rng * (1 << (n - a)) // (1 << rl)
+
+
+@pytest.mark.parametrize("n", range(12, 24))
+def test_default_plans_are_admissible_candidates(G, n):
+    plans = G.candidate_plans(n)
+    assert len(plans) == len(set(plans)) > 100
+    for cap in (0, 1, 31, 32, 512):
+        p = G.default_plan(n, cap)
+        assert G.plan_admissible(n, p), (n, cap, p)
+        assert p in plans, (n, cap, p)
+        assert p[5] & 8, "the defaults use the compile-time passes"
+        # the single-workgroup squaring is a batch default for n <= 13 and 32+ slots only
+        assert bool(p[5] & 16) == (cap >= 32 and n <= 13), (n, cap, p)
+
+
+@pytest.mark.parametrize("n", [12, 15, 19, 22, 23])
+def test_candidate_plans_are_well_formed(G, n):
+    for p in G.candidate_plans(n):
+        a, cl, cng, rl, rng, fl = p
+        assert G.plan_admissible(n, p)
+        assert 0 <= fl < 32 and (fl & 6) != 6, p            # fused and persistent exclude each other
+        if fl & 8:
+            assert fl in (8, 24), p                          # compile-time passes: no other variant
+            assert 5 <= a <= 11 and 6 <= n - a <= 14, p      # instantiated shapes
+        if fl & 16:
+            assert n <= 13, p
+        tc, tr = threads(n, p)
+        assert 64 <= tc <= 1024 and 64 <= tr <= 1024, p
+        if fl & 8:
+            assert (cl < 5 or tc <= 512) and (rl < 5 or tr <= 512), p
+        assert cng <= 1 << (n - a) and rng <= 1 << a, p
+
+
+def test_malformed_plans_are_refused(G):
+    n = 22
+    good = G.default_plan(n)
+    assert G.plan_admissible(n, good)
+    a, cl, cng, rl, rng, fl = good
+    bad = [
+        (a, cl, cng, rl, rng, 6),       # fused and persistent together
+        (a, cl, cng, rl, rng, 9),       # compile-time passes with factored twiddles
+        (a, cl, cng, rl, rng, 32),      # unknown flag bit
+        (a, cl, cng, rl, rng, -1),
+        (a, cl, cng, rl, rng, 24),      # single-workgroup squaring above n = 13
+        (7, cl, cng, rl, rng, 8),       # 15 row levels: no instantiated row pass
+        (a, cl, 4, rl, rng, 8),         # fewer than 8 columns per workgroup
+        (a, cl, cng, rl, 1 << 9, 8),    # more rows per workgroup than rows
+        (n, 4, 1, 4, 1, 0),             # no row levels left
+        (a, 6, cng, rl, rng, 8),        # 64 points per thread
+    ]
+    for p in bad:
+        assert not G.plan_admissible(n, p), p
+    with pytest.raises(ValueError):
+        G.plan_admissible(n, (1, 2, 3))
+    for n_bad in (11, 24):
+        with pytest.raises(G.GfeError):
+            G.default_plan(n_bad)
+
+
+def test_prime_set_selection_matches_the_oracle(G, oracle):
+    rs = np.random.RandomState(11)
+    for _ in range(400):
+        n = int(rs.randint(12, 24))
+        b = 2 * int(rs.randint(512, 1_000_000_001))
+        assert G.select(b, n) == oracle.select(b, n), (b, n)
+
+
+@pytest.mark.parametrize("n", [12, 15, 16, 22])
+def test_prime_set_bounds(G, oracle, n):
+    """Around each bound of the rule (b^2 < P1 P2 / 2 / N for NTT2 with the 31-bit, then the 32-bit
+    primes; b <= 10^9 for NTT3 with the 31-bit primes) both restatements agree and switch sets."""
+    N = 1 << n
+    S = [127 * 2**24 + 1, 63 * 2**25 + 1]
+    U = [125 * 2**25 + 1, 243 * 2**24 + 1]
+    for P1, P2 in (S, U):
+        lim = (P1 * P2 // 2) // N          # b^2 must stay below this
+        r = math.isqrt(lim - 1)            # largest b with b^2 < lim
+        for b in (r - 1, r, r + 1, r + 2):
+            if b < 1024:
+                continue
+            assert G.select(b, n) == oracle.select(b, n), (b, n)
+        below, above = G.select(r, n), G.select(r + 1, n)
+        assert below[0] == 2 and (above != below), (n, r, below, above)
+    for b in (999_999_998, 1_000_000_000, 1_000_000_002):
+        sel = G.select(b, n)
+        assert sel == oracle.select(b, n)
+        assert sel == (3, b > 1_000_000_000), (b, n, sel)
