@@ -25,6 +25,14 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// The candidate range of gfe_create (include/gfe.h): argument errors are GFE_EINVAL, reported
+// before any device work.
+int check_candidate(uint32_t b, uint32_t n) {
+    if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
+    if (b < 1024 || b > 2000000000u) return gfe::set_error(GFE_EINVAL, "b must be in [1024, 2000000000]");
+    return GFE_OK;
+}
+
 void fill_residue(gfe::gint &g, gfe_result *out) {
     uint64_t r = 0, o = 0;
     const bool one = g.isOne(r, o);
@@ -40,6 +48,7 @@ extern "C" {
 
 int gfe_run_quick(uint32_t b, uint32_t n, int device, gfe_result *out) {
     if (!out) return gfe::set_error(GFE_EINVAL, "null output");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
     memset(out, 0, sizeof(*out));
     try {
         const double t0 = now_s();
@@ -62,6 +71,7 @@ int gfe_run_quick(uint32_t b, uint32_t n, int device, gfe_result *out) {
 int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl, int32_t *digits_out,
                 gfe_result *out) {
     if (!out) return gfe::set_error(GFE_EINVAL, "null output");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
     memset(out, 0, sizeof(*out));
     try {
         const double t0 = now_s();
@@ -86,6 +96,7 @@ int gfe_run_prp(uint32_t b, uint32_t n, int device, uint64_t iters, int32_t b_gl
 int gfe_run_prp_resumable(uint32_t b, uint32_t n, int device, const char *ctx_path, uint64_t save_every,
                           uint64_t stop_after, int32_t *done, gfe_result *out) {
     if (!out || !ctx_path || !*ctx_path) return gfe::set_error(GFE_EINVAL, "resumable prp arguments");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
     memset(out, 0, sizeof(*out));
     if (done) *done = 0;
     try {
@@ -116,6 +127,7 @@ int gfe_run_prp_resumable(uint32_t b, uint32_t n, int device, const char *ctx_pa
 int gfe_run_prp_trace(uint32_t b, uint32_t n, int device, uint64_t iters, uint64_t every, uint64_t *res64,
                       uint64_t *old64, gfe_result *out) {
     if (!out || !res64 || !old64 || every == 0) return gfe::set_error(GFE_EINVAL, "trace arguments");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
     memset(out, 0, sizeof(*out));
     try {
         const double t0 = now_s();
@@ -147,6 +159,7 @@ int gfe_run_prp_trace(uint32_t b, uint32_t n, int device, uint64_t iters, uint64
 
 int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_hash64, gfe_result *out) {
     if (!out || depth < 1 || depth > 12) return gfe::set_error(GFE_EINVAL, "proof arguments");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
     memset(out, 0, sizeof(*out));
     try {
         const double t0 = now_s();
@@ -189,6 +202,7 @@ static bool write_gint_record(gfe::file &f, const std::vector<int32_t> &canon, u
 int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int flags,
                     uint64_t *mu_hash64, gfe_result *out) {
     if (!out || !main_path || depth < 1 || depth > 12 || (flags & ~3)) return gfe::set_error(GFE_EINVAL, "proof arguments");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
     memset(out, 0, sizeof(*out));
     const bool write_ckpts = (flags & GFE_PROOF_CKPT_FILES) != 0;
     try {

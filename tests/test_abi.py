@@ -76,3 +76,31 @@ def test_host_residue_function(libgfe):
         want = oracle.residue(d, n, b)
         assert (mine["res64"], mine["old64"], mine["hash64"]) == (want.res64, want.old64, want.hash64)
         assert np.array_equal(mine["digits"], oracle.unbalance(d, n, b))
+
+
+def test_argument_errors_need_no_device(libgfe):
+    """Out-of-range candidates and malformed arguments are GFE_EINVAL (-1) with a message, before any
+    device work, on every entry point that takes them."""
+    import genefer22_amd as G
+    L = G.lib()
+    h = ctypes.c_void_p()
+    r = G.Result()
+    cases = [
+        lambda: L.gfe_create(1026, 11, 3, 0, ctypes.byref(h)),
+        lambda: L.gfe_create(1026, 24, 3, 0, ctypes.byref(h)),
+        lambda: L.gfe_create(1022, 12, 3, 0, ctypes.byref(h)),
+        lambda: L.gfe_create(2000000002, 12, 3, 0, ctypes.byref(h)),
+        lambda: L.gfe_batch_create(11, 4, 0, ctypes.byref(h)),
+        lambda: L.gfe_batch_create(15, 0, 0, ctypes.byref(h)),
+        lambda: L.gfe_run_quick(1026, 24, 0, ctypes.byref(r)),
+        lambda: L.gfe_run_prp(1000, 12, 0, 10, 0, None, ctypes.byref(r)),
+        lambda: L.gfe_run_prp_trace(1026, 12, 0, 10, 0, None, None, ctypes.byref(r)),
+        lambda: L.gfe_run_prp_resumable(1026, 12, 0, None, 0, 0, None, ctypes.byref(r)),
+        lambda: L.gfe_run_prp_resumable(1026, 25, 0, b"/tmp/x.ctx", 0, 0, None, ctypes.byref(r)),
+        lambda: L.gfe_run_proof(1026, 12, 0, 0, None, ctypes.byref(r)),
+        lambda: L.gfe_write_proof(1026, 12, 0, 3, b"/tmp/x", 4, None, ctypes.byref(r)),
+        lambda: L.gfe_write_proof(900, 12, 0, 3, b"/tmp/x", 0, None, ctypes.byref(r)),
+    ]
+    for i, call in enumerate(cases):
+        assert call() == -1, i
+        assert L.gfe_last_error(), i
