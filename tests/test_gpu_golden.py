@@ -63,10 +63,14 @@ def test_c3_full_tests_batched(gfe):
 
 
 def test_c4_checkpoints_to_100000(gfe):
-    """C4: n = 22, b in [1e7, 3.5e7]: RES64 at every 2000th of the first 100,000 squarings."""
+    """C4: n = 22, b in [1e7, 3.5e7]: RES64 at every 2000th of the first 100,000 squarings; and the
+    same residue at 100,000 with the Gerbicz-Li product updated every 2000 iterations (the loop's
+    residue does not depend on the cadence, src/genefer.h:529-535)."""
     for r in golden("ref_c4.json"):
         tr = gfe.run_prp_trace(r["b"], 22, r["iters"], r["every"])
         assert [h(a) for a, _ in tr] == r["res64"], r["b"]
+        g = gfe.run_prp(r["b"], 22, iters=r["iters"], b_gl=2000)
+        assert h(g["res64"]) == r["res64"][-1], r["b"]
 
 
 def test_c5_serving_sample(gfe):
