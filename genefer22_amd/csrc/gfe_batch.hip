@@ -1281,13 +1281,24 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     double best_ms = 1e30;
     const bool graphs = B.use_graphs;
     B.use_graphs = false;  // plans are compared on eager launches (no capture per candidate plan)
-    for (const gfe::Plan &P : plans) {
-        B.plan = P;
+    // every plan on `reps` rounds, then the fastest eight again on 8 x reps rounds
+    std::vector<std::pair<double, size_t>> screen;
+    for (size_t k = 0; k < plans.size(); ++k) {
+        B.plan = plans[k];
         if (B.zero_carries() != GFE_OK) break;
         double ms = 0;
         if (gfe_batch_time_rounds(b, 1, &ms) != GFE_OK) continue;
         if (gfe_batch_time_rounds(b, reps, &ms) != GFE_OK) continue;
-        if (ms < best_ms) { best_ms = ms; best = P; }
+        screen.emplace_back(ms, k);
+    }
+    std::sort(screen.begin(), screen.end());
+    for (size_t j = 0; j < std::min<size_t>(8, screen.size()); ++j) {
+        B.plan = plans[screen[j].second];
+        if (B.zero_carries() != GFE_OK) break;
+        double ms = 0;
+        if (gfe_batch_time_rounds(b, 1, &ms) != GFE_OK) continue;
+        if (gfe_batch_time_rounds(b, 8 * reps, &ms) != GFE_OK) continue;
+        if (ms < best_ms) { best_ms = ms; best = plans[screen[j].second]; }
     }
     B.plan = best;
     B.use_graphs = graphs;

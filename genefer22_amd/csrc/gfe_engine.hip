@@ -1077,25 +1077,37 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
     }
     const std::vector<gfe::Plan> plans = gfe::candidate_plans(g.n);
     const gfe::Plan orig = g.plan;
-    gfe::Plan best = orig;
-    double best_ms = 1e30;
-    for (const gfe::Plan &P : plans) {
+    // ms for `count` squarings under plan P (after one untimed), < 0 on a launch error
+    auto time_plan = [&](const gfe::Plan &P, uint32_t count, double *ms) -> int {
         g.plan = P;
         if (hipMemcpyAsync(g.regs, trial.data(), g.N * 4, hipMemcpyHostToDevice, g.stream) != hipSuccess)
             return set_error(GFE_EHIP, "tune upload");
         g.carry_pending = false;  // register 0 was just rewritten
         g.square(0);
         (void)hipEventRecord(g.events[0], g.stream);
-        for (uint32_t i = 0; i < reps; ++i) g.square(0);
+        for (uint32_t i = 0; i < count; ++i) g.square(0);
         (void)hipEventRecord(g.events[1], g.stream);
-        if (hipEventSynchronize(g.events[1]) != hipSuccess) {
-            g.plan = orig;
-            return set_error(GFE_EHIP, "tune sync");
-        }
-        if (hipGetLastError() != hipSuccess) continue;
+        if (hipEventSynchronize(g.events[1]) != hipSuccess) return set_error(GFE_EHIP, "tune sync");
         float t = 0;
         (void)hipEventElapsedTime(&t, g.events[0], g.events[1]);
-        if (t < best_ms) { best_ms = t; best = P; }
+        *ms = (hipGetLastError() != hipSuccess) ? -1.0 : (double)t;
+        return GFE_OK;
+    };
+    // every plan on `reps` squarings, then the fastest eight again on 8 x reps: a few squarings
+    // rank the plans, the short list is decided on timings long enough to be stable
+    std::vector<std::pair<double, size_t>> screen;
+    for (size_t k = 0; k < plans.size(); ++k) {
+        double ms = 0;
+        if (time_plan(plans[k], reps, &ms) != GFE_OK) { g.plan = orig; return GFE_EHIP; }
+        if (ms >= 0) screen.emplace_back(ms, k);
+    }
+    std::sort(screen.begin(), screen.end());
+    gfe::Plan best = orig;
+    double best_ms = 1e30;
+    for (size_t j = 0; j < std::min<size_t>(8, screen.size()); ++j) {
+        double ms = 0;
+        if (time_plan(plans[screen[j].second], 8 * reps, &ms) != GFE_OK) { g.plan = orig; return GFE_EHIP; }
+        if (ms >= 0 && ms < best_ms) { best_ms = ms; best = plans[screen[j].second]; }
     }
     g.plan = best;
     g.carry_pending = false;
