@@ -173,7 +173,7 @@ __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ dat
         }
         fwd_chunk<M, LOGE, NE>(md, x, w);
         if constexpr (ci < GM::NCH - 1) {
-            __syncthreads();
+            if constexpr (ci > 0) __syncthreads();  // chunk 0 follows no LDS read
             uint32_t *p = L + base + (base >> PAD);
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, PAD>(rr)] = x[rr];
@@ -264,13 +264,13 @@ constexpr uint32_t NORM_RANGE = 2048;
 // x^N = -1) exactly as carry_wg_body does, and leave them in LDS for lanes q < 8.  The patch area
 // (2^LOGG x 8 words) is released by the barrier that precedes the pass's first LDS exchange.
 template <int LOGE, int LOGG>
-__device__ __forceinline__ void col_absorb_carries(int32_t (&dg)[1 << LOGE], const int32_t *__restrict__ src,
+__device__ __forceinline__ bool col_absorb_carries(int32_t (&dg)[1 << LOGE], const int32_t *__restrict__ src,
                                                    const int64_t *__restrict__ wc, uint32_t b, uint32_t C,
                                                    uint32_t c0, uint32_t q, uint32_t base, int YL, uint32_t nrange,
                                                    uint32_t *lds) {
     constexpr int E = 1 << LOGE;
     const bool starts = (C >= NORM_RANGE) ? (c0 % NORM_RANGE) == 0 : c0 == 0;
-    if (!starts) return;  // uniform over the workgroup
+    if (!starts) return false;  // uniform over the workgroup
     const uint32_t every = (C >= NORM_RANGE) ? 1u : NORM_RANGE / C;
     int32_t *P = reinterpret_cast<int32_t *>(lds);
     const double binv = 1.0 / (double)b;
@@ -293,6 +293,7 @@ __device__ __forceinline__ void col_absorb_carries(int32_t (&dg)[1 << LOGE], con
             if (r % every == 0) dg[rr] = P[r * 8 + q];
         }
     }
+    return true;
 }
 
 // Forward levels 0..a-1 of one prime: signed digits `src` -> residues `dst` (both N values).  With
@@ -308,6 +309,7 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
     const uint32_t c = blk * (uint32_t)NG + q;
     uint32_t *L = lds + q * col_pass_stride(LOGG, LOGE, NG);
     uint32_t x[E], w[E];
+    bool patched = false;  // the carry patch was read from LDS (uniform over the workgroup)
     static_for<0, GM::NCH>([&](auto cc) {
         constexpr int ci = decltype(cc)::value;
         constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
@@ -319,8 +321,8 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) dg[rr] = *at_bytes(src + (size_t)((uint32_t)rr << YL) * C, off);
             if (wc)
-                col_absorb_carries<LOGE, LOGG>(dg, src, wc, b, C, blk * (uint32_t)NG, q, base, YL,
-                                               (uint32_t)(((size_t)C << LOGG) / NORM_RANGE), lds);
+                patched = col_absorb_carries<LOGE, LOGG>(dg, src, wc, b, C, blk * (uint32_t)NG, q, base, YL,
+                                                         (uint32_t)(((size_t)C << LOGG) / NORM_RANGE), lds);
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) x[rr] = to_res(dg[rr], md.p);
         } else {
@@ -335,7 +337,7 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) *at_bytes(dst + (size_t)((uint32_t)rr << YL) * C, off) = x[rr];
         } else {
-            __syncthreads();
+            if (ci > 0 || patched) __syncthreads();  // chunk 0 follows no LDS read but the carry patch's
             uint32_t *p = L + base + (base >> LOGE);
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
@@ -374,7 +376,7 @@ __device__ __forceinline__ void col_inv_pass(const M &md, uint32_t *__restrict__
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) *at_bytes(data + (size_t)((uint32_t)rr << YL) * C, off) = x[rr];
         } else {
-            __syncthreads();
+            if constexpr (ci < GM::NCH - 1) __syncthreads();  // the first chunk follows no LDS read
             uint32_t *p = L + base + (base >> LOGE);
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
