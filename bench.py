@@ -10,6 +10,9 @@ One JSON line on stdout (progress on stderr):
                  residues, 8 rns N bytes per launch) over its average launch duration, against 8 TB/s;
   * passes     : every pass's algorithmic HBM rate (and the whole squaring's) against 8 TB/s;
   * valu       : the same kernel's butterfly rate and estimated VALU-slot use (integer-bound path);
+  * job_n22    : the n = 22 job end to end (config.job_n22): 100,000 PRP squarings with Gerbicz-Li
+                 updates every 2000, engine set-up and read-out included, host clock; RES64 against
+                 the reference's value at 100,000;
   * batched_n15: candidates/hour at n = 15, 512 seeded candidates in one batch vs one candidate per
                  launch, from timed iteration blocks (full-test length = esize squarings);
   * serving    : p50/p99 time-to-result of seeded Poisson requests of mixed n (12..20) through the
@@ -229,6 +232,25 @@ def parity_n22(G, O, b: int, iters: int, threads: int):
             "match": g["res64"] == r.res64, "ref_impl": ref.impl, "ref_max_error": err}
 
 
+def bench_c4_job(G, b: int, iters: int, b_gl: int):
+    """The n = 22 job as the PRP loop runs it (src/genefer.h:479-552): engine set-up, the first `iters`
+    squarings with the Gerbicz-Li product updated every b_gl squarings, and the residue read-out,
+    timed on the host clock; RES64 compared with the reference's value at 100,000 squarings
+    (tests/golden/ref_c4.json, written from the reference's CPU transform)."""
+    t0 = time.perf_counter()
+    g = G.run_prp(b, 22, iters=iters, b_gl=b_gl)
+    wall = time.perf_counter() - t0
+    out = {"b": b, "iters": g["iters"], "b_gl": b_gl, "wall_s": round(wall, 3),
+           "ns_per_squaring_job": round(wall / g["iters"] * 1e9, 1), "res64": f"{g['res64']:016X}"}
+    path = os.path.join(ROOT, "tests", "golden", "ref_c4.json")
+    if iters == 100_000 and os.path.exists(path):
+        ref = json.load(open(path))["results"][0]
+        if ref["b"] == b:
+            out["ref_res64"] = ref["res64"][-1]
+            out["match"] = out["res64"] == ref["res64"][-1]
+    return out
+
+
 def cpu_baseline(O, b: int, threads: int, budget_s: float):
     """The reference's CPU transform at n = 22 (same b), squarings timed for about budget_s."""
     ref = O.RefEngine(b, 22, threads=threads)
@@ -262,6 +284,8 @@ def main():
     ap.add_argument("--light-requests", type=int, default=100, help="requests of the light-load serving replay")
     ap.add_argument("--light-rate", type=float, default=8.0, help="arrivals/s of the light-load serving replay")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--job-iters", type=int, default=100_000,
+                    help="squarings of the timed n=22 PRP job with Gerbicz-Li updates every 2000 (0: skip)")
     ap.add_argument("--plan", default=None, help="fixed n=22 pass plan a,cl,cng,rl,rng,flags (default: the library's)")
     ap.add_argument("--no-tune", action="store_true", help="time the library's default batch plan instead of the autotuned one")
     ap.add_argument("--tune-n22", action="store_true",
@@ -277,6 +301,10 @@ def main():
     G.lib()
     res = bench_n22(G, args.steps, args.warmup, args.plan, args.tune_n22)
     log(f"[bench] n=22: {res['ns_per_squaring']:.0f} ns/squaring; kernels {res['kernel_us']}")
+
+    job = bench_c4_job(G, res["b"], args.job_iters, 2000) if args.job_iters > 0 else None
+    if job:
+        log(f"[bench] n=22 job: {job}")
 
     bplan = [int(v) for v in args.batch_plan.split(",")] if args.batch_plan else None
     batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64, not args.no_tune, bplan)
@@ -319,7 +347,7 @@ def main():
         "config": {
             "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] (NTT3, 31-bit primes)",
             "n": 22, "b": res["b"], "plan": res["plan"], "plan_tuple": res["plan_tuple"], "kernel_us": res["kernel_us"],
-            "batched_n15": batched, "serving": serving, "serving_light": serving_light, "parity": parity,
+            "job_n22": job, "batched_n15": batched, "serving": serving, "serving_light": serving_light, "parity": parity,
         },
         "roofline": res["roofline"],
         "passes": res["passes"],

@@ -11,14 +11,17 @@ JSON it writes.  Workloads come from genefer22_amd.workloads (seeded).
 Outputs (all values are the reference's, hex RES64 / old64):
   ref_c1.json  : C1 (n=12, b in [1024, 2^20)), 8 full quick tests (RES64, old64, GL passed)
   ref_c2.json  : C2 (n=16, b in [1e8, 1.5e9]), 2 candidates, RES64 every 1000 squarings to 200,000
+                 and the SHA-256 of the canonical digits at 200,000 (int32 little-endian, digit 0 first)
   ref_c3t.json : C3 (n=15, b in [1e8, 2e9)), all 512 candidates, RES64 after 2048 squarings
   ref_c3f.json : C3, the first 12 candidates, full quick tests
   ref_c4.json  : C4 (n=22, b in [1e7, 3.5e7]), 1 candidate, RES64 every 2000 squarings to 100,000
+                 and the SHA-256 of the canonical digits at 100,000
   ref_c5.json  : C5 serving mix, the first 48 requests, RES64 after their 4096 squarings
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -56,7 +59,9 @@ def job_trace(b, n, iters, every, threads, check_error=False):
     t0 = time.time()
     r = _ref(b, n, threads, check_error)
     tr = r.prp_trace(iters, every)
+    canon = r.get_digits(canonical=True).astype("<i4")
     return {"b": b, "n": n, "iters": iters, "every": every,
+            "sha256": hashlib.sha256(canon.tobytes()).hexdigest(),
             "res64": [f"{a:016X}" for a, _ in tr], "old64": [f"{o:016X}" for _, o in tr],
             "impl": r.impl, "max_error": r.error(), "seconds": round(time.time() - t0, 1)}
 

@@ -1,5 +1,7 @@
 """GPU parity against fixtures written by the reference's own CPU transforms (tests/golden/
-make_golden.py), on the measurement configurations C1-C5.  Bit-exact RES64 and old64."""
+make_golden.py), on the measurement configurations C1-C5.  Bit-exact RES64 and old64, and on the
+long traces the SHA-256 of the whole canonical residue."""
+import hashlib
 import json
 import os
 
@@ -21,6 +23,13 @@ def h(x):
     return f"{x:016X}"
 
 
+def canonical_sha256(gfe, digits, n, b):
+    """SHA-256 of the canonical digits in [0, b) (int32 little-endian, digit 0 first), as the
+    fixture writer hashes the reference's canonical register."""
+    canon = gfe.residue(digits, n, b, canonical=True)["digits"]
+    return hashlib.sha256(canon.astype("<i4").tobytes()).hexdigest()
+
+
 def test_c1_full_quick_tests(gfe):
     """C1: n = 12, b in [1024, 2^20): full quick tests, one engine per candidate."""
     for r in golden("ref_c1.json"):
@@ -37,11 +46,15 @@ def test_c1_full_quick_tests_batched(gfe):
 
 
 def test_c2_checkpoints_to_200000(gfe):
-    """C2: n = 16, b in [1e8, 1.5e9]: RES64 at every 1000th of the first 200,000 squarings."""
+    """C2: n = 16, b in [1e8, 1.5e9]: RES64 at every 1000th of the first 200,000 squarings, and the
+    whole canonical residue at 200,000 (SHA-256)."""
     for r in golden("ref_c2.json"):
         tr = gfe.run_prp_trace(r["b"], 16, r["iters"], r["every"])
         assert [h(a) for a, _ in tr] == r["res64"], r["b"]
         assert [h(o) for _, o in tr] == r["old64"]
+        g = gfe.run_prp(r["b"], 16, iters=r["iters"], b_gl=-1, digits=True)
+        assert h(g["res64"]) == r["res64"][-1], r["b"]
+        assert canonical_sha256(gfe, g["digits"], 16, r["b"]) == r["sha256"], r["b"]
 
 
 def test_c3_all_512_truncated_in_one_batch(gfe):
@@ -65,12 +78,14 @@ def test_c3_full_tests_batched(gfe):
 def test_c4_checkpoints_to_100000(gfe):
     """C4: n = 22, b in [1e7, 3.5e7]: RES64 at every 2000th of the first 100,000 squarings; and the
     same residue at 100,000 with the Gerbicz-Li product updated every 2000 iterations (the loop's
-    residue does not depend on the cadence, src/genefer.h:529-535)."""
+    residue does not depend on the cadence, src/genefer.h:529-535), whole canonical residue included
+    (SHA-256)."""
     for r in golden("ref_c4.json"):
         tr = gfe.run_prp_trace(r["b"], 22, r["iters"], r["every"])
         assert [h(a) for a, _ in tr] == r["res64"], r["b"]
-        g = gfe.run_prp(r["b"], 22, iters=r["iters"], b_gl=2000)
+        g = gfe.run_prp(r["b"], 22, iters=r["iters"], b_gl=2000, digits=True)
         assert h(g["res64"]) == r["res64"][-1], r["b"]
+        assert canonical_sha256(gfe, g["digits"], 22, r["b"]) == r["sha256"], r["b"]
 
 
 def test_c5_serving_sample(gfe):
