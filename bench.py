@@ -20,7 +20,9 @@ One JSON line on stdout (progress on stderr):
                  arrival rate (serving) and at a light one (serving_light: latency near service time);
   * parity     : RES64 after P PRP iterations on the GPU vs the reference's own CPU transform;
   * cpu_baseline: the reference's CPU transform (compiled from its sources into oracle/_ref) timed on
-                 this host's cores on a bounded sample of the same workload.
+                 this host's cores on a bounded sample of the same workload;
+  * small_configs: the plain path at n = 12, 15, 16 (C1, C3, C2) beside the reference's CPU
+                 transform on one core, and C3 candidates/hour of one CPU process per core.
 """
 from __future__ import annotations
 
@@ -269,6 +271,42 @@ def cpu_baseline(O, b: int, threads: int, budget_s: float):
                       f"transform, {threads} thread(s)"}
 
 
+def bench_small_configs(G, O, cores: int, budget_s: float = 2.0):
+    """The plain one-candidate path at C1 (n = 12), C3 (n = 15) and C2 (n = 16), first seeded b of
+    each: GPU ns per squaring (library default plan, HIP events) beside the reference's CPU
+    transform on one core (about budget_s of squarings each, as the reference's bench times
+    squareDup, src/genefer.h:1285-1307); for C3 also full-test candidates/hour of `cores`
+    single-threaded CPU processes against the batched GPU leg."""
+    out = {}
+    for name, cfg in (("C1", W.C1), ("C3", W.C3), ("C2", W.C2)):
+        n = cfg["n"]
+        b = W.config_bases(cfg, 1)[0]
+        t = G.Transform(b, n)
+        t.set(1)
+        t.time_squarings(64)
+        gpu_ns = t.time_squarings(1024) / 1024 * 1e6
+        t.close()
+        ref = O.RefEngine(b, n, threads=1)
+        ref.set(1)
+        for _ in range(3):
+            ref.square_dup(False)
+        count, t0 = 0, time.perf_counter()
+        while True:
+            ref.square_dup(count & 1)
+            count += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s or count >= 200_000:
+                break
+        cpu_ns = el / count * 1e9
+        out[name] = {"n": n, "b": b, "gpu_ns_per_squaring": round(gpu_ns, 1),
+                     "cpu_ns_per_squaring_1core": round(cpu_ns, 1), "cpu_impl": ref.impl, "cpu_squarings": count}
+    bases = W.config_bases(W.C3, 512)
+    mean_es = sum(W.esize(b, 15) for b in bases) / len(bases)
+    out["C3"]["cpu_candidates_per_hour"] = round(cores * 3600.0 / (out["C3"]["cpu_ns_per_squaring_1core"] * 1e-9 * mean_es), 1)
+    out["C3"]["cpu_processes"] = cores
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,7 +358,7 @@ def main():
         log(f"[bench] serving (light load): {serving_light}")
 
     threads = min(16, os.cpu_count() or 1)
-    parity = cpu = None
+    parity = cpu = small = None
     if not args.no_cpu:
         import oracle as O
         if O.RefEngine.available():
@@ -328,6 +366,8 @@ def main():
             log(f"[bench] parity: {parity}")
             cpu = cpu_baseline(O, res["b"], threads, args.cpu_seconds)
             log(f"[bench] cpu baseline: {cpu}")
+            small = bench_small_configs(G, O, threads)
+            log(f"[bench] plain path at C1/C3/C2 beside the CPU: {small}")
         else:
             log("[bench] oracle/_ref/libgfref.so missing: no CPU baseline")
 
@@ -348,6 +388,7 @@ def main():
             "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] (NTT3, 31-bit primes)",
             "n": 22, "b": res["b"], "plan": res["plan"], "plan_tuple": res["plan_tuple"], "kernel_us": res["kernel_us"],
             "job_n22": job, "batched_n15": batched, "serving": serving, "serving_light": serving_light, "parity": parity,
+            "small_configs": small,
         },
         "roofline": res["roofline"],
         "passes": res["passes"],
