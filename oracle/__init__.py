@@ -1,7 +1,7 @@
 """The oracle: CPU restatement of genefer's exact squaring engine and of its driver.
 
-TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
-leg may import this package.  The product (genefer22_amd/) never imports, loads or calls it.
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg (the reference CPU timings and the RES64 check beside them) may import this package.  The product (genefer22_amd/) never imports, loads or calls it.
 
 Layers:
   * liboracle.so (gfe_oracle.c): the transform engine, CRT, carry and gint functions, each

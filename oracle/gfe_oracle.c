@@ -1,7 +1,7 @@
 /* gfe_oracle.c -- CPU restatement of genefer's exact-integer squaring engine (the oracle).
  *
- * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg may load it; the product (genefer22_amd/) never links, loads or calls it.
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg (CPU timings and the RES64 check beside them) may load it; the product (genefer22_amd/) never links, loads or calls it.
  *
  * What it restates (reference files, path:line):
  *   - prime sets, generators and Montgomery constants: src/transformGPU.h:37-115, 129-164;
