@@ -1,7 +1,8 @@
 /* gfe_oracle.h -- CPU restatement of genefer's exact-integer squaring engine.
  *
  * TEST INFRASTRUCTURE ONLY: this is the checker.  Only tests/, __graft_entry__.smoke() and
- * bench.py's CPU-baseline leg (CPU timings and the RES64 check beside them) may load liboracle.so; the product (genefer22_amd/) never does.
+ * bench.py's CPU-baseline leg (CPU timings and the RES64 check beside them) may load
+ * liboracle.so; the product (genefer22_amd/) never does.
  *
  * Every function states the reference file:line it restates.  "parity unpinned" marks a
  * function that no golden vector, reference build or closed form checks (none at present:
