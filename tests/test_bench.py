@@ -1,0 +1,50 @@
+"""bench.py's output contract: one JSON line with the headline metric, the roofline and CPU-baseline
+objects, and the refusal of more than one GPU (checked without a device)."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run_bench(*args, timeout=300):
+    return subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], cwd=ROOT,
+                          capture_output=True, text=True, timeout=timeout)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = run_bench("--gpus", "2", timeout=60)
+    assert r.returncode == 2
+    assert r.stdout == ""
+    assert "one GPU" in r.stderr
+
+
+@pytest.mark.gpu
+def test_bench_json_line():
+    r = run_bench("--steps", "50", "--warmup", "5", "--no-batch", "--no-serve", "--job-iters", "0",
+                  "--cpu-seconds", "1")
+    assert r.returncode == 0, r.stderr[-3000:]
+    lines = [l for l in r.stdout.splitlines() if l.strip()]
+    assert len(lines) == 1, r.stdout
+    line = json.loads(lines[0])
+    for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step", "higher_is_better",
+              "scaling", "vs_baseline", "dtype", "data", "config", "roofline", "cpu_baseline"):
+        assert k in line, k
+    assert line["n_gpus"] == 1 and line["steps"] == 50 and line["warmup"] == 5
+    assert line["unit"] == "ns/squaring" and line["higher_is_better"] is False
+    assert line["value"] > 0 and abs(line["value"] - line["ms_per_step"] * 1e6) < 1.0
+    assert line["config"]["workload"] and line["config"]["n"] == 22
+    rf = line["roofline"]
+    assert rf["bound"] in ("hbm", "mfma") and rf["unit"] == "GB/s" and rf["peak"] == 8000.0
+    assert rf["achieved"] > 0 and abs(rf["frac"] - rf["achieved"] / rf["peak"]) < 1e-3
+    par = line["config"]["parity"]
+    cpu = line["cpu_baseline"]
+    if par is not None:  # the reference CPU build is present
+        assert par["match"], par
+        assert cpu["kind"] == "reference" and cpu["cores"] >= 1 and cpu["value"] > 0 and cpu["sample"]
+        small = line["config"]["small_configs"]
+        assert set(small) == {"C1", "C2", "C3"}
+        assert small["C3"]["cpu_candidates_per_hour"] > 0
