@@ -804,6 +804,7 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
     S.tag = tag;
     S.t_submit = now_s() - t0;
     select_primes(b, n, &S.rns, &S.is32);
+    S.is32 = transform_is32(S.rns, S.is32);
     // E = b^(2^n) and its bits (src/genefer.h:1476)
     mpz_t E;
     mpz_init(E);

@@ -38,6 +38,8 @@ std::vector<Plan> candidate_plans(uint32_t n);
 unsigned persistent_grid(unsigned nblk, unsigned groups, unsigned resident);
 bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out);
 void select_primes(uint32_t b, uint32_t n, int *rns, int *is32);
+// The prime set the transform computes in for a set chosen by select_primes.
+int transform_is32(int rns, int is32);
 uint32_t prime_of(int is32, int k);
 uint32_t mulw_const(uint32_t w, uint32_t p);
 CrtConst make_crt(int rns, int is32, uint32_t n, uint32_t b = 0);
@@ -48,6 +50,7 @@ extern thread_local std::string g_last_error;
 struct Engine {
     uint32_t b = 0, n = 0, num_regs = 0;
     int device = 0, rns = 0, is32 = 0;
+    int ref_is32 = 0;  // the reference's set (select_primes): the context payload's residues
     size_t N = 0;
     Plan plan;
     PrimeArgs pr;

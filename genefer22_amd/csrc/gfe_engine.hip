@@ -120,6 +120,15 @@ void select_primes(uint32_t b, uint32_t n, int *rns, int *is32) {
     else { *rns = 3; *is32 = 1; }
 }
 
+// The reference takes the 32-bit primes for NTT3 when b > 10^9 because its digits may reach b
+// (src/transform_ocl.cpp:42-50).  Registers here hold balanced digits, |d| <= b/2 + 4 <= 10^9 + 4
+// (gfe_set_digits enforces it on input), so every digit is a residue of the 31-bit primes
+// (p >= 2,013,265,921) and the squared (or doubled) coefficient, at most N (b/2 + 4)^2 2 <= 2^84
+// for n <= 23, stays below half their product (2^91.9): NTT3 computes in the 31-bit primes for
+// every b <= 2 10^9, whose lazy reductions make the passes cheaper.  The reference's set still
+// decides what gfe_select reports and the residues of the context payload.
+int transform_is32(int rns, int is32) { return rns == 3 ? 0 : is32; }
+
 // Twiddles Psi[k] = psi^bitrev_n(k), k in [1, N), psi = g^((p-1)/2N), in mulw form.
 static void make_twiddles(int is32, int k, uint32_t n, uint32_t *out) {
     const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k];
@@ -493,7 +502,8 @@ Engine::~Engine() {
 int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     b = b_; n = n_; num_regs = num_regs_; device = device_;
     N = (size_t)1 << n;
-    select_primes(b, n, &rns, &is32);
+    select_primes(b, n, &rns, &ref_is32);
+    is32 = transform_is32(rns, ref_is32);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(GFE_ENODEV, "no HIP device");
     if (device < 0 || device >= ndev) return set_error(GFE_ENODEV, "device index out of range");
@@ -921,7 +931,7 @@ int gfe_save(gfe_engine *e, uint32_t num_regs, void *buf, size_t size) {
     uint32_t *z = reinterpret_cast<uint32_t *>((char *)buf + sizeof(int));
     for (uint32_t r = 0; r < num_regs; ++r)
         for (int k = 0; k < g.rns; ++k) {
-            const uint32_t p = gfe::prime_of(g.is32, k);
+            const uint32_t p = gfe::prime_of(g.ref_is32, k);
             const int32_t *dr = d.data() + (size_t)r * g.N;
             uint32_t *zr = z + ((size_t)r * g.rns + k) * g.N;
             for (size_t i = 0; i < g.N; ++i) zr[i] = dr[i] < 0 ? (uint32_t)dr[i] + p : (uint32_t)dr[i];
@@ -964,7 +974,7 @@ int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size) {
     if (kind != engine_kind(g)) return set_error(GFE_EKIND, "context of another engine kind");
     if (size < gfe_context_size(e, num_regs)) return set_error(GFE_EINVAL, "short context");
     const uint32_t *z = reinterpret_cast<const uint32_t *>((const char *)buf + sizeof(int));
-    const uint32_t p0 = gfe::prime_of(g.is32, 0);
+    const uint32_t p0 = gfe::prime_of(g.ref_is32, 0);
     std::vector<int32_t> d((size_t)num_regs * g.N);
     for (uint32_t r = 0; r < num_regs; ++r) {
         const uint32_t *z0 = z + (size_t)r * g.rns * g.N;
@@ -975,7 +985,7 @@ int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size) {
             dr[i] = (v >= p0 / 2) ? (int32_t)(int64_t(v) - p0) : (int32_t)v;
         }
         for (int k = 1; k < g.rns; ++k) {
-            const uint32_t p = gfe::prime_of(g.is32, k);
+            const uint32_t p = gfe::prime_of(g.ref_is32, k);
             const uint32_t *zk = z0 + (size_t)k * g.N;
             for (size_t i = 0; i < g.N; ++i) {
                 const uint32_t want = dr[i] < 0 ? (uint32_t)dr[i] + p : (uint32_t)dr[i];
@@ -996,7 +1006,9 @@ int gfe_info(const gfe_engine *e, char *buf, size_t len) {
     if (!e || !buf || len == 0) return set_error(GFE_EINVAL, "info");
     const Engine &g = e->eng;
     std::ostringstream ss;
-    ss << "NTT" << g.rns << (g.is32 ? "u" : "s") << " n=" << g.n << " b=" << g.b << "; " << gfe::plan_str(g.plan, g.n);
+    ss << "NTT" << g.rns << (g.is32 ? "u" : "s");
+    if (g.ref_is32 != g.is32) ss << " (reference set NTT" << g.rns << (g.ref_is32 ? "u" : "s") << ")";
+    ss << " n=" << g.n << " b=" << g.b << "; " << gfe::plan_str(g.plan, g.n);
     snprintf(buf, len, "%s", ss.str().c_str());
     return GFE_OK;
 }

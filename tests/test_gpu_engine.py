@@ -31,6 +31,17 @@ def test_prime_set_selection(gfe, oracle, b, n, sel):
 
 
 @pytest.mark.parametrize("b,n,sel", CASES)
+def test_transform_set(gfe, b, n, sel):
+    """NTT3 computes in the 31-bit primes for every b (balanced digits keep it exact); the info
+    string names the reference's set beside it when they differ.  NTT2 keeps the reference's set."""
+    rns, is32 = sel
+    info = gfe.Transform(b, n).info()
+    want = f"NTT{rns}{'u' if is32 and rns == 2 else 's'}"
+    assert info.startswith(want), info
+    assert ("reference set NTT3u" in info) == (rns == 3 and is32), info
+
+
+@pytest.mark.parametrize("b,n,sel", CASES)
 def test_square_dup_matches_oracle(gfe, oracle, b, n, sel):
     """r0 after k PRP iterations (squareDup with the exponent bits) equals the oracle's."""
     iters = 300
