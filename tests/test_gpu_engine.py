@@ -214,6 +214,25 @@ def _admissible(gfe, b, n, plan):
         t.close()
 
 
+def test_ntt3_bound_at_its_extreme(gfe, oracle):
+    """NTT3 computes in the 31-bit primes for every b: at n = 23 and b = 1,999,999,998 (the
+    reference's rule names the 32-bit set) every digit at the balanced maximum b/2 makes the
+    largest coefficient, N (b/2)^2 = 2^82.8, doubled by dup: still below half the 31-bit
+    primes' product (2^91.9).  Two squarings equal the oracle's, which computes in the
+    reference's set."""
+    n, b = 23, 1999999998
+    assert gfe.select(b, n) == (3, True)
+    d = np.full(1 << n, b // 2, dtype=np.int32)
+    t = gfe.Transform(b, n)
+    o = oracle.Engine(b, n)
+    assert t.info().startswith("NTT3s")
+    t.set_digits(d, 0); o.set_digits(d, 0)
+    for dup in (True, False):
+        t.square_dup(dup); o.square_dup(dup)
+        assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b)), dup
+    t.close()
+
+
 @pytest.mark.parametrize("b,n", [(2000000, 21), (150000, 23), (30000000, 23)])
 def test_largest_n_from_full_size_digits(gfe, oracle, b, n):
     """The largest transforms the reference's command line admits (n = 21, 23; NTT2 with 32-bit and
