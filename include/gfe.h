@@ -35,7 +35,9 @@ extern "C" {
 typedef struct gfe_engine gfe_engine;
 
 /* transform::create_gpu (src/transform.h:114-120) -> create_ocl (src/transform_ocl.cpp:20-52):
- * the prime set (2 or 3 primes of 31 or 32 bits) is chosen from (b, n) by the reference's rule.
+ * the prime set (2 or 3 primes of 31 or 32 bits) is chosen from (b, n) by the reference's rule;
+ * it names the engine kind and the context payload's residues.  Three-prime transforms compute in
+ * the 31-bit primes for every b (balanced digits, |d| <= b/2 + 4, keep them exact).
  * 12 <= n <= 23 as the reference's command line allows (src/main.cpp:258-259), 1024 <= b <= 2e9. */
 int gfe_create(uint32_t b, uint32_t n, uint32_t num_regs, int device, gfe_engine **out);
 int gfe_destroy(gfe_engine *e);
@@ -74,11 +76,13 @@ int gfe_save(gfe_engine *e, uint32_t num_regs, void *buf, size_t size);
 int gfe_load(gfe_engine *e, uint32_t num_regs, const void *buf, size_t size);
 /* transform::getError, src/transform.h:71 -- always 0: the arithmetic is exact. */
 double gfe_get_error(const gfe_engine *e);
-/* transform::info, src/transform.h:72 -- prime set and pass plan as text. */
+/* transform::info, src/transform.h:72 -- prime set computed in (and the reference's set when it
+ * differs) and pass plan as text. */
 int gfe_info(const gfe_engine *e, char *buf, size_t len);
 /* Waits for the engine's queued work. */
 int gfe_sync(gfe_engine *e);
-/* Prime-set selection without creating an engine: *rns = 2 or 3, *is32 = 0 (31-bit) or 1 (32-bit). */
+/* The reference's prime-set selection without creating an engine: *rns = 2 or 3, *is32 = 0 (31-bit)
+ * or 1 (32-bit). */
 int gfe_select(uint32_t b, uint32_t n, int *rns, int *is32);
 const char *gfe_last_error(void);
 
