@@ -110,3 +110,17 @@ def test_prime_set_bounds(G, oracle, n):
         sel = G.select(b, n)
         assert sel == oracle.select(b, n)
         assert sel == (3, b > 1_000_000_000), (b, n, sel)
+
+
+def test_ntt3_in_31bit_primes_is_exact_for_every_b():
+    """NTT3 computes in the 31-bit primes for every b <= 2e9: balanced digits (|d| <= b/2 + 4) are
+    residues of each 31-bit prime, and the largest squared-and-doubled coefficient of the negacyclic
+    product, N (b/2 + 4)^2 2, stays below half the three primes' product for every n the engine
+    admits.  The same bound fails for the two-prime set, which keeps the reference's rule."""
+    S = [127 * 2**24 + 1, 63 * 2**25 + 1, 15 * 2**27 + 1]
+    b_max = 2_000_000_000
+    d_max = b_max // 2 + 4
+    assert d_max < min(S)
+    for n in range(12, 24):
+        assert (1 << n) * d_max * d_max * 2 < S[0] * S[1] * S[2] // 2, n
+    assert (1 << 12) * d_max * d_max * 2 > S[0] * S[1] // 2
