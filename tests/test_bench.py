@@ -48,3 +48,19 @@ def test_bench_json_line():
         small = line["config"]["small_configs"]
         assert set(small) == {"C1", "C2", "C3"}
         assert small["C3"]["cpu_candidates_per_hour"] > 0
+
+
+def test_measured_counters_cover_the_default_plan():
+    """bench.py's roofline.traffic and VALU figures come from the profiled plan's counters: the
+    n = 22 default plan must be one of them, or the line would report no traffic."""
+    import importlib.util
+    import genefer22_amd
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    if not os.path.exists(genefer22_amd.LIB_PATH):
+        from genefer22_amd import build
+        build.build()
+    genefer22_amd.lib()
+    m = bench.measured("row", genefer22_amd.default_plan(22))
+    assert m.get("traffic") and m.get("valu_instr_per_butterfly"), m
