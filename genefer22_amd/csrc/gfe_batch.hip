@@ -485,7 +485,7 @@ Batch::~Batch() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-void make_twiddles_set(int is32, int k, uint32_t n, uint32_t *out);  // gfe_engine.hip
+void make_twiddle_tables(const int *sets, const int *primes, int count, uint32_t n, uint32_t *out);  // gfe_engine.hip
 void set_lds_attr(const void *fn);
 
 template <int TWF> void set_row_attrs() {
@@ -530,13 +530,13 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
         return set_error(GFE_ENOMEM, "pinned");
     if (hipHostMalloc((void **)&readback, N * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
     if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return set_error(GFE_EHIP, "event");
-    std::vector<uint32_t> h(N);
-    for (int set = 0; set < 2; ++set)
-        for (int k = 0; k < 3; ++k) {
-            make_twiddles_set(set, k, n, h.data());
-            if (hipMemcpy(tw + ((size_t)set * 3 + k) * N, h.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess)
-                return set_error(GFE_EHIP, "twiddle upload");
-        }
+    {
+        std::vector<uint32_t> h(6 * N);
+        const int sets[6] = {0, 0, 0, 1, 1, 1}, primes[6] = {0, 1, 2, 0, 1, 2};
+        make_twiddle_tables(sets, primes, 6, n, h.data());
+        if (hipMemcpy(tw, h.data(), 6 * N * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(GFE_EHIP, "twiddle upload");
+    }
     for (int k = 0; k < 3; ++k) {
         prS.p[k] = prime_of(0, k); prU.p[k] = prime_of(1, k);
         CrtConst c0 = make_crt(3, 0, n), c1 = make_crt(3, 1, n);

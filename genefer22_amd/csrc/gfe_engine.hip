@@ -19,6 +19,7 @@
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gfe.h"
@@ -129,23 +130,40 @@ void select_primes(uint32_t b, uint32_t n, int *rns, int *is32) {
 // decides what gfe_select reports and the residues of the context payload.
 int transform_is32(int rns, int is32) { return rns == 3 ? 0 : is32; }
 
-// Twiddles Psi[k] = psi^bitrev_n(k), k in [1, N), psi = g^((p-1)/2N), in mulw form.
+// Host Montgomery product a b 2^-32 mod p (a, b < p; q = p^-1 mod 2^32): t - m p has a zero low
+// word, so the difference of the high words is exact.
+static inline uint32_t hmont(uint32_t a, uint32_t b, uint32_t p, uint32_t q) {
+    const uint64_t t = (uint64_t)a * b;
+    const uint32_t m = (uint32_t)t * q;
+    const int64_t r = (int64_t)(t >> 32) - (int64_t)(((uint64_t)m * p) >> 32);
+    return (uint32_t)(r < 0 ? r + p : r);
+}
+
+// Twiddles Psi[j] = psi^bitrev_n(j), j in [0, N), psi = g^((p-1)/2N), in mulw form (Psi[0] = 1 is
+// the j = 0 factor of factored row twiddles).  The powers psi^e 2^32 mod p follow by Montgomery
+// products with psi 2^32 mod p, and e's bit reversal by a reversed increment.
 static void make_twiddles(int is32, int k, uint32_t n, uint32_t *out) {
-    const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k];
+    const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k], q = pinv32(p);
     const size_t N = (size_t)1 << n;
-    const uint32_t psi = hpow(g, (p - 1) >> (n + 1), p);
-    std::vector<uint32_t> pw(N);
-    pw[0] = 1;
-    for (size_t e = 1; e < N; ++e) pw[e] = hmul(pw[e - 1], psi, p);
-    out[0] = mulw_const(1, p);  // Psi[0] = 1: the j = 0 factor of factored row twiddles
-    for (size_t j = 1; j < N; ++j) {
-        size_t r = 0;
-        for (uint32_t bit = 0; bit < n; ++bit) r |= ((j >> bit) & 1) << (n - 1 - bit);
-        out[j] = mulw_const(pw[r], p);
+    const uint32_t psim = mulw_const(hpow(g, (p - 1) >> (n + 1), p), p);
+    uint32_t w = mulw_const(1, p);
+    size_t r = 0;
+    for (size_t e = 0; e < N; ++e) {
+        out[r] = w;
+        w = hmont(w, psim, p, q);
+        size_t bit = N >> 1;
+        while (r & bit) { r ^= bit; bit >>= 1; }
+        r |= bit;
     }
 }
 
-void make_twiddles_set(int is32, int k, uint32_t n, uint32_t *out) { make_twiddles(is32, k, n, out); }
+// Tables of `count` (set, prime) pairs, table i at out + i N, one host thread per table.
+void make_twiddle_tables(const int *sets, const int *primes, int count, uint32_t n, uint32_t *out) {
+    const size_t N = (size_t)1 << n;
+    std::vector<std::thread> th;
+    for (int i = 0; i < count; ++i) th.emplace_back(make_twiddles, sets[i], primes[i], n, out + (size_t)i * N);
+    for (auto &t : th) t.join();
+}
 
 void set_lds_attr(const void *fn) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -524,10 +542,11 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
         return set_error(GFE_ENOMEM, "device allocation failed");
     }
     dev_ok = true;
-    std::vector<uint32_t> htw(N);
-    for (int k = 0; k < rns; ++k) {
-        make_twiddles(is32, k, n, htw.data());
-        if (hipMemcpy(tw + (size_t)k * N, htw.data(), N * 4, hipMemcpyHostToDevice) != hipSuccess)
+    {
+        std::vector<uint32_t> htw((size_t)rns * N);
+        const int sets[3] = {is32, is32, is32}, primes[3] = {0, 1, 2};
+        make_twiddle_tables(sets, primes, rns, n, htw.data());
+        if (hipMemcpy(tw, htw.data(), (size_t)rns * N * 4, hipMemcpyHostToDevice) != hipSuccess)
             return set_error(GFE_EHIP, "twiddle upload");
     }
     const CrtConst c = make_crt(rns, is32, n, b);
