@@ -14,10 +14,14 @@ One JSON line on stdout (progress on stderr):
                  updates every 2000, engine set-up and read-out included, host clock; RES64 against
                  the reference's value at 100,000;
   * batched_n15: candidates/hour at n = 15, 512 seeded candidates in one batch vs one candidate per
-                 launch, from timed iteration blocks (full-test length = esize squarings);
+                 launch, from timed iteration blocks (full-test length = esize squarings); the timed
+                 (autotuned) plan is then run on all 512 candidates for 2048 squarings and every
+                 RES64 / old64 compared with the reference's (the run fails on a mismatch); with
+                 --c3-full the whole workload (512 full tests) runs end to end on the host clock;
   * serving    : p50/p99 time-to-result of seeded Poisson requests of mixed n (12..20) through the
-                 serving loop, and requests/hour against the serial plain path, at a saturating
-                 arrival rate (serving) and at a light one (serving_light: latency near service time);
+                 serving loop, and candidates/hour (each truncated to its first 4096 squarings)
+                 against the serial plain path: the 2000-request C5 workload at about 80 % of capacity
+                 (serving) and 100 requests at a light rate (serving_light: latency near service time);
   * parity     : RES64 after P PRP iterations on the GPU vs the reference's own CPU transform;
   * cpu_baseline: the reference's CPU transform (compiled from its sources into oracle/_ref) timed on
                  this host's cores on a bounded sample of the same workload;
@@ -43,19 +47,25 @@ HBM_PEAK_GBS = 8000.0
 # (integer multiplies, min and add3 issue at this rate on gfx950; 32-bit adds somewhat faster).
 VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9
 
-# Per-launch measurements of the dominant kernel for plans measured with rocprofv3 on MI355X
-# (profiles/n22_spec_plan_kernels.md): HBM bytes = FETCH_SIZE x 1.998 + WRITE_SIZE x 1.000 (the copy
-# kernel's calibration), and VALU instructions per radix-2 butterfly = SQ_INSTS_VALU x 64 / butterflies.
-# Keyed by the plan fields the kernel depends on: the row pass by (a, row_loge, row_ng, flags).
+# Per-launch counters of the dominant kernel for plans profiled with rocprofv3 on MI355X: HBM bytes
+# = FETCH_SIZE x 1.998 + WRITE_SIZE x 1.000 (the copy kernel's calibration) and VALU instructions per
+# radix-2 butterfly = SQ_INSTS_VALU x 64 / butterflies.  These are profile constants, not live
+# measurements: each entry names the kernel (as rocprofv3 prints it) and the summary under profiles/
+# it comes from, and the line reports them with that source.  Keyed by the plan fields the kernel
+# depends on: the row pass by (a, row_loge, row_ng, flags).
 MEASURED = {
-    ("row", 8, 5, 1, 8): {"traffic": int((144.9 + 48.0) * 2**20), "valu_instr_per_butterfly": 11.9},
+    ("row", 8, 5, 1, 8): {"kernel": "gfe::ks_row_sq<gfe::ModS, 5, 14>", "fetch_mib": 144.9, "write_mib": 48.0,
+                          "valu_instr_per_butterfly": 11.9, "profile": "profiles/n22_spec_plan_kernels.md"},
 }
 
 
 def measured(kernel: str, plan) -> dict:
     a, cl, cng, rl, rng, fl = plan
     key = (kernel, a, rl, rng, fl) if kernel == "row" else (kernel, a, cl, cng, fl)
-    return MEASURED.get(key, {})
+    m = dict(MEASURED.get(key, {}))
+    if m:
+        m["traffic"] = int((m["fetch_mib"] + m["write_mib"]) * 2**20)
+    return m
 
 
 def log(*a):
@@ -83,7 +93,7 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     dup_pattern = 0x9B4D_63A1_5C27_E8F0
     t.time_squarings(max(warmup, 1), dup_pattern)
     ms = t.time_squarings(steps, dup_pattern)
-    kreps = min(steps, 200)
+    kreps = 200
     _, km = t.time_squarings(kreps, dup_pattern, per_kernel=True)
     fused = bool(t.get_plan()[5] & 2)
     names = ["col_fwd_all", "row", "col_inv_norm", "fold"] if fused else ["col_fwd", "row", "col_inv", "norm"]
@@ -119,12 +129,16 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
         "roofline": {
             "bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": meas.get("traffic"),
-            "alg_bytes_per_launch": alg_bytes,
+            "alg_bytes_per_launch": alg_bytes, "kernel_us": round(km[dom] * 1e3, 2),
+            "duration_source": f"HIP events around {kreps} back-to-back launches of the kernel alone on the engine's stream",
+            "traffic_source": (f"profile constant: {meas['profile']} ({meas['kernel']}, FETCH_SIZE x 1.998 + WRITE_SIZE)"
+                               if meas else None),
             "note": "the kernel is bound by integer VALU issue (see valu), not by HBM",
         },
         "valu": {
             "kernel": names[dom], "butterflies": bfly, "butterflies_per_s": round(bfly_rate),
             "valu_instr_per_butterfly": ipb,
+            "valu_instr_source": f"profile constant: {meas['profile']} (SQ_INSTS_VALU)" if meas else None,
             "achieved_lane_instr_per_s": round(bfly_rate * ipb) if ipb else None,
             "peak_lane_instr_per_s": VALU_PEAK_LANE_INSTR,
             "frac": round(bfly_rate * ipb / VALU_PEAK_LANE_INSTR, 4) if ipb else None,
@@ -134,7 +148,7 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     return res
 
 
-def bench_batched_n15(G, count: int, block: int, tune: bool = True, plan=None):
+def bench_batched_n15(G, count: int, block: int, tune: bool = True, plan=None, full: bool = False):
     """Candidates/hour at n = 15: `count` seeded candidates in one batch vs one candidate per launch.
 
     Both legs time squaring iterations with HIP events on resident state; candidates/hour =
@@ -158,6 +172,11 @@ def bench_batched_n15(G, count: int, block: int, tune: bool = True, plan=None):
     per_iter_b = bt.time_rounds(4 * block) / (4 * block)
     plan_tuple_b = list(bt.get_plan())
     bt.close()
+    parity = batch_parity_c3(G, bases, plan_tuple_b)
+    log(f"[bench] n=15 parity under the timed plan: {parity}")
+    if not parity["match"]:
+        raise RuntimeError(f"n=15 batch residues under plan {plan_tuple_b} differ from the reference: {parity}")
+    e2e = batch_end_to_end_c3(G, bases, plan_tuple_b) if full else None
     plain_ms = []
     for b in bases[:4]:
         t = G.Transform(b, n)
@@ -176,7 +195,57 @@ def bench_batched_n15(G, count: int, block: int, tune: bool = True, plan=None):
         "candidates_per_hour_batched": round(cand_h_batched, 1),
         "candidates_per_hour_plain": round(cand_h_plain, 1),
         "speedup": round(cand_h_batched / cand_h_plain, 2),
+        "candidates_per_hour_note": "from timed 64-round blocks x mean esize; the Gerbicz-Li products and "
+                                    "register operations are not in it (end_to_end measures them)",
+        "parity": parity,
+        "end_to_end": e2e,
     }
+
+
+def _run_batch(G, bases, iters, plan):
+    bt = G.Batch(15, len(bases))
+    bt.set_plan(plan)
+    for i, b in enumerate(bases):
+        if bt.submit(b, iters, i) is None:
+            raise RuntimeError("batch refused a candidate")
+    got = {}
+    while len(got) < len(bases):
+        bt.step()
+        for r in bt.poll():
+            got[r["tag"]] = r
+    bt.close()
+    return [got[i] for i in range(len(bases))]
+
+
+def batch_parity_c3(G, bases, plan):
+    """Every candidate's RES64 / old64 after 2048 PRP squarings under the timed batch plan against
+    the reference's CPU transform (tests/golden/ref_c3t.json)."""
+    ref = {r["b"]: r for r in json.load(open(os.path.join(ROOT, "tests", "golden", "ref_c3t.json")))["results"]}
+    t0 = time.perf_counter()
+    res = _run_batch(G, bases, 2048, plan)
+    wall = time.perf_counter() - t0
+    checked = [r for r in res if r["b"] in ref]
+    bad = [r["b"] for r in checked if (f"{r['res64']:016X}", f"{r['old64']:016X}") != (ref[r["b"]]["res64"], ref[r["b"]]["old64"])]
+    return {"plan_tuple": list(plan), "squarings": 2048, "candidates": len(res), "checked": len(checked),
+            "mismatches": bad[:8], "match": not bad and len(checked) == len(res), "wall_s": round(wall, 2),
+            "reference": "tests/golden/ref_c3t.json (the reference's CPU transform)"}
+
+
+def batch_end_to_end_c3(G, bases, plan):
+    """The whole C3 workload: every candidate's full quick test (PRP loop, Gerbicz-Li updates and
+    check, read-out) in one batch under the timed plan, on the host clock; RES64 against the
+    reference's full tests where tests/golden/ref_c3f.json holds them."""
+    path = os.path.join(ROOT, "tests", "golden", "ref_c3f.json")
+    ref = {r["b"]: r for r in json.load(open(path))["results"]} if os.path.exists(path) else {}
+    t0 = time.perf_counter()
+    res = _run_batch(G, bases, 0, plan)
+    wall = time.perf_counter() - t0
+    checked = [r for r in res if r["b"] in ref]
+    bad = [r["b"] for r in checked if f"{r['res64']:016X}" != ref[r["b"]]["res64"]]
+    return {"candidates": len(res), "wall_s": round(wall, 1),
+            "candidates_per_hour_measured": round(len(res) / wall * 3600.0, 1),
+            "gl_ok": sum(1 for r in res if r["gl_ok"] == 1), "checked_against_reference": len(checked),
+            "mismatches": bad[:8], "match": not bad}
 
 
 def plain_ms_by_n(G, reqs) -> dict:
@@ -217,6 +286,9 @@ def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
         "plain_alone_p50_s": round(serve.percentile(alone, 50), 5),
         "plain_alone_p99_s": round(serve.percentile(alone, 99), 5),
         "requests_per_hour": summary["requests_per_hour"],
+        "candidates_per_hour": summary["requests_per_hour"],
+        "truncation": f"each candidate runs the first {iters} squarings of its PRP loop (full tests at n = 19-20 "
+                      f"take 13-27 M squarings each)",
         "plain_serial_requests_per_hour": round(count / serial * 3600.0, 1),
         "speedup_vs_plain_serial": round(serial / summary["makespan_s"], 2),
         "plain_ms_per_squaring_by_n": {int(k): round(v, 5) for k, v in ms_by_n.items()},
@@ -227,7 +299,7 @@ def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
 def parity_n22(G, O, b: int, iters: int, threads: int):
     """RES64 after `iters` PRP iterations: GPU driver vs the reference CPU transform."""
     g = G.run_prp(b, 22, iters=iters, b_gl=-1)
-    ref = O.RefEngine(b, 22, threads=threads)
+    ref = O.RefEngine(b, 22, threads=threads, check_error=True)
     es, r, _ = ref.prp(iters=iters, b_gl=-1)
     err = ref.error()
     return {"iters": iters, "gpu_res64": f"{g['res64']:016X}", "ref_res64": f"{r.res64:016X}",
@@ -315,10 +387,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the parity leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the n=15 batched leg")
     ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--c3-full", action="store_true",
+                    help="also run the whole C3 workload end to end (every candidate's full test, about 6 minutes)")
     ap.add_argument("--batch-plan", default=None, help="fixed n=15 batch plan a,cl,cng,rl,rng,flags (default: autotuned)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the C5 serving replay")
+    ap.add_argument("--serve-rate", type=float, default=24.0,
+                    help="arrivals/s of the C5 replay (about 80 %% of the serving loop's capacity for this mix)")
     ap.add_argument("--light-requests", type=int, default=100, help="requests of the light-load serving replay")
     ap.add_argument("--light-rate", type=float, default=8.0, help="arrivals/s of the light-load serving replay")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -345,7 +420,7 @@ def main():
         log(f"[bench] n=22 job: {job}")
 
     bplan = [int(v) for v in args.batch_plan.split(",")] if args.batch_plan else None
-    batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64, not args.no_tune, bplan)
+    batched = None if args.no_batch else bench_batched_n15(G, args.batch, 64, not args.no_tune, bplan, args.c3_full)
     if batched:
         log(f"[bench] n=15: {batched}")
 

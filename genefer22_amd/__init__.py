@@ -87,6 +87,7 @@ ABI = [
     ("gfe_default_plan", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, I32P]),
     ("gfe_plan_admissible", ctypes.c_int, [ctypes.c_uint32, I32P]),
     ("gfe_candidate_plans", ctypes.c_int, [ctypes.c_uint32, I32P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
+    ("gfe_host_twiddles", ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U32P]),
     ("gfe_run_prp_resumable", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
                                              ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Result)]),
     ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),
@@ -317,6 +318,14 @@ def candidate_plans(n: int) -> list[tuple]:
     buf = np.zeros(6 * cnt.value, dtype=np.int32)
     _ck(lib().gfe_candidate_plans(n, _i32(buf), cnt.value, ctypes.byref(cnt)))
     return [tuple(int(x) for x in buf[6 * i:6 * i + 6]) for i in range(cnt.value)]
+
+
+def host_twiddles(n: int, is32: int, max_threads: int = 8) -> np.ndarray:
+    """The engine's twiddle tables of a prime set (no device): 3 x 2^n uint32, entry j of prime k is
+    psi_k^bitrev_n(j) 2^32 mod p_k; `max_threads` host threads build them (0: the calling thread)."""
+    out = np.zeros((3, 1 << n), dtype=np.uint32)
+    _ck(lib().gfe_host_twiddles(n, is32, max_threads, out.ctypes.data_as(U32P)))
+    return out
 
 
 def run_prp_resumable(b: int, n: int, ctx_path: str, save_every: int = 0, stop_after: int = 0,

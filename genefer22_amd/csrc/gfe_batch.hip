@@ -485,7 +485,6 @@ Batch::~Batch() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-void make_twiddle_tables(const int *sets, const int *primes, int count, uint32_t n, uint32_t *out);  // gfe_engine.hip
 void set_lds_attr(const void *fn);
 
 template <int TWF> void set_row_attrs() {

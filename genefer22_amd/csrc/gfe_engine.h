@@ -42,8 +42,15 @@ void select_primes(uint32_t b, uint32_t n, int *rns, int *is32);
 int transform_is32(int rns, int is32);
 uint32_t prime_of(int is32, int k);
 uint32_t mulw_const(uint32_t w, uint32_t p);
+// psi 2^32 mod p for prime k of a set: psi = g^((p-1) / 2^(n+1)), the primitive 2^(n+1)-th root the
+// twiddle tables are powers of; and p^-1 mod 2^32.
+uint32_t psi_mont(int is32, int k, uint32_t n);
+uint32_t prime_qinv(uint32_t p);
+// Sets the kernels' LDS limits once per process (engine, compile-time and single-workgroup passes).
+void engine_set_lds_attributes();
 CrtConst make_crt(int rns, int is32, uint32_t n, uint32_t b = 0);
 int carry_rounds(uint32_t b, uint32_t n);
+void make_twiddle_tables(const int *sets, const int *primes, int count, uint32_t n, uint32_t *out, int max_threads = 8);
 int set_error(int code, const std::string &msg);
 extern thread_local std::string g_last_error;
 
@@ -60,7 +67,6 @@ struct Engine {
     int64_t *carry = nullptr;
     uint32_t *tw = nullptr;
     CrtConst *crt = nullptr;
-    uint32_t *bdev = nullptr;
     int rounds = 0;   // carry rounds of the fused inverse column pass
     int cus = 0;      // compute units of the device
     std::map<std::tuple<const void *, int, size_t>, unsigned> resident;  // per (kernel, threads, LDS)
@@ -76,6 +82,12 @@ struct Engine {
 
     ~Engine();
     int init(uint32_t b, uint32_t n, uint32_t num_regs, int device);
+    // Binds the engine to device memory it does not own (no allocation, nothing freed on
+    // destruction): register 0 = `digits`, the squaring's work space, twiddles, carries and CRT
+    // constants at the given addresses, work queued on `stream`, pass plan `plan`.  Only squarings
+    // (square, flush_carry) may run on a bound engine.
+    void bind(uint32_t b, uint32_t n, int device, hipStream_t stream, int32_t *digits, uint32_t *ws, uint32_t *tw,
+              int64_t *carry, CrtConst *crt, const Plan &plan);
     size_t lds_col() const;
     size_t lds_row() const;
     void launch_col_fwd(uint32_t src, uint32_t *dst);

@@ -112,6 +112,13 @@ static uint32_t pinv32(uint32_t p) {
 // Operand of ModS::mulw / ModU::mulw for the constant w: its Montgomery form w 2^32 mod p.
 uint32_t mulw_const(uint32_t w, uint32_t p) { return (uint32_t)((((uint64_t)w) << 32) % p); }
 
+uint32_t psi_mont(int is32, int k, uint32_t n) {
+    const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k];
+    return mulw_const(hpow(g, (p - 1) >> (n + 1), p), p);
+}
+
+uint32_t prime_qinv(uint32_t p) { return pinv32(p); }
+
 // create_ocl's rule, src/transform_ocl.cpp:35-50.
 void select_primes(uint32_t b, uint32_t n, int *rns, int *is32) {
     const uint64_t bb = (uint64_t)b * b, N = (uint64_t)1 << n;
@@ -157,11 +164,21 @@ static void make_twiddles(int is32, int k, uint32_t n, uint32_t *out) {
     }
 }
 
-// Tables of `count` (set, prime) pairs, table i at out + i N, one host thread per table.
-void make_twiddle_tables(const int *sets, const int *primes, int count, uint32_t n, uint32_t *out) {
+// Tables of `count` (set, prime) pairs, table i at out + i N, one host thread per table.  When a
+// thread cannot be started (std::system_error on thread or process limits) the tables not handed
+// to a thread are built on the calling thread instead; `max_threads` = 0 builds all of them here.
+void make_twiddle_tables(const int *sets, const int *primes, int count, uint32_t n, uint32_t *out, int max_threads) {
     const size_t N = (size_t)1 << n;
     std::vector<std::thread> th;
-    for (int i = 0; i < count; ++i) th.emplace_back(make_twiddles, sets[i], primes[i], n, out + (size_t)i * N);
+    int started = 0;
+    try {
+        th.reserve((size_t)count);
+        for (; started < count && started < max_threads; ++started)
+            th.emplace_back(make_twiddles, sets[started], primes[started], n, out + (size_t)started * N);
+    } catch (const std::exception &) {
+        // `started` counts the threads that did start; the rest are built below
+    }
+    for (int i = started; i < count; ++i) make_twiddles(sets[i], primes[i], n, out + (size_t)i * N);
     for (auto &t : th) t.join();
 }
 
@@ -503,6 +520,12 @@ static void set_lds_attributes() {
     });
 }
 
+void engine_set_lds_attributes() {
+    set_lds_attributes();
+    spec_set_lds_attributes();
+    fused_set_lds_attributes();
+}
+
 // ------------------------------------------------------------------------------------------------
 // Engine
 
@@ -511,7 +534,7 @@ Engine::~Engine() {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         (void)hipFree(regs); (void)hipFree(ws); (void)hipFree(zp); (void)hipFree(carry);
-        (void)hipFree(tw); (void)hipFree(crt); (void)hipFree(bdev);
+        (void)hipFree(tw); (void)hipFree(crt);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto &ev : events) (void)hipEventDestroy(ev);
     }
@@ -536,8 +559,7 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     if (hipMalloc(&regs, (size_t)num_regs * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)rns * N * 4) != hipSuccess ||
         hipMalloc(&zp, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&carry, (N / 8) * 8) != hipSuccess ||
 
-        hipMalloc(&tw, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&crt, sizeof(CrtConst)) != hipSuccess ||
-        hipMalloc(&bdev, 4) != hipSuccess) {
+        hipMalloc(&tw, (size_t)rns * N * 4) != hipSuccess || hipMalloc(&crt, sizeof(CrtConst)) != hipSuccess) {
         dev_ok = true;
         return set_error(GFE_ENOMEM, "device allocation failed");
     }
@@ -551,12 +573,27 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     }
     const CrtConst c = make_crt(rns, is32, n, b);
     if (hipMemcpy(crt, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "crt upload");
-    if (hipMemcpy(bdev, &b, 4, hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "b upload");
     for (int k = 0; k < 3; ++k) { pr.p[k] = prime_of(is32, k); pr.q[k] = pinv32(pr.p[k]); }
     if (hipMemsetAsync(regs, 0, (size_t)num_regs * N * 4, stream) != hipSuccess) return set_error(GFE_EHIP, "memset");
     for (int i = 0; i < 12; ++i) { hipEvent_t ev; if (hipEventCreate(&ev) != hipSuccess) return set_error(GFE_EHIP, "event"); events.push_back(ev); }
     if (hipStreamSynchronize(stream) != hipSuccess) return set_error(GFE_EHIP, "init sync");
     return GFE_OK;
+}
+
+void Engine::bind(uint32_t b_, uint32_t n_, int device_, hipStream_t stream_, int32_t *digits, uint32_t *ws_,
+                  uint32_t *tw_, int64_t *carry_, CrtConst *crt_, const Plan &plan_) {
+    b = b_; n = n_; num_regs = 1; device = device_;
+    N = (size_t)1 << n;
+    select_primes(b, n, &rns, &ref_is32);
+    is32 = transform_is32(rns, ref_is32);
+    stream = stream_;
+    regs = digits; ws = ws_; zp = nullptr; tw = tw_; carry = carry_; crt = crt_;
+    plan = plan_;
+    rounds = carry_rounds(b, n);
+    cus = 256;
+    for (int k = 0; k < 3; ++k) { pr.p[k] = prime_of(is32, k); pr.q[k] = pinv32(pr.p[k]); }
+    carry_pending = false;
+    dev_ok = false;  // nothing here is the engine's to free
 }
 
 size_t Engine::lds_col() const {
@@ -836,6 +873,7 @@ int gfe_destroy(gfe_engine *e) {
 
 int gfe_set(gfe_engine *e, uint32_t a) {
     GFE_GUARD(e);
+    if ((uint64_t)a > (uint64_t)(e->eng.b / 2) + 4) return set_error(GFE_EINVAL, "set: a exceeds b/2 + 4");
     e->eng.flush_carry();
     e->eng.set(a);
     return e->eng.check("set");
@@ -1051,40 +1089,38 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
     float t = 0;
     (void)hipEventElapsedTime(&t, e0, e1);
     *ms = t;
-    if (kernel_ms && g.plan.fuse_sq) {  // one kernel per squaring
-        double acc = 0;
-        for (uint32_t i = 0; i < count; ++i) {
+    if (kernel_ms) {
+        // each pass alone: `count` back-to-back launches between one pair of events on the engine's
+        // stream, so the average is the pass's own duration (launch gaps amortised).  Register 0
+        // stays a valid residue: one squaring (dup = 0) leaves the product's residues in ws, the
+        // normalisation rewrites register 0 from them identically each time, and the forward
+        // column pass only reads it; the row and inverse column passes work on ws alone.
+        auto time_k = [&](auto &&launch, double *out) -> int {
             (void)hipEventRecord(g.events[2], g.stream);
-            g.launch_fused((dup_bits >> (i % 64)) & 1);
+            for (uint32_t i = 0; i < count; ++i) launch();
             (void)hipEventRecord(g.events[3], g.stream);
             if (hipEventSynchronize(g.events[3]) != hipSuccess) return set_error(GFE_EHIP, "event sync");
             float d = 0;
             (void)hipEventElapsedTime(&d, g.events[2], g.events[3]);
-            acc += d;
-        }
-        for (int k = 0; k < 5; ++k) kernel_ms[k] = (k == 0) ? acc / count : 0.0;
-    } else if (kernel_ms) {
-        double acc[5] = {0, 0, 0, 0, 0};
-        const int nk = 4;
-        for (uint32_t i = 0; i < count; ++i) {
-            const int dup = (dup_bits >> (i % 64)) & 1;
-            (void)hipEventRecord(g.events[2], g.stream);
+            *out = d / count;
+            return GFE_OK;
+        };
+        for (int k = 0; k < 5; ++k) kernel_ms[k] = 0.0;
+        int rc = GFE_OK;
+        if (g.plan.fuse_sq) {
+            rc = time_k([&]() { g.launch_fused(0); }, &kernel_ms[0]);
+        } else {
+            g.flush_carry();
             g.launch_col_fwd(0, g.ws);
-            (void)hipEventRecord(g.events[3], g.stream);
             g.launch_row(gfe::ROW_SQUARE, g.ws);
-            (void)hipEventRecord(g.events[4], g.stream);
-            g.launch_col_inv(dup);
-            (void)hipEventRecord(g.events[5], g.stream);
-            g.launch_norm(dup);
-            (void)hipEventRecord(g.events[6], g.stream);
-            if (hipEventSynchronize(g.events[6]) != hipSuccess) return set_error(GFE_EHIP, "event sync");
-            for (int k = 0; k < nk; ++k) {
-                float d = 0;
-                (void)hipEventElapsedTime(&d, g.events[2 + k], g.events[3 + k]);
-                acc[k] += d;
-            }
+            g.launch_col_inv(0);
+            if (rc == GFE_OK) rc = time_k([&]() { g.launch_norm(0); }, &kernel_ms[3]);
+            g.flush_carry();
+            if (rc == GFE_OK) rc = time_k([&]() { g.launch_col_fwd(0, g.ws); }, &kernel_ms[0]);
+            if (rc == GFE_OK) rc = time_k([&]() { g.launch_row(gfe::ROW_SQUARE, g.ws); }, &kernel_ms[1]);
+            if (rc == GFE_OK) rc = time_k([&]() { g.launch_col_inv(0); }, &kernel_ms[2]);
         }
-        for (int k = 0; k < 5; ++k) kernel_ms[k] = (k < nk) ? acc[k] / count : 0.0;
+        if (rc != GFE_OK) return rc;
     }
     return g.check("time_squarings");
 }
@@ -1173,6 +1209,14 @@ int gfe_candidate_plans(uint32_t n, int32_t *plans, uint32_t max, uint32_t *coun
     *count = (uint32_t)c.size();
     if (plans)
         for (uint32_t i = 0; i < std::min<uint32_t>(max, *count); ++i) gfe::plan_to_array(c[i], plans + 6 * (size_t)i);
+    return GFE_OK;
+}
+
+int gfe_host_twiddles(uint32_t n, int is32, int max_threads, uint32_t *out) {
+    if (!out || n < 1 || n > 23 || (is32 != 0 && is32 != 1) || max_threads < 0)
+        return set_error(GFE_EINVAL, "host_twiddles");
+    const int sets[3] = {is32, is32, is32}, primes[3] = {0, 1, 2};
+    gfe::make_twiddle_tables(sets, primes, 3, n, out, max_threads);
     return GFE_OK;
 }
 

@@ -42,7 +42,9 @@ typedef struct gfe_engine gfe_engine;
 int gfe_create(uint32_t b, uint32_t n, uint32_t num_regs, int device, gfe_engine **out);
 int gfe_destroy(gfe_engine *e);
 
-/* transform::set, src/transform.h:57 -- r0 = a. */
+/* transform::set, src/transform.h:57 -- r0 = a.  Registers hold balanced digits, so a must not
+ * exceed b/2 + 4 (GFE_EINVAL otherwise); the reference's callers set 1 or small bases
+ * (src/genefer.h:445-458, 489). */
 int gfe_set(gfe_engine *e, uint32_t a);
 /* transform::squareDup, src/transform.h:58 -- r0 = r0^2, or 2 r0^2 when dup != 0. */
 int gfe_square_dup(gfe_engine *e, int dup);
@@ -87,9 +89,12 @@ int gfe_select(uint32_t b, uint32_t n, int *rns, int *is32);
 const char *gfe_last_error(void);
 
 /* Times `count` squarings (dup pattern: dup_bits bit i for squaring i mod 64) with HIP events on
- * the engine's stream, after the queued work.  *ms = total; kernel_ms[0..4] = average duration of
- * one launch of each pass (column forward, row, column inverse, normalise with carry; [4] = 0),
- * measured in a second, per-launch-timed run of the same count (kernel_ms may be null). */
+ * the engine's stream, after the queued work.  *ms = total.  When kernel_ms is not null,
+ * kernel_ms[0..4] = average duration of one launch of each pass (column forward, row, column
+ * inverse, normalisation; [4] = 0; a single-workgroup plan reports its one kernel in [0]), each
+ * from `count` back-to-back launches of that pass alone between one pair of events; that run
+ * squares register 0 once more (dup = 0), so register 0 ends as the value squared count (+ 1)
+ * times. */
 int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double *ms, double *kernel_ms);
 
 /* Autotuner: times every admissible pass plan for this engine's n on the device and keeps the
@@ -113,6 +118,13 @@ int gfe_set_plan(gfe_engine *e, const int32_t plan[6]);
 int gfe_default_plan(uint32_t n, uint32_t capacity, int32_t plan[6]);
 int gfe_plan_admissible(uint32_t n, const int32_t plan[6]);
 int gfe_candidate_plans(uint32_t n, int32_t *plans, uint32_t max, uint32_t *count);
+
+/* The twiddle tables the engine uploads (host only, no device; the reference's root tables,
+ * src/transformGPU.h:1046-1078, in this engine's layout): for each of the three primes of the
+ * 31-bit (is32 = 0) or 32-bit (is32 = 1) set, N = 2^n entries Psi[j] = psi^bitrev_n(j) 2^32 mod p,
+ * psi = g^((p-1) / 2N), written to out + k N.  The tables are built on up to max_threads host
+ * threads (0: on the calling thread only); the result does not depend on it. */
+int gfe_host_twiddles(uint32_t n, int is32, int max_threads, uint32_t *out);
 
 #ifdef __cplusplus
 }

@@ -45,8 +45,10 @@ int gfe_batch_create(uint32_t n, uint32_t capacity, int device, gfe_batch **out)
 int gfe_batch_destroy(gfe_batch *bt);
 /* Device bytes one slot holds (registers, work space, multiplicand, carries): the admission unit. */
 size_t gfe_batch_slot_bytes(uint32_t n);
-/* Chooses the pass plan by timing the admissible ones on `reps` rounds of the batch's current
- * slots (or of synthetic full slots when empty); the reference's tune, src/transformGPU.h:851-900. */
+/* Chooses the pass plan by timing the admissible ones on `reps` rounds of the batch's busy slots
+ * (their registers are saved and restored, so their programs continue unchanged); returns
+ * GFE_EINVAL when no slot is busy: submit the candidates first.  The reference's tune,
+ * src/transformGPU.h:851-900. */
 int gfe_batch_tune(gfe_batch *bt, uint32_t reps);
 int gfe_batch_info(const gfe_batch *bt, char *buf, size_t len);
 /* The batch's pass plan, as gfe_get_plan / gfe_set_plan of gfe.h. */

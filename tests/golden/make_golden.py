@@ -13,7 +13,11 @@ Outputs (all values are the reference's, hex RES64 / old64):
   ref_c2.json  : C2 (n=16, b in [1e8, 1.5e9]), 2 candidates, RES64 every 1000 squarings to 200,000
                  and the SHA-256 of the canonical digits at 200,000 (int32 little-endian, digit 0 first)
   ref_c3t.json : C3 (n=15, b in [1e8, 2e9)), all 512 candidates, RES64 after 2048 squarings
-  ref_c3f.json : C3, the first 12 candidates, full quick tests
+  ref_c3f.json : C3, full quick tests of the first --c3f-count candidates (default all 512); written
+                 after every finished candidate and resumed from what the file already holds
+  ref_c1i32.json : C1 full quick tests through the reference's exact integer transform (impl "i32",
+                 src/transformCPUi32.h), the first 4 bases of ref_c1.json
+  ref_c3i32.json : C3, 16 candidates, RES64 after 2048 squarings through the "i32" transform
   ref_c4.json  : C4 (n=22, b in [1e7, 3.5e7]), 1 candidate, RES64 every 2000 squarings to 100,000
                  and the SHA-256 of the canonical digits at 100,000
   ref_c5.json  : C5 serving mix, the first 48 requests, RES64 after their 4096 squarings
@@ -35,21 +39,21 @@ sys.path.insert(0, ROOT)
 from genefer22_amd import workloads as W  # noqa: E402
 
 
-def _ref(b, n, threads, check_error=False):
+def _ref(b, n, threads, check_error=False, impl=""):
     import oracle as O
-    return O.RefEngine(b, n, threads=threads, check_error=check_error)
+    return O.RefEngine(b, n, threads=threads, check_error=check_error, impl=impl)
 
 
-def job_full(b, n, threads):
+def job_full(b, n, threads, impl=""):
     t0 = time.time()
-    r = _ref(b, n, threads)
+    r = _ref(b, n, threads, impl=impl)
     es, res, gl = r.prp(iters=0, b_gl=0, do_gl=True)
     return {"b": b, "n": n, "esize": es, "res64": f"{res.res64:016X}", "old64": f"{res.old64:016X}",
             "is_prp": res.is_one, "gl_ok": gl, "impl": r.impl, "seconds": round(time.time() - t0, 1)}
 
 
-def job_trunc(b, n, iters, threads, check_error=False):
-    r = _ref(b, n, threads, check_error)
+def job_trunc(b, n, iters, threads, check_error=False, impl=""):
+    r = _ref(b, n, threads, check_error, impl)
     es, res, _ = r.prp(iters=iters, b_gl=0, do_gl=False)
     return {"b": b, "n": n, "iters": iters, "res64": f"{res.res64:016X}", "old64": f"{res.old64:016X}",
             "impl": r.impl, "max_error": r.error()}
@@ -66,16 +70,36 @@ def job_trace(b, n, iters, every, threads, check_error=False):
             "impl": r.impl, "max_error": r.error(), "seconds": round(time.time() - t0, 1)}
 
 
-def write(name, obj):
+def _full_resumable(ex, name, cfg_name, bases, n, src):
+    """Full quick tests of `bases`, the file rewritten (in base order) after each finished test and
+    the tests the file already holds skipped, so a long run can be stopped and resumed."""
+    from concurrent.futures import as_completed
+    path = os.path.join(HERE, name)
+    done = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            for r in json.load(f)["results"]:
+                done[r["b"]] = r
+    futs = {ex.submit(job_full, b, n, 1): b for b in bases if b not in done}
+    for fut in as_completed(futs):
+        done[futs[fut]] = fut.result()
+        write(name, {"source": src, "config": cfg_name,
+                     "results": [done[b] for b in bases if b in done]}, quiet=True)
+        print(f"{name}: {len(done)}/{len(bases)}", flush=True)
+
+
+def write(name, obj, quiet=False):
     with open(os.path.join(HERE, name), "w") as f:
         json.dump(obj, f, indent=1)
-    print(f"wrote {name}", flush=True)
+    if not quiet:
+        print(f"wrote {name}", flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
-    ap.add_argument("--only", default="c1,c2,c3t,c3f,c4,c5")
+    ap.add_argument("--only", default="c1,c1i32,c3i32,c2,c3t,c3f,c4,c5")
+    ap.add_argument("--c3f-count", type=int, default=512)
     args = ap.parse_args()
     only = set(args.only.split(","))
     ex = ProcessPoolExecutor(max_workers=args.jobs)
@@ -98,10 +122,18 @@ def main():
         bases = W.config_bases(W.C2, 2)
         futs = [ex.submit(job_trace, b, 16, 200_000, 1000, 4) for b in bases]
         write("ref_c2.json", {"source": src, "config": "C2", "results": [f.result() for f in futs]})
+    if "c1i32" in only:
+        bases = W.config_bases(W.C1, 4)
+        futs = [ex.submit(job_full, b, 12, 1, "i32") for b in bases]
+        write("ref_c1i32.json", {"source": src.replace("transforms", "transform_i32"), "config": "C1 (i32)",
+                                 "results": [f.result() for f in futs]})
+    if "c3i32" in only:
+        bases = W.config_bases(W.C3, 16)
+        futs = [ex.submit(job_trunc, b, 15, 2048, 1, False, "i32") for b in bases]
+        write("ref_c3i32.json", {"source": src.replace("transforms", "transform_i32"),
+                                 "config": "C3 truncated (i32)", "results": [f.result() for f in futs]})
     if "c3f" in only:
-        bases = W.config_bases(W.C3, 12)
-        futs = [ex.submit(job_full, b, 15, 1) for b in bases]
-        write("ref_c3f.json", {"source": src, "config": "C3 full tests", "results": [f.result() for f in futs]})
+        _full_resumable(ex, "ref_c3f.json", "C3 full tests", W.config_bases(W.C3, args.c3f_count), 15, src)
     if "c4" in only:
         b = W.config_bases(W.C4, 1)[0]
         r = job_trace(b, 22, 100_000, 2000, args.jobs, True)

@@ -51,8 +51,9 @@ def test_bench_json_line():
 
 
 def test_measured_counters_cover_the_default_plan():
-    """bench.py's roofline.traffic and VALU figures come from the profiled plan's counters: the
-    n = 22 default plan must be one of them, or the line would report no traffic."""
+    """bench.py's roofline.traffic and VALU figures are profile constants of the profiled plan: the
+    n = 22 default plan must be one of them, the kernel that plan's row pass instantiates must be the
+    one the entry names, and the named profile summary must hold that kernel and those figures."""
     import importlib.util
     import genefer22_amd
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
@@ -62,5 +63,15 @@ def test_measured_counters_cover_the_default_plan():
         from genefer22_amd import build
         build.build()
     genefer22_amd.lib()
-    m = bench.measured("row", genefer22_amd.default_plan(22))
+    plan = genefer22_amd.default_plan(22)
+    m = bench.measured("row", plan)
     assert m.get("traffic") and m.get("valu_instr_per_butterfly"), m
+    a, cl, cng, rl, rng, fl = plan
+    assert fl == 8, plan
+    assert m["kernel"].endswith(f", {rl}, {22 - a}>"), (m["kernel"], plan)
+    text = open(os.path.join(ROOT, m["profile"])).read()
+    short = m["kernel"].replace("gfe::", "")
+    assert m["kernel"] in text and short in text
+    row = next(l for l in text.splitlines() if l.startswith(f"| `{short}`") and "MiB" not in l and "M |" in l)
+    assert f"{m['fetch_mib']}" in row and f"{m['write_mib']}" in row, row
+    assert f"{m['valu_instr_per_butterfly']} VALU instructions" in text

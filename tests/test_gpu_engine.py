@@ -195,9 +195,10 @@ def test_pending_carries_reach_every_use(gfe, oracle, b, n):
     t.square_dup(False); o.square_dup(False)
     pattern = 0b1011
     t.time_squarings(6, pattern)                      # timed squarings, then the per-kernel leg
-    t.time_squarings(3, pattern, per_kernel=True)    # 3 timed squarings, then 3 timed kernel by kernel
-    for i in list(range(6)) + list(range(3)) + list(range(3)):
+    t.time_squarings(3, pattern, per_kernel=True)    # 3 timed squarings, then each pass timed alone
+    for i in list(range(6)) + list(range(3)):
         o.square_dup((pattern >> (i % 64)) & 1)
+    o.square_dup(False)                               # the per-pass leg squares once (dup = 0)
     assert np.array_equal(got(0), want(0))
     assert np.abs(t.get_digits(0)).max() <= b // 2 + 4
     t.close()
@@ -250,4 +251,18 @@ def test_largest_n_from_full_size_digits(gfe, oracle, b, n):
     got = t.get_digits(0)
     assert np.abs(got.astype(np.int64)).max() <= b // 2 + 4
     assert np.array_equal(canon(oracle, got, n, b), canon(oracle, o.get_digits(0), n, b))
+    t.close()
+
+
+def test_set_rejects_values_beyond_the_balanced_bound(gfe, oracle):
+    """Registers hold balanced digits: set(a) accepts a <= b/2 + 4 and refuses larger a (a value
+    that would wrap int32 or exceed a 31-bit prime's residues would otherwise come back wrong)."""
+    b, n = 1000, 12
+    t = gfe.Transform(b, n)
+    t.set(b // 2 + 4)
+    d = t.get_digits(0)
+    assert d[0] == b // 2 + 4 and not d[1:].any()
+    for a in (b // 2 + 5, 2**31, 2**32 - 1):
+        with pytest.raises(gfe.GfeError):
+            t.set(a)
     t.close()

@@ -124,3 +124,25 @@ def test_ntt3_in_31bit_primes_is_exact_for_every_b():
     for n in range(12, 24):
         assert (1 << n) * d_max * d_max * 2 < S[0] * S[1] * S[2] // 2, n
     assert (1 << 12) * d_max * d_max * 2 > S[0] * S[1] // 2
+
+
+@pytest.mark.parametrize("is32", [0, 1])
+def test_host_twiddle_tables(G, is32):
+    """The engine's twiddle tables from Montgomery-product recurrences equal direct powers
+    psi^bitrev(j) 2^32 mod p, psi a primitive 2N-th root (psi^N = -1); built on the calling thread
+    alone (the fallback when no thread can be started) and on several threads they agree."""
+    n = 10
+    N = 1 << n
+    primes = ([127 * 2**24 + 1, 63 * 2**25 + 1, 15 * 2**27 + 1], [125 * 2**25 + 1, 243 * 2**24 + 1, 235 * 2**24 + 1])[is32]
+    gens = ([3, 5, 31], [3, 7, 3])[is32]
+    serial = G.host_twiddles(n, is32, max_threads=0)
+    assert np.array_equal(serial, G.host_twiddles(n, is32, max_threads=3))
+    assert np.array_equal(serial, G.host_twiddles(n, is32, max_threads=1))
+    for k, (p, g) in enumerate(zip(primes, gens)):
+        psi = pow(g, (p - 1) // (2 * N), p)
+        assert pow(psi, N, p) == p - 1
+        for j in range(N):
+            e = int(format(j, f"0{n}b")[::-1], 2)
+            assert int(serial[k, j]) == (pow(psi, e, p) << 32) % p, (k, j)
+    with pytest.raises(G.GfeError):
+        G.host_twiddles(24, is32)
