@@ -94,6 +94,10 @@ ABI = [
     ("gfe_residue", ctypes.c_int, [I32P, ctypes.c_uint32, ctypes.c_uint32, I32P, ctypes.POINTER(Result)]),
     ("gfe_write_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                        ctypes.c_int, U64P, ctypes.POINTER(Result)]),
+    ("gfe_run_server", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_uint32, U64P, ctypes.POINTER(Result)]),
+    ("gfe_run_check", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, U64P,
+                                     ctypes.POINTER(Result)]),
     ("gfe_batch_create", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("gfe_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("gfe_batch_slot_bytes", ctypes.c_size_t, [ctypes.c_uint32]),
@@ -374,6 +378,29 @@ def write_proof(b: int, n: int, depth: int, main_path: str, checkpoints: bool = 
     _ck(lib().gfe_write_proof(b, n, device, depth, main_path.encode(), flags, mu, ctypes.byref(r)))
     out = r.as_dict()
     out["mu_hash64"] = list(mu)
+    return out
+
+
+def run_server(b: int, n: int, main_path: str, rnd: tuple[int, int, int], device: int = 0) -> dict:
+    """server(): reads `<main_path>.proof`, recomputes the proof key and writes the certificate
+    `<main_path>.cert` with the random values rnd = (rnd1, rnd2, rnd3) (src/genefer.h:771-880);
+    returns the result fields (pkey, and res64 / old64 / is_prp of mu_0) and the check key."""
+    r = Result()
+    ck = ctypes.c_uint64()
+    _ck(lib().gfe_run_server(b, n, device, main_path.encode(), rnd[0], rnd[1], rnd[2], ctypes.byref(ck), ctypes.byref(r)))
+    out = r.as_dict()
+    out["ckey"] = ck.value
+    return out
+
+
+def run_check(b: int, n: int, main_path: str, device: int = 0) -> dict:
+    """check(): reads the certificate `<main_path>.cert` and returns gl_ok (1: both Gerbicz tests
+    passed) and the check key (src/genefer.h:882-1062)."""
+    r = Result()
+    ck = ctypes.c_uint64()
+    _ck(lib().gfe_run_check(b, n, device, main_path.encode(), ctypes.byref(ck), ctypes.byref(r)))
+    out = r.as_dict()
+    out["ckey"] = ck.value
     return out
 
 

@@ -296,6 +296,213 @@ public:
         return std::vector<int32_t>(_g.data(), _g.data() + _g.getSize());
     }
 
+    // The server's side of the proof protocol (src/genefer.h:771-880): verifies nothing by itself
+    // but turns a proof file into the proof key and a certificate that only a correct proof lets a
+    // checker close.  From the records mu_0..mu_depth of `proof_path` (q_k = hash32(mu_k)):
+    // v1 = prod_k mu_k^{w_k} as the prover formed it (proof key = hash64(v1)), v2 folded as
+    // v2 <- v2^{q_k} mu_k, and the exponent p2 = sum_i w_i e_i over the B-bit chunks e_i of E
+    // (B = B_PietrzakLi), with the weights w grown as the prover grew them.  The certificate
+    // (`cert_path`) holds int version 1, int B, the residue record of v2^rnd2 (v2 first multiplied
+    // by 2^rnd1 when p2 > rnd1 2^B, p2 then losing rnd1 2^B), p2 rnd2 + rnd3 in GMP's raw format
+    // (outside the CRC, as the reference's file writes it) and the CRC trailer; the check key is
+    // hash64(2^rnd3 v1^rnd2).  The reference draws rnd1 in [2, 64), rnd2 in [16, 256) and rnd3 in
+    // [4, 65536) from std::rand; here they are arguments.  mu_0 is the PRP residue: is_prp, res64
+    // and old64 come from it.  Needs 4 registers; throws on a malformed or corrupted file.
+    void server(const std::string &proof_path, const std::string &cert_path, uint32_t rnd1, uint32_t rnd2,
+                uint32_t rnd3, bool &is_prp, uint64_t &res64, uint64_t &old64, uint64_t &pkey, uint64_t &ckey) {
+        transformHIP &T = *_t;
+        file pf(proof_path, "rb");
+        if (!pf.exists()) throw std::runtime_error("cannot open " + proof_path);
+        int version = 0, depth = 0;
+        if (!pf.read(reinterpret_cast<char *>(&version), sizeof(version)) ||
+            !pf.read(reinterpret_cast<char *>(&depth), sizeof(depth)) || version != 1 || depth < 1 || depth > 20)
+            throw std::runtime_error("bad proof file header " + proof_path);
+        const size_t L = size_t(1) << depth;
+        const int B = B_PietrzakLi(esize(), depth);
+        mpz_vec w(L);
+        read_record(pf, proof_path);  // v1 = mu_0^w_0
+        is_prp = _g.isOne(res64, old64);
+        uint32_t q = _g.gethash32();
+        mpz_set_ui(w[0], q);
+        T.setInt(_g);
+        power(0, q);
+        T.copy(1, 0);
+        T.set(1);  // v2 = 1
+        T.copy(2, 0);
+        for (int k = 1; k <= depth; ++k) {
+            read_record(pf, proof_path);  // mu_k -> r3
+            q = _g.gethash32();
+            T.setInt(_g);
+            T.copy(3, 0);
+            power(0, q);  // v1 = v1 mu_k^q_k
+            T.mul(1);
+            T.copy(1, 0);
+            power(2, q);  // v2 = v2^q_k mu_k
+            T.mul(3);
+            T.copy(2, 0);
+            const size_t i = size_t(1) << (depth - k);
+            for (size_t j = 0; j < L; j += 2 * i) mpz_mul_ui(w[i + j], w[j], q);
+        }
+        if (!pf.check_crc32()) throw std::runtime_error("bad proof file (crc32) " + proof_path);
+        T.copy(0, 1);
+        T.getInt(_g);
+        pkey = _g.gethash64();
+        // p2 = sum_i w_i (E >> (i B) mod 2^B)
+        mpz_t p2, t, e;
+        mpz_init_set_ui(p2, 0);
+        mpz_init(t);
+        mpz_init_set(e, _E);
+        for (size_t i = 0; i < L; ++i) {
+            mpz_tdiv_r_2exp(t, e, (mp_bitcnt_t)B);
+            mpz_addmul(p2, t, w[i]);
+            mpz_tdiv_q_2exp(e, e, (mp_bitcnt_t)B);
+        }
+        mpz_clear(e);
+        mpz_set_ui(t, rnd1);
+        mpz_mul_2exp(t, t, (mp_bitcnt_t)B);
+        if (mpz_cmp(p2, t) > 0) {
+            mpz_sub(p2, p2, t);
+            T.set(2);
+            power(0, rnd1);
+            T.mul(2);
+            T.copy(2, 0);
+        }
+        power(1, rnd2);
+        T.copy(1, 0);
+        T.set(2);
+        power(0, rnd3);
+        T.mul(1);
+        T.getInt(_g);
+        ckey = _g.gethash64();
+        power(2, rnd2);
+        T.getInt(_g);
+        mpz_mul_ui(p2, p2, rnd2);
+        mpz_add_ui(p2, p2, rnd3);
+        {
+            file cf(cert_path, "wb");
+            const int cversion = 1;
+            bool ok = cf.exists() && cf.write(reinterpret_cast<const char *>(&cversion), sizeof(cversion)) &&
+                      cf.write(reinterpret_cast<const char *>(&B), sizeof(B));
+            _g.unbalance();
+            ok = ok && write_record(cf, _g.data(), _g.getSize(), _g.getBase());
+            ok = ok && mpz_out_raw(cf.handle(), p2) != 0;
+            if (ok) cf.write_crc32();
+            ok = ok && cf.close();
+            mpz_clear(p2);
+            mpz_clear(t);
+            if (!ok) throw std::runtime_error("cannot write " + cert_path);
+        }
+    }
+
+    // The checker's side (src/genefer.h:882-1062, without its context file): reads the certificate
+    // (int version, int B, residue record u, p2 in GMP's raw format, CRC) and computes
+    // v2' = u^(2^B) under the Gerbicz test of block L = B_GerbiczLi(B) (d(t) = prod of u(jL),
+    // checked as u(0) d(t)^(2^L) == d(t+1)), then 2^p2 under the Gerbicz-Li test of block
+    // GL = B_GerbiczLi(bits of p2); the check key is hash64(v2' 2^p2), which equals the server's
+    // when the proof was correct.  Returns false when a test fails.  Needs 4 registers.
+    bool check(const std::string &cert_path, uint64_t &ckey) {
+        transformHIP &T = *_t;
+        int B = 0;
+        mpz_t p2;
+        mpz_init(p2);
+        {
+            file cf(cert_path, "rb");
+            if (!cf.exists()) {
+                mpz_clear(p2);
+                throw std::runtime_error("cannot open " + cert_path);
+            }
+            int version = 0;
+            bool ok = cf.read(reinterpret_cast<char *>(&version), sizeof(version)) &&
+                      cf.read(reinterpret_cast<char *>(&B), sizeof(B)) && version == 1 && B > 0;
+            if (ok) read_record(cf, cert_path);
+            ok = ok && mpz_inp_raw(p2, cf.handle()) != 0 && cf.check_crc32();
+            if (!ok) {
+                mpz_clear(p2);
+                throw std::runtime_error("bad certificate file " + cert_path);
+            }
+        }
+        const int p2size = (int)mpz_sizeinbase(p2, 2);
+        const int L = B_GerbiczLi((size_t)B), GLb = B_GerbiczLi((size_t)p2size);
+        // v2' = u^(2^B), d(t) = prod u(jL)
+        const int i0 = p2size + B - 1;
+        T.setInt(_g);
+        T.copy(1, 0);
+        for (int i = i0; i >= p2size; --i) {
+            const int j = i0 - i;
+            if (j % L == 0 && j != 0) {
+                T.copy(2, 0);
+                T.mul(1);
+                T.copy(1, 0);
+                T.copy(0, 2);
+            }
+            T.squareDup(false);
+        }
+        T.copy(2, 0);
+        if (B % L != 0)
+            for (int i = L - (B % L); i > 0; --i) T.squareDup(false);  // u((t + 1) L)
+        T.mul(1);  // d(t + 1)
+        T.copy(3, 0);
+        T.copy(0, 1);
+        for (int i = L; i > 0; --i) T.squareDup(false);
+        T.copy(1, 0);
+        T.setInt(_g);  // u(0) d(t)^(2^L) == d(t + 1)
+        T.mul(1);
+        T.getInt(_g);
+        uint64_t h1 = _g.gethash64();
+        T.copy(0, 3);
+        T.getInt(_g);
+        uint64_t h2 = _g.gethash64();
+        if (h1 != h2) {
+            mpz_clear(p2);
+            return false;
+        }
+        // 2^p2 with the Gerbicz-Li product d(t) every GL bits
+        T.set(1);
+        T.copy(1, 0);
+        for (int i = p2size - 1; i >= 0; --i) {
+            T.squareDup(mpz_tstbit(p2, (mp_bitcnt_t)i) != 0);
+            if (i % GLb == 0 && i / GLb != 0) {
+                T.copy(3, 0);
+                T.mul(1);
+                T.copy(1, 0);
+                T.copy(0, 3);
+            }
+        }
+        T.copy(3, 0);
+        T.mul(2);  // v2' 2^p2
+        T.getInt(_g);
+        ckey = _g.gethash64();
+        T.copy(0, 3);
+        T.mul(1);  // d(t + 1) = d(t) 2^p2
+        T.copy(2, 0);
+        T.copy(0, 1);
+        for (int i = GLb - 1; i >= 0; --i) T.squareDup(false);
+        T.copy(1, 0);
+        mpz_t res;
+        mpz_init(res);
+        gl_chunk_sum(res, p2, (unsigned long)GLb);
+        mpz_clear(p2);
+        T.set(1);
+        for (int i = (int)mpz_sizeinbase(res, 2) - 1; i >= 0; --i) T.squareDup(mpz_tstbit(res, (mp_bitcnt_t)i) != 0);
+        mpz_clear(res);
+        T.mul(1);
+        T.getInt(_g);
+        h1 = _g.gethash64();
+        T.copy(0, 2);
+        T.getInt(_g);
+        h2 = _g.gethash64();
+        return h1 == h2;
+    }
+
+    // A residue record of the reference's files (gint::write, src/gint.h:160-167): uint32 size,
+    // uint32 base, size int32 canonical digits.
+    static bool write_record(file &f, const int32_t *canon, size_t size, uint32_t base) {
+        const uint32_t sz = (uint32_t)size;
+        return f.write(reinterpret_cast<const char *>(&sz), sizeof(sz)) &&
+               f.write(reinterpret_cast<const char *>(&base), sizeof(base)) &&
+               f.write(reinterpret_cast<const char *>(canon), size * sizeof(int32_t));
+    }
+
 private:
     std::unique_ptr<transformHIP> _t;
 
@@ -310,6 +517,25 @@ private:
     gint _g;
     mpz_t _E;
     std::map<size_t, std::vector<int32_t>> _ckpts;
+
+    // mpz_t array with its own clean-up
+    struct mpz_vec {
+        std::vector<__mpz_struct> v;
+        explicit mpz_vec(size_t n) : v(n) { for (auto &x : v) mpz_init(&x); }
+        ~mpz_vec() { for (auto &x : v) mpz_clear(&x); }
+        mpz_ptr operator[](size_t i) { return &v[i]; }
+    };
+
+    // One residue record into the gint (gint::read, src/gint.h:151-158: size and base must be the
+    // candidate's).
+    void read_record(file &f, const std::string &path) {
+        uint32_t size = 0, base = 0;
+        if (!f.read(reinterpret_cast<char *>(&size), sizeof(size)) ||
+            !f.read(reinterpret_cast<char *>(&base), sizeof(base)) || size != _g.getSize() || base != _g.getBase() ||
+            !f.read(reinterpret_cast<char *>(_g.data()), _g.getSize() * sizeof(int32_t)))
+            throw std::runtime_error("bad residue record in " + path);
+        _g.reset();
+    }
 
     // Checkpoint i into r0 (registers) or into the gint (host copies).
     void load_ckpt(size_t i, bool in_regs) {

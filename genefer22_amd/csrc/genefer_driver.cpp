@@ -190,13 +190,8 @@ int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_ha
     }
 }
 
-// A residue record of the reference's files (gint::write, src/gint.h:160-167): uint32 size, uint32
-// base, size int32 canonical digits.
 static bool write_gint_record(gfe::file &f, const std::vector<int32_t> &canon, uint32_t b) {
-    const uint32_t size = (uint32_t)canon.size();
-    return f.write(reinterpret_cast<const char *>(&size), sizeof(size)) &&
-           f.write(reinterpret_cast<const char *>(&b), sizeof(b)) &&
-           f.write(reinterpret_cast<const char *>(canon.data()), canon.size() * sizeof(int32_t));
+    return gfe::genefer::write_record(f, canon.data(), canon.size(), b);
 }
 
 int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int flags,
@@ -248,6 +243,53 @@ int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *m
         return GFE_OK;
     } catch (const std::exception &ex) {
         return gfe::set_error(GFE_EHIP, ex.what());
+    }
+}
+
+int gfe_run_server(uint32_t b, uint32_t n, int device, const char *main_path, uint32_t rnd1, uint32_t rnd2,
+                   uint32_t rnd3, uint64_t *ckey, gfe_result *out) {
+    if (!out || !ckey || !main_path || !*main_path) return gfe::set_error(GFE_EINVAL, "server arguments");
+    if (rnd1 < 2 || rnd1 >= 64 || rnd2 < 16 || rnd2 >= 256 || rnd3 < 4 || rnd3 >= 65536)
+        return gfe::set_error(GFE_EINVAL, "server random values out of range (rnd1 in [2, 64), rnd2 in [16, 256), rnd3 in [4, 65536))");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
+    memset(out, 0, sizeof(*out));
+    try {
+        const double t0 = now_s();
+        gfe::genefer G(b, n, device, 4);
+        const std::string main(main_path);
+        bool one = false;
+        uint64_t r = 0, o = 0, pk = 0, ck = 0;
+        G.server(main + ".proof", main + ".cert", rnd1, rnd2, rnd3, one, r, o, pk, ck);
+        out->res64 = r;
+        out->old64 = o;
+        out->is_prp = one ? 1 : 0;
+        out->pkey = pk;
+        out->esize = G.esize();
+        out->gl_ok = -1;
+        *ckey = ck;
+        out->seconds = now_s() - t0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EINVAL, ex.what());
+    }
+}
+
+int gfe_run_check(uint32_t b, uint32_t n, int device, const char *main_path, uint64_t *ckey, gfe_result *out) {
+    if (!out || !ckey || !main_path || !*main_path) return gfe::set_error(GFE_EINVAL, "check arguments");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
+    memset(out, 0, sizeof(*out));
+    try {
+        const double t0 = now_s();
+        gfe::genefer G(b, n, device, 4);
+        uint64_t ck = 0;
+        const bool ok = G.check(std::string(main_path) + ".cert", ck);
+        out->gl_ok = ok ? 1 : 0;
+        out->esize = G.esize();
+        *ckey = ok ? ck : 0;
+        out->seconds = now_s() - t0;
+        return GFE_OK;
+    } catch (const std::exception &ex) {
+        return gfe::set_error(GFE_EINVAL, ex.what());
     }
 }
 

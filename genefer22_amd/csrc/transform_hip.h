@@ -37,6 +37,8 @@ public:
         return ok;
     }
     uint32_t crc32() const { return _crc; }
+    // The stream itself, for records written outside the running CRC (GMP's raw integers).
+    std::FILE *handle() const { return _f; }
     bool read(char *p, size_t n) {
         if (!_f) return false;
         const size_t r = std::fread(p, 1, n, _f);

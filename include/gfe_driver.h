@@ -57,6 +57,25 @@ int gfe_run_proof(uint32_t b, uint32_t n, int device, int depth, uint64_t *mu_ha
 int gfe_write_proof(uint32_t b, uint32_t n, int device, int depth, const char *main_path, int flags,
                     uint64_t *mu_hash64, gfe_result *out);
 
+/* The server role of the proof protocol (server(), src/genefer.h:771-880): reads
+ * `<main_path>.proof`, recomputes the proof key from its records (out->pkey; out->res64 / old64 /
+ * is_prp are mu_0's, the PRP residue) and writes the certificate `<main_path>.cert` (int version 1,
+ * int B = B_PietrzakLi(esize, depth), the residue record of v2^rnd2, p2 rnd2 + rnd3 in GMP's raw
+ * integer format outside the CRC, the CRC-32 trailer).  *ckey receives the check key that a check
+ * of this certificate must reproduce.  The reference draws rnd1 in [2, 64), rnd2 in [16, 256) and
+ * rnd3 in [4, 65536) from std::rand seeded with the time; here the caller passes them (outside
+ * those ranges: GFE_EINVAL), so a certificate can be reproduced.  A missing, malformed or corrupted
+ * proof file is GFE_EINVAL. */
+int gfe_run_server(uint32_t b, uint32_t n, int device, const char *main_path, uint32_t rnd1, uint32_t rnd2,
+                   uint32_t rnd3, uint64_t *ckey, gfe_result *out);
+
+/* The check role (check(), src/genefer.h:882-1062, without its context file): reads the certificate
+ * `<main_path>.cert`, raises its residue to 2^B under the Gerbicz test and computes 2^p2 under the
+ * Gerbicz-Li test.  out->gl_ok = 1 when both tests pass (*ckey then holds the check key, equal to
+ * the server's for a correct proof), 0 when one fails (*ckey = 0).  A missing or malformed
+ * certificate is GFE_EINVAL. */
+int gfe_run_check(uint32_t b, uint32_t n, int device, const char *main_path, uint64_t *ckey, gfe_result *out);
+
 /* The PRP loop with the reference's context file at ctx_path (src/genefer.h:198-276, 479-552):
  * resumes from it (or from ctx_path.old) when it holds a PRP-loop context of this engine kind,
  * else starts from r0 = 1.  Writes a context every `save_every` squarings of this call (0: none)

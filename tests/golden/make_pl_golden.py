@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 import oracle as O  # noqa: E402
 
-B, N_LOG, DEPTHS = 1026, 12, (3, 5)
+B, N_LOG, DEPTHS = 1026, 12, (3, 5, 7)
 
 
 def proof(ck: dict, depth: int, n: int, b: int, F: int):
