@@ -31,11 +31,12 @@ U64P = ctypes.POINTER(ctypes.c_uint64)
 
 
 def build(quiet: bool = True) -> None:
-    """Compile liboracle.so (and _ref/libgfref.so when the reference checkout is present)."""
+    """Compile liboracle.so (and, when the reference checkout is present, _ref/libgfref.so and the
+    reference's own program _ref/genefer)."""
     out = subprocess.DEVNULL if quiet else None
     subprocess.run(["make", "-C", _HERE, "liboracle.so"], check=True, stdout=out)
     if os.path.isdir("/root/reference/src"):
-        subprocess.run(["make", "-C", _HERE, "-j8", "ref"], check=True, stdout=out)
+        subprocess.run(["make", "-C", _HERE, "-j8", "ref", "cli"], check=True, stdout=out)
 
 
 def lib():
