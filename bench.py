@@ -96,15 +96,19 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
     kreps = 200
     _, km = t.time_squarings(kreps, dup_pattern, per_kernel=True)
     fused = bool(t.get_plan()[5] & 2)
+    limb = bool(t.get_plan()[5] & 32)
     names = ["col_fwd_all", "row", "col_inv_norm", "fold"] if fused else ["col_fwd", "row", "col_inv", "norm"]
     rns = 3 if "NTT3" in plan else 2
     N = 1 << n
+    # the transform: rns primes x N residues, or for the two-digit-limb transform 4 primes x N/2 limbs
+    tn, tp = (n - 1, 4) if limb else (n, rns)
+    words = tp << tn
     dom = max(range(4), key=lambda i: km[i])
     if fused:  # digits in, residues out | row | residues in, digits out | segment carries
         ng = t.get_plan()[2]
         alg = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 4 * rns * N + 4 * N + 8 * N // ng, 3: 16 * N // ng}
     else:
-        alg = {0: 4 * N + 4 * rns * N, 1: 8 * rns * N, 2: 8 * rns * N, 3: 4 * rns * N + 4 * N}
+        alg = {0: 4 * N + 4 * words, 1: 8 * words, 2: 8 * words, 3: 4 * words + 4 * N}
     alg_bytes = alg[dom]
     achieved = alg_bytes / (km[dom] * 1e-3) / 1e9
     # achieved algorithmic HBM rate of every pass (one read + one write of what the pass must move)
@@ -117,13 +121,13 @@ def bench_n22(G, steps: int, warmup: int, plan_arg: str | None, tune: bool):
                           "frac": round(sq_bytes / (ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     # butterflies of the dominant pass (row: (n - a) levels fwd + inv, N/2 per level per prime)
     a = t.get_plan()[0]
-    levels = {0: a, 1: 2 * (n - a), 2: a}.get(dom, 0)
-    bfly = rns * levels * (N // 2)
+    levels = {0: a, 1: 2 * (tn - a), 2: a}.get(dom, 0)
+    bfly = tp * levels * ((1 << tn) // 2)
     bfly_rate = bfly / (km[dom] * 1e-3) if levels else 0.0
     meas = measured(names[dom], t.get_plan())
     ipb = meas.get("valu_instr_per_butterfly")
     res = {
-        "b": b, "plan": plan, "plan_tuple": list(t.get_plan()), "ms_per_step": ms / steps, "ns_per_squaring": ms / steps * 1e6,
+        "b": b, "plan": plan, "plan_tuple": list(t.get_plan()), "limb": limb, "ms_per_step": ms / steps, "ns_per_squaring": ms / steps * 1e6,
         "kernel_us": {names[i]: round(km[i] * 1e3, 2) for i in range(4)},
         "passes": passes,
         "roofline": {
@@ -460,7 +464,8 @@ def main():
         "dtype": "u32",
         "data": "synthetic (seeded b; residue 2^k mod b^N+1 from set(1) and a fixed dup pattern)",
         "config": {
-            "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] (NTT3, 31-bit primes)",
+            "workload": "one candidate b^(2^22)+1, N=4194304, b seeded in [1e7,3.5e7] ("
+                        + ("2^21 limbs of two digits, four 31-bit primes" if res["limb"] else "NTT3, 31-bit primes") + ")",
             "n": 22, "b": res["b"], "plan": res["plan"], "plan_tuple": res["plan_tuple"], "kernel_us": res["kernel_us"],
             "job_n22": job, "batched_n15": batched, "serving": serving, "serving_light": serving_light, "parity": parity,
             "small_configs": small,

@@ -35,15 +35,26 @@ class Result(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+MAX_PROOF_DEPTH = 10  # GFE_BATCH_MAX_DEPTH
+
+
+def _proof_dict(d: dict) -> dict:
+    """Turns the mu_hash64 array of a result into the list of its depth + 1 hashes."""
+    d["mu_hash64"] = list(d["mu_hash64"])[:d["depth"] + 1] if d["depth"] else []
+    d.pop("pad", None)
+    return d
+
+
 class ServeResult(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_uint64), ("n", ctypes.c_uint32), ("b", ctypes.c_uint32),
                 ("res64", ctypes.c_uint64), ("old64", ctypes.c_uint64), ("hash64", ctypes.c_uint64),
                 ("esize", ctypes.c_uint64), ("iters", ctypes.c_uint64), ("is_prp", ctypes.c_int32),
                 ("gl_ok", ctypes.c_int32), ("t_arrival", ctypes.c_double), ("t_admit", ctypes.c_double),
-                ("t_done", ctypes.c_double)]
+                ("t_done", ctypes.c_double), ("depth", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("pkey", ctypes.c_uint64), ("mu_hash64", ctypes.c_uint64 * (MAX_PROOF_DEPTH + 1))]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return _proof_dict({k: getattr(self, k) for k, _ in self._fields_})
 
 
 class ServerStats(ctypes.Structure):
@@ -86,6 +97,7 @@ ABI = [
     ("gfe_run_prp_trace", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, U64P, U64P, ctypes.POINTER(Result)]),
     ("gfe_default_plan", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, I32P]),
     ("gfe_plan_admissible", ctypes.c_int, [ctypes.c_uint32, I32P]),
+    ("gfe_engine_plan", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, I32P]),
     ("gfe_candidate_plans", ctypes.c_int, [ctypes.c_uint32, I32P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("gfe_host_twiddles", ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U32P]),
     ("gfe_run_prp_resumable", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
@@ -100,6 +112,13 @@ ABI = [
                                      ctypes.POINTER(Result)]),
     ("gfe_batch_create", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("gfe_batch_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("gfe_batch_create_mixed", ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("gfe_batch_submit_n", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_int32)]),
+    ("gfe_batch_create_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_void_p)]),
+    ("gfe_batch_proof_slot_bytes", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32]),
+    ("gfe_batch_proof_device_bytes", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
     ("gfe_batch_slot_bytes", ctypes.c_size_t, [ctypes.c_uint32]),
     ("gfe_batch_tune", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("gfe_batch_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]),
@@ -122,10 +141,12 @@ ABI = [
     ("gfe_server_clock", ctypes.c_double, [ctypes.c_void_p]),
     ("gfe_server_prepare", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("gfe_server_submit", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]),
+    ("gfe_server_submit_proof", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                               ctypes.c_uint32, ctypes.c_uint64]),
     ("gfe_server_pump", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     ("gfe_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("gfe_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
-    ("gfe_serve_replay", ctypes.c_int, [U32P, U32P, U64P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_int,
+    ("gfe_serve_replay", ctypes.c_int, [U32P, U32P, U64P, U32P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_int,
                                         ctypes.c_uint64, U32P, ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
 ]
 
@@ -134,10 +155,12 @@ class BatchResult(ctypes.Structure):
     _fields_ = [("tag", ctypes.c_uint64), ("b", ctypes.c_uint32), ("n", ctypes.c_uint32),
                 ("res64", ctypes.c_uint64), ("old64", ctypes.c_uint64), ("hash64", ctypes.c_uint64),
                 ("esize", ctypes.c_uint64), ("iters", ctypes.c_uint64), ("is_prp", ctypes.c_int32),
-                ("gl_ok", ctypes.c_int32), ("t_submit", ctypes.c_double), ("t_done", ctypes.c_double)]
+                ("gl_ok", ctypes.c_int32), ("t_submit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("depth", ctypes.c_int32), ("pad", ctypes.c_int32), ("pkey", ctypes.c_uint64),
+                ("mu_hash64", ctypes.c_uint64 * (MAX_PROOF_DEPTH + 1))]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return _proof_dict({k: getattr(self, k) for k, _ in self._fields_})
 
 
 def lib():
@@ -315,6 +338,13 @@ def plan_admissible(n: int, plan) -> bool:
     return lib().gfe_plan_admissible(n, _i32(p)) == 0
 
 
+def engine_plan(b: int, n: int) -> tuple:
+    """The plan an engine of (b, n) starts with (no device): the limb transform's where it applies."""
+    p = np.zeros(6, dtype=np.int32)
+    _ck(lib().gfe_engine_plan(b, n, _i32(p)))
+    return tuple(int(x) for x in p)
+
+
 def candidate_plans(n: int) -> list[tuple]:
     """The plans the autotuner times at n (no device)."""
     cnt = ctypes.c_uint32(0)
@@ -426,12 +456,19 @@ def read_proof_file(path: str) -> dict:
 
 
 class Batch:
-    """Many candidates of one n squared in shared launches (include/gfe_batch.h)."""
+    """Many candidates of one n squared in shared launches (include/gfe_batch.h); with depth > 0 a
+    proof batch (every submission: full test and Pietrzak-Li proof, checkpoints in HBM)."""
 
-    def __init__(self, n: int, capacity: int, device: int = 0):
-        self.n, self.capacity = n, capacity
+    def __init__(self, n: int, capacity: int, device: int = 0, depth: int = 0, mixed: bool = False):
+        """mixed: candidates of n = 12 and 13 in one batch (n must then be 13), one launch per block."""
+        self.n, self.capacity, self.depth, self.mixed = n, capacity, depth, mixed
         h = ctypes.c_void_p()
-        _ck(lib().gfe_batch_create(n, capacity, device, ctypes.byref(h)))
+        if mixed:
+            if n != 13 or depth:
+                raise GfeError("a mixed batch holds quick tests of n = 12 and 13 (n = 13, depth 0)")
+            _ck(lib().gfe_batch_create_mixed(capacity, device, ctypes.byref(h)))
+        else:
+            _ck(lib().gfe_batch_create_proof(n, capacity, depth, device, ctypes.byref(h)))
         self._h = h
 
     def close(self):
@@ -471,10 +508,11 @@ class Batch:
         """Squaring blocks as captured HIP graphs (default) or launched kernel by kernel."""
         _ck(lib().gfe_batch_set_graphs(self._h, 1 if on else 0))
 
-    def submit(self, b: int, iters: int = 0, tag: int = 0) -> int | None:
-        """Slot of the new candidate, or None when the batch is full."""
+    def submit(self, b: int, iters: int = 0, tag: int = 0, n: int | None = None) -> int | None:
+        """Slot of the new candidate (of n, a mixed batch's 12 or 13; default the batch's n), or None
+        when the batch is full."""
         slot = ctypes.c_int32(-1)
-        rc = lib().gfe_batch_submit(self._h, b, iters, tag, ctypes.byref(slot))
+        rc = lib().gfe_batch_submit_n(self._h, self.n if n is None else n, b, iters, tag, ctypes.byref(slot))
         if rc == -6:
             return None
         _ck(rc)
@@ -499,8 +537,8 @@ class Batch:
         _ck(min(r, 0))
         return r
 
-    def get_digits(self, slot: int, reg: int = 0) -> np.ndarray:
-        d = np.zeros(1 << self.n, dtype=np.int32)
+    def get_digits(self, slot: int, reg: int = 0, n: int | None = None) -> np.ndarray:
+        d = np.zeros(1 << (self.n if n is None else n), dtype=np.int32)
         _ck(lib().gfe_batch_get_digits(self._h, slot, reg, _i32(d)))
         return d
 
@@ -544,8 +582,9 @@ class Server:
     def prepare(self, n: int, capacity: int = 0):
         _ck(lib().gfe_server_prepare(self._h, n, capacity))
 
-    def submit(self, n: int, b: int, iters: int = 0, tag: int = 0):
-        _ck(lib().gfe_server_submit(self._h, n, b, iters, tag))
+    def submit(self, n: int, b: int, iters: int = 0, tag: int = 0, depth: int = 0):
+        """A quick test (or its first `iters` squarings), or with depth > 0 a proof request."""
+        _ck(lib().gfe_server_submit_proof(self._h, n, b, iters, depth, tag))
 
     def pump(self) -> int:
         busy = ctypes.c_int32(0)
@@ -565,17 +604,19 @@ class Server:
 
 
 def serve_replay(requests, device: int = 0, hbm_budget: int = 0, prepare=()) -> tuple[list[dict], dict]:
-    """Open-loop replay of (n, b, iters, arrival_seconds) requests through the serving loop.
-    Returns (results in request order, server statistics)."""
+    """Open-loop replay of (n, b, iters, arrival_seconds[, proof depth]) requests through the
+    serving loop.  Returns (results in request order, server statistics)."""
     k = len(requests)
     n = np.ascontiguousarray([r[0] for r in requests], dtype=np.uint32)
     b = np.ascontiguousarray([r[1] for r in requests], dtype=np.uint32)
     it = np.ascontiguousarray([r[2] for r in requests], dtype=np.uint64)
     t = np.ascontiguousarray([r[3] for r in requests], dtype=np.float64)
+    dp = np.ascontiguousarray([r[4] if len(r) > 4 else 0 for r in requests], dtype=np.uint32)
     pn = np.ascontiguousarray(list(prepare) or [0], dtype=np.uint32)
     res = (ServeResult * k)()
     st = ServerStats()
     _ck(lib().gfe_serve_replay(n.ctypes.data_as(U32P), b.ctypes.data_as(U32P), it.ctypes.data_as(U64P),
-                               t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k, device, hbm_budget,
+                               dp.ctypes.data_as(U32P), t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k, device,
+                               hbm_budget,
                                pn.ctypes.data_as(U32P), len(prepare), res, ctypes.byref(st)))
     return [res[i].as_dict() for i in range(k)], st.as_dict()

@@ -1,7 +1,8 @@
 // gfe_batch.hip -- batched squaring engine: many candidates of one n per launch (include/gfe_batch.h).
 //
 // Device state of a batch of `cap` slots (per slot s, strides fixed by n):
-//   regs  : cap x 3 x N int32 digits (r0, r1 = Gerbicz-Li d(t), r2 = scratch)
+//   regs  : cap x R x N int32 digits (r0, r1 = Gerbicz-Li d(t), r2 = scratch; R = 3, or
+//           3 + 2^depth in a proof batch: r3.. = the Pietrzak-Li checkpoints)
 //   ws,zp : cap x 3 x N uint32 residues (transform work space, multiplicand)
 //   carry : cap x N/8 int64 workgroup carries (or segment carries of fused plans)
 //   crt   : cap CrtConst, slots: cap SlotInfo (b, prime count, prime width)
@@ -43,10 +44,10 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_fwd(BArgs A, in
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * N;
-    uint32_t *dst = (to_zp ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    uint32_t *dst = (to_zp ? A.zp : A.ws) + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
     if (si.is32) col_fwd_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
     else col_fwd_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
 }
@@ -59,9 +60,9 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_inv(BArgs A) {
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *dst = A.ws + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
+    uint32_t *dst = A.ws + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
     if (si.is32) col_inv_body<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
     else col_inv_body<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
 }
@@ -74,12 +75,13 @@ __global__ void __launch_bounds__(1024) kb_row(BArgs A) {
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *data = ((MODE == ROW_FWD) ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
-    const uint32_t *zp = A.zp + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
-    if (si.is32) row_body<ModU, LOGE, MODE, TWF>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
-    else row_body<ModS, LOGE, MODE, TWF>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds);
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
+    uint32_t *data = ((MODE == ROW_FWD) ? A.zp : A.ws) + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *zp = A.zp + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
+    const bool mul = MODE == ROW_SQUARE && A.mulf && A.mulf[s];
+    if (si.is32) row_body<ModU, LOGE, MODE, TWF>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds, mul);
+    else row_body<ModS, LOGE, MODE, TWF>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.n, A.a, A.ng, blockIdx.x, lds, mul);
 }
 
 // Pipelined plans: persistent column and normalisation kernels over the blocks of each
@@ -92,10 +94,10 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_fwd_p(BArgs A, 
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * N;
-    uint32_t *dst = (to_zp ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    uint32_t *dst = (to_zp ? A.zp : A.ws) + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
     if (si.is32) col_fwd_loop<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
     else col_fwd_loop<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, src, dst, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
 }
@@ -108,9 +110,9 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kb_col_inv_p(BArgs A) 
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *data = A.ws + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
+    uint32_t *data = A.ws + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
     if (si.is32) col_inv_loop<ModU, LOGE>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
     else col_inv_loop<ModS, LOGE>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, tw, A.n, A.a, A.ng, blockIdx.x, gridDim.x, lds);
 }
@@ -119,12 +121,12 @@ __global__ void __launch_bounds__(256) kb_norm_p(BArgs A) {
     const int s = bslot(A);
     if (s < 0) return;
     const SlotInfo si = A.slots[s];
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const CrtConst &c = A.crt[s];
     const bool dup = A.dup ? (A.dup[s] != 0) : false;
-    const uint32_t *ws = A.ws + (size_t)s * 3 * N;
+    const uint32_t *ws = A.ws + (size_t)s * 3 * NS;
     int32_t *dst = A.regs + s * A.reg_stride;
-    int64_t *wc = A.carry + (size_t)s * (N >> 3);
+    int64_t *wc = A.carry + (size_t)s * (NS >> 3);
     if (si.rns == 3) {
         if (si.is32) norm_loop<3, ModU, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x, gridDim.x);
         else norm_loop<3, ModS, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x, gridDim.x);
@@ -142,7 +144,7 @@ __global__ void __launch_bounds__(512) kb_col_fwd_all(BArgs A, int to_zp) {
     if (s < 0) return;
     const SlotInfo si = A.slots[s];
     const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * ((size_t)1 << A.n);
-    uint32_t *dst = (to_zp ? A.zp : A.ws) + (size_t)s * 3 * ((size_t)1 << A.n);
+    uint32_t *dst = (to_zp ? A.zp : A.ws) + (size_t)s * 3 * ((size_t)1 << A.sn);
     const uint32_t *tw = A.tw + (size_t)si.is32 * 3 * ((size_t)1 << A.n);
     if (si.rns == 3) {
         if (si.is32) col_fwd_all_body<3, ModU, LOGE>(A.prU, src, dst, tw, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
@@ -159,13 +161,13 @@ __global__ void __launch_bounds__(512) kb_col_inv_norm(BArgs A) {
     const int s = bslot(A);
     if (s < 0) return;
     const SlotInfo si = A.slots[s];
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const CrtConst &c = A.crt[s];
     const bool dup = A.dup ? (A.dup[s] != 0) : false;
-    const uint32_t *ws = A.ws + (size_t)s * 3 * N;
+    const uint32_t *ws = A.ws + (size_t)s * 3 * NS;
     const uint32_t *tw = A.tw + (size_t)si.is32 * 3 * N;
     int32_t *dst = A.regs + s * A.reg_stride;
-    int64_t *seg = A.carry + (size_t)s * (N >> 3);
+    int64_t *seg = A.carry + (size_t)s * (NS >> 3);
     if (si.rns == 3) {
         if (si.is32) col_inv_norm_body<3, ModU, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
         else col_inv_norm_body<3, ModS, LOGE>(c, ws, tw, dst, seg, si.b, dup, si.rounds, A.n, A.a, A.ng, xcd_block(blockIdx.x, gridDim.x), lds);
@@ -178,22 +180,22 @@ __global__ void __launch_bounds__(512) kb_col_inv_norm(BArgs A) {
 __global__ void __launch_bounds__(256) kb_fold(BArgs A) {
     const int s = bslot(A);
     if (s < 0) return;
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const size_t nseg = N / (size_t)A.ng;
     const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nseg) fold_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * (N >> 3), nseg, A.ng, A.slots[s].b, j);
+    if (j < nseg) fold_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * (NS >> 3), nseg, A.ng, A.slots[s].b, j);
 }
 
 __global__ void __launch_bounds__(256) kb_norm(BArgs A) {
     const int s = bslot(A);
     if (s < 0) return;
     const SlotInfo si = A.slots[s];
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const CrtConst &c = A.crt[s];
     const bool dup = A.dup ? (A.dup[s] != 0) : false;
-    const uint32_t *ws = A.ws + (size_t)s * 3 * N;
+    const uint32_t *ws = A.ws + (size_t)s * 3 * NS;
     int32_t *dst = A.regs + s * A.reg_stride;
-    int64_t *wc = A.carry + (size_t)s * (N >> 3);
+    int64_t *wc = A.carry + (size_t)s * (NS >> 3);
     if (si.rns == 3) {
         if (si.is32) norm_body<3, ModU, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
         else norm_body<3, ModS, 8>(c, ws, dst, wc, si.b, dup, A.n, blockIdx.x);
@@ -206,10 +208,10 @@ __global__ void __launch_bounds__(256) kb_norm(BArgs A) {
 __global__ void __launch_bounds__(256) kb_carry_wg(BArgs A) {
     const int s = bslot(A);
     if (s < 0) return;
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const unsigned nwg = (unsigned)(N >> 11);
     const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w < nwg) carry_wg_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * (N >> 3), nwg, A.slots[s].b, w);
+    if (w < nwg) carry_wg_body(A.regs + s * A.reg_stride, A.carry + (size_t)s * (NS >> 3), nwg, A.slots[s].b, w);
 }
 
 // r_dst = r_src of each active slot.
@@ -228,7 +230,7 @@ __global__ void __launch_bounds__(256) kb_copy(BArgs A) {
 __global__ void __launch_bounds__(256) kb_set(BArgs A) {
     const int s = bslot(A);
     if (s < 0) return;
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < N) A.regs[s * A.reg_stride + A.dst_reg * N + i] = (i == 0) ? A.value : 0;
 }
@@ -240,8 +242,24 @@ static double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-enum ItemKind { IT_RUN = 0, IT_COPY, IT_SET, IT_INITMUL, IT_MUL, IT_READ_RES, IT_READ_H1, IT_READ_H2 };
-enum RunKind { RUN_PRP = 0, RUN_ZERO = 1, RUN_RES = 2 };
+enum ItemKind { IT_RUN = 0, IT_COPY, IT_SET, IT_INITMUL, IT_MUL, IT_READ_RES, IT_READ_H1, IT_READ_H2, IT_READ_MU,
+                IT_READ_PKEY };
+// RUN_POW: rounds of a power r0 = r0^e by square-and-multiply (r0 = 1 and rm = the base before it):
+// round kinds from Slot::powseq, 0 = square, 1 = multiply by rm.
+enum RunKind { RUN_PRP = 0, RUN_ZERO = 1, RUN_RES = 2, RUN_POW = 3 };
+
+// Non-negative big integers as little-endian 32-bit words (the Pietrzak-Li exponents w_j).
+typedef std::vector<uint32_t> Words;
+static void words_mul(Words &x, uint32_t q) {
+    uint64_t c = 0;
+    for (auto &v : x) {
+        const uint64_t t = (uint64_t)v * q + c;
+        v = (uint32_t)t;
+        c = t >> 32;
+    }
+    if (c) x.push_back((uint32_t)c);
+    while (!x.empty() && x.back() == 0) x.pop_back();
+}
 
 struct Item {
     int kind;
@@ -253,11 +271,20 @@ struct Item {
 struct Slot {
     bool busy = false, done = false;
     uint32_t b = 0;
+    uint32_t n = 0;                   // the candidate's n
     int rns = 0, is32 = 0;
     uint64_t tag = 0, esize = 0, iters = 0;
     double t_submit = 0, t_done = 0;
     std::vector<uint64_t> ebits;      // E = b^(2^n), little-endian 64-bit words
     std::vector<uint8_t> resbits;     // Gerbicz-Li exponent `res`, most significant bit first
+    std::vector<uint8_t> powseq;      // round kinds of the queued power runs (RUN_POW)
+    // Pietrzak-Li proof (proof batches): its items are queued phase by phase, each phase after the
+    // read-back of mu_(k-1) gives the exponent q = hash32(mu_(k-1)) (src/genefer.h:620-734).
+    bool pl_pending = false;          // proof phases still to queue
+    int pl_k = 0;                     // next mu_k to read
+    std::vector<Words> w;             // exponents w_j, j < 2^depth / 2
+    std::vector<uint64_t> mu_hash;    // gethash64 of mu_0, mu_1, ...
+    uint64_t pkey = 0;                // proof key: gethash64 of v1 (after the last phase)
     std::deque<Item> q;
     uint64_t pos = 0;                 // progress inside the head run
     uint64_t res64 = 0, old64 = 0, hash64 = 0, h1 = 0;
@@ -272,9 +299,16 @@ struct Slot {
 
 struct Batch {
     uint32_t n = 0, cap = 0;
+    uint32_t nregs = 3;   // registers per slot: 3, or 3 + 2^depth in a proof batch
+    int depth = 0;        // Pietrzak-Li depth of every submission (0: quick tests)
     int device = 0;
-    size_t N = 0;
+    size_t N = 0;        // digits of the transform being launched (the slots' n; a mixed batch switches it
+                         // per group of register operations, with_n)
+    size_t NA = 0;       // digits per register in the allocation: 2^nalloc
+    uint32_t nalloc = 0; // the batch's n: per-slot strides
+    bool mixed = false;  // slots of n = 12 and 13 share the batch (squaring blocks: kf_square_mixed)
     Plan plan;
+    Plan op_plan[2];     // mixed batches: the four-step plans of n = 12 and 13 (register operations)
     hipStream_t stream = nullptr;
     int32_t *regs = nullptr;
     uint32_t *ws = nullptr, *zp = nullptr, *tw = nullptr;
@@ -282,9 +316,10 @@ struct Batch {
     CrtConst *crt = nullptr;
     SlotInfo *slots_dev = nullptr;
     uint8_t *act_dev = nullptr, *dup_dev = nullptr;   // [64][cap] each
+    uint8_t *mul_dev = nullptr;                       // [64][cap]: rounds that multiply by rm
     // Pinned ring of flag uploads: a buffer is rewritten only after its last upload has ended.
     static constexpr int NSTAGE = 8;
-    uint8_t *stage[NSTAGE] = {};                      // each 2 x 64 x cap
+    uint8_t *stage[NSTAGE] = {};                      // each 3 x 64 x cap
     hipEvent_t stage_ev[NSTAGE] = {};
     int stage_i = 0;
     // Pinned mirrors of the per-slot tables.  Entry s is written only by submit() into a free
@@ -326,12 +361,15 @@ struct Batch {
     uint64_t plan_key() const;
     hipGraphExec_t block_graph(int rounds);
     void drop_graphs();
-    int run_block(int rounds, uint32_t nact);
+    int run_block(int rounds, uint32_t nact, bool has_mul = false, uint32_t nbig = 0);
     // The single-workgroup squaring at n = 13 pays off once 128 or more slots are active: with
     // fewer, one workgroup per candidate leaves most CUs idle and the four-step passes finish a
     // round sooner (one candidate: 76 ms against 166 ms for 4096 squarings on MI355X).  At n = 12
     // it is the faster path at any occupancy.
-    bool fused_block(uint32_t nact) const { return plan.fuse_sq && (n <= 12 || nact >= 128u); }
+    // Blocks with multiplying rounds (powers) run the four-step passes.
+    bool fused_block(uint32_t nact, bool has_mul = false) const {
+        return plan.fuse_sq && !has_mul && (n <= 12 || nact >= 128u);
+    }
     // Squaring rounds leave the normalisation's workgroup carries in `carry` and the next round's
     // column forward pass adds them (compile-time plans); carry_wg_body clears what it applies, so
     // `carry` always holds carries not yet in register 0, or zeros, and a squaring round may absorb
@@ -351,8 +389,23 @@ struct Batch {
     bool ok = false;
 
     ~Batch();
-    int init(uint32_t n, uint32_t cap, int device);
+    int init(uint32_t n, uint32_t cap, int device, int depth = 0, bool mixed = false);
     BArgs args(const uint8_t *active, const uint8_t *dup) const;
+    size_t slot_N(int s) const { return (size_t)1 << slots[s].n; }
+    int32_t *reg(int s, uint32_t r) const { return regs + (size_t)s * nregs * NA + (size_t)r * slot_N(s); }
+    // Runs f with the transform of n = m (a mixed batch's register operations on its slots of n = m).
+    template <class F> void with_n(uint32_t m, const F &f) {
+        if (!mixed || m == n) { f(); return; }
+        const uint32_t n0 = n;
+        const size_t N0 = N;
+        const Plan p0 = plan;
+        n = m; N = (size_t)1 << m; plan = op_plan[m - 12];
+        f();
+        n = n0; N = N0; plan = p0;
+    }
+    void queue_power(Slot &S, uint32_t src, const Words &e);
+    void queue_pl_phase(Slot &S, int k);
+    void pl_read_done(Slot &S, gint &g);
     size_t lds_col() const { return (size_t)plan.col_ng * ((1u << plan.a) + ((1u << plan.a) >> plan.col_loge)) * 4; }
     size_t lds_row() const {
         const uint32_t G = 1u << (n - plan.a);
@@ -362,14 +415,14 @@ struct Batch {
     void launch_row(const BArgs &A, int mode);
     void launch_col_inv(const BArgs &A);
     void launch_norm_carry(const BArgs &A);
-    void square(const uint8_t *active, const uint8_t *dup);
+    void square(const uint8_t *active, const uint8_t *dup, const uint8_t *mulf = nullptr);
     void op_copy(const uint8_t *active, uint32_t dst, uint32_t src);
     void op_set(const uint8_t *active, uint32_t dst, int32_t value);
     void op_initmul(const uint8_t *active, uint32_t src);
     void op_mul(const uint8_t *active);
     uint8_t *stage_acquire();
     void stage_release();
-    int submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot);
+    int submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot, uint32_t n_slot = 0);
     int step(int32_t *active);
     int read_residue(int s, uint32_t reg, gint &g);
     int read_issue(int s, int kind);
@@ -414,7 +467,8 @@ hipGraphExec_t Batch::block_graph(int rounds) {
         }
     hipGraph_t graph = nullptr;
     if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
-    for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
+    for (int r = 0; r < rounds; ++r)
+        square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap, mul_dev + (size_t)r * cap);
     if (hipStreamEndCapture(stream, &graph) != hipSuccess || !graph) {
         (void)hipGetLastError();
         return nullptr;
@@ -440,23 +494,35 @@ hipGraphExec_t Batch::block_graph(int rounds) {
 // Enqueues `rounds` squaring rounds (flag rows already uploaded): one graph launch, or the
 // kernels one by one when graphs are off or the plan uses persistent grids (their sizes come from
 // occupancy queries, which stay out of captures).
-int Batch::run_block(int rounds, uint32_t nact) {
-    if (fused_block(nact)) {  // one launch runs the whole block, one workgroup per slot
+// A mixed batch runs its slots of n = 12 in the single-workgroup kernel and its slots of n = 13 in
+// the same launch when 128 or more of them are active (nbig), otherwise on the four-step passes of
+// n = 13 after it (fused_block's rule for one n).
+int Batch::run_block(int rounds, uint32_t nact, bool has_mul, uint32_t nbig) {
+    const bool mixed_fused = mixed && (nbig == 0 || nbig >= 128u);
+    if (mixed || fused_block(nact, has_mul)) {  // one launch runs the whole block, one workgroup per slot
         flush_carries();
         FusedArgs A;
         memset(&A, 0, sizeof(A));
         A.regs = regs; A.tw = tw; A.crt = crt; A.slots = slots_dev; A.act = act_dev; A.dupf = dup_dev;
-        A.reg_stride = (long long)(3 * N); A.stride = cap; A.rounds = rounds; A.prS = prS; A.prU = prU;
-        hipLaunchKernelGGL(fused_square_fn((int)n), dim3(zdim()), dim3(fused_threads((int)n)), fused_lds_bytes((int)n),
-                           stream, A);
-        return check("fused square block");
+        A.reg_stride = (long long)(nregs * NA); A.stride = cap; A.rounds = rounds; A.prS = prS; A.prU = prU;
+        A.tws = (uint32_t)NA;
+        if (mixed) {
+            A.only_n = mixed_fused ? 0 : 12;
+            hipLaunchKernelGGL(fused_square_mixed_fn(), dim3(zdim()), dim3(512), fused_lds_bytes(13), stream, A);
+        } else {
+            hipLaunchKernelGGL(fused_square_fn((int)n), dim3(zdim()), dim3(fused_threads((int)n)), fused_lds_bytes((int)n),
+                               stream, A);
+        }
+        if (!mixed || mixed_fused) return check("fused square block");
+        // the slots of n = 13 on the four-step passes (the four-step kernels skip the other slots)
     }
     if (defer_carry()) carries_pending = true;
     if (use_graphs && !plan.pipe) {
         if (hipGraphExec_t g = block_graph(rounds)) return check_hip(hipGraphLaunch(g, stream), "graph launch");
         return set_error(GFE_EHIP, "squaring block graph capture failed");
     }
-    for (int r = 0; r < rounds; ++r) square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap);
+    for (int r = 0; r < rounds; ++r)
+        square(act_dev + (size_t)r * cap, dup_dev + (size_t)r * cap, mul_dev + (size_t)r * cap);
     return check("square block");
 }
 
@@ -467,6 +533,7 @@ Batch::~Batch() {
     drop_graphs();
     (void)hipFree(regs); (void)hipFree(ws); (void)hipFree(zp); (void)hipFree(tw); (void)hipFree(carry);
     (void)hipFree(crt); (void)hipFree(slots_dev); (void)hipFree(act_dev); (void)hipFree(dup_dev);
+    (void)hipFree(mul_dev);
     for (int i = 0; i < NSTAGE; ++i) {
         if (stage[i]) (void)hipHostFree(stage[i]);
         if (stage_ev[i]) (void)hipEventDestroy(stage_ev[i]);
@@ -496,9 +563,14 @@ template <int TWF> void set_row_attrs() {
     set_lds_attr((const void *)kb_row<5, ROW_FWD, TWF>);
 }
 
-int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
+int Batch::init(uint32_t n_, uint32_t cap_, int device_, int depth_, bool mixed_) {
     n = n_; cap = cap_; device = device_;
+    depth = depth_;
+    mixed = mixed_;
+    nregs = depth > 0 ? 3u + (1u << depth) : 3u;
     N = (size_t)1 << n;
+    NA = N;
+    nalloc = n;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_error(GFE_ENODEV, "no HIP device");
     if (device < 0 || device >= ndev) return set_error(GFE_ENODEV, "device index out of range");
@@ -507,18 +579,19 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return set_error(GFE_EHIP, "stream");
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
     plan = default_batch_plan(n, cap);
-    if (hipMalloc(&regs, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&ws, (size_t)cap * 3 * N * 4) != hipSuccess ||
-        hipMalloc(&zp, (size_t)cap * 3 * N * 4) != hipSuccess || hipMalloc(&tw, (size_t)6 * N * 4) != hipSuccess ||
-        hipMalloc(&carry, (size_t)cap * (N >> 3) * 8) != hipSuccess || hipMalloc(&crt, (size_t)cap * sizeof(CrtConst)) != hipSuccess ||
+    if (hipMalloc(&regs, (size_t)cap * nregs * NA * 4) != hipSuccess || hipMalloc(&ws, (size_t)cap * 3 * NA * 4) != hipSuccess ||
+        hipMalloc(&zp, (size_t)cap * 3 * NA * 4) != hipSuccess || hipMalloc(&tw, (size_t)6 * NA * 4) != hipSuccess ||
+        hipMalloc(&carry, (size_t)cap * (NA >> 3) * 8) != hipSuccess || hipMalloc(&crt, (size_t)cap * sizeof(CrtConst)) != hipSuccess ||
         hipMalloc(&slots_dev, (size_t)cap * sizeof(SlotInfo)) != hipSuccess ||
-        hipMalloc(&act_dev, (size_t)64 * cap) != hipSuccess || hipMalloc(&dup_dev, (size_t)64 * cap) != hipSuccess)
+        hipMalloc(&act_dev, (size_t)64 * cap) != hipSuccess || hipMalloc(&dup_dev, (size_t)64 * cap) != hipSuccess ||
+        hipMalloc(&mul_dev, (size_t)64 * cap) != hipSuccess)
         return set_error(GFE_ENOMEM, "device allocation failed");
     for (int i = 0; i < NSTAGE; ++i) {
-        if (hipHostMalloc((void **)&stage[i], (size_t)2 * 64 * cap) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
+        if (hipHostMalloc((void **)&stage[i], (size_t)3 * 64 * cap) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
         if (hipEventCreateWithFlags(&stage_ev[i], hipEventDisableTiming) != hipSuccess) return set_error(GFE_EHIP, "event");
     }
     for (int i = 0; i < NREAD; ++i) {
-        if (hipHostMalloc((void **)&rbuf[i], N * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
+        if (hipHostMalloc((void **)&rbuf[i], NA * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
         if (hipEventCreateWithFlags(&rev[i], hipEventDisableTiming) != hipSuccess) return set_error(GFE_EHIP, "event");
         rfree.push_back(i);
     }
@@ -527,7 +600,7 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     if (hipHostMalloc((void **)&slot_host, (size_t)cap * sizeof(SlotInfo)) != hipSuccess ||
         hipHostMalloc((void **)&crt_host, (size_t)cap * sizeof(CrtConst)) != hipSuccess)
         return set_error(GFE_ENOMEM, "pinned");
-    if (hipHostMalloc((void **)&readback, N * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
+    if (hipHostMalloc((void **)&readback, NA * 4) != hipSuccess) return set_error(GFE_ENOMEM, "pinned");
     if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess) return set_error(GFE_EHIP, "event");
     {
         std::vector<uint32_t> h(6 * N);
@@ -542,11 +615,12 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
         prS.q[k] = c0.q[k]; prU.q[k] = c1.q[k];
     }
     std::vector<SlotInfo> si(cap);
-    for (auto &x : si) { x.b = 1024; x.rns = 0; x.is32 = 0; x.rounds = 1; }
+    for (auto &x : si) { x.b = 1024; x.rns = 0; x.is32 = 0; x.rounds = 1; x.n = (int32_t)n; }
     if (hipMemcpy(slots_dev, si.data(), cap * sizeof(SlotInfo), hipMemcpyHostToDevice) != hipSuccess)
         return set_error(GFE_EHIP, "slot upload");
-    if (hipMemsetAsync(regs, 0, (size_t)cap * 3 * N * 4, stream) != hipSuccess ||
-        hipMemsetAsync(carry, 0, (size_t)cap * (N >> 3) * 8, stream) != hipSuccess ||
+    if (hipMemsetAsync(regs, 0, (size_t)cap * nregs * NA * 4, stream) != hipSuccess ||
+        hipMemsetAsync(mul_dev, 0, (size_t)64 * cap, stream) != hipSuccess ||
+        hipMemsetAsync(carry, 0, (size_t)cap * (NA >> 3) * 8, stream) != hipSuccess ||
         hipMemsetAsync(act_dev, 0, (size_t)64 * cap, stream) != hipSuccess ||
         hipMemsetAsync(dup_dev, 0, (size_t)64 * cap, stream) != hipSuccess)
         return set_error(GFE_EHIP, "memset");
@@ -561,6 +635,11 @@ int Batch::init(uint32_t n_, uint32_t cap_, int device_) {
     set_lds_attr((const void *)kb_col_fwd_all<3>); set_lds_attr((const void *)kb_col_fwd_all<4>);
     set_lds_attr((const void *)kb_col_inv_norm<3>); set_lds_attr((const void *)kb_col_inv_norm<4>);
     slots.resize(cap);
+    for (auto &S : slots) S.n = n;
+    if (mixed) {  // register operations of each n on the four-step passes
+        op_plan[0] = default_plan(12);
+        op_plan[1] = default_plan(13);
+    }
     t0 = now_s();
     return check_hip(hipStreamSynchronize(stream), "init");
 }
@@ -570,7 +649,8 @@ BArgs Batch::args(const uint8_t *active, const uint8_t *dup) const {
     BArgs A;
     memset(&A, 0, sizeof(A));
     A.regs = regs; A.ws = ws; A.zp = zp; A.carry = carry; A.crt = crt; A.slots = slots_dev; A.tw = tw;
-    A.active = active; A.dup = dup; A.prS = prS; A.prU = prU; A.reg_stride = (long long)(3 * N); A.n = (int)n;
+    A.active = active; A.dup = dup; A.prS = prS; A.prU = prU; A.reg_stride = (long long)(nregs * NA); A.n = (int)n;
+    A.sn = (int)nalloc;
     return A;
 }
 
@@ -710,11 +790,12 @@ void Batch::flush_carries() {
 // Clears every slot's carries (after a plan whose passes leave other values in `carry`).
 int Batch::zero_carries() {
     carries_pending = false;
-    return check_hip(hipMemsetAsync(carry, 0, (size_t)cap * (N >> 3) * 8, stream), "carry clear");
+    return check_hip(hipMemsetAsync(carry, 0, (size_t)cap * (NA >> 3) * 8, stream), "carry clear");
 }
 
-void Batch::square(const uint8_t *active, const uint8_t *dup) {
+void Batch::square(const uint8_t *active, const uint8_t *dup, const uint8_t *mulf) {
     BArgs A = args(active, dup);
+    A.mulf = mulf;
     A.src_reg = 0;
     A.absorb = defer_carry() ? 1 : 0;
     launch_col_fwd(A, 0);
@@ -791,7 +872,10 @@ int Batch::backlog() {
 }
 
 
-int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
+int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out, uint32_t n_slot) {
+    if (n_slot == 0) n_slot = nalloc;
+    if (mixed ? (n_slot < 12 || n_slot > nalloc) : n_slot != nalloc)
+        return set_error(GFE_EINVAL, "submit: n does not belong to this batch");
     int s = -1;
     for (uint32_t i = 0; i < cap; ++i)
         if (!slots[i].busy) { s = (int)i; break; }
@@ -800,19 +884,20 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
     S = Slot();
     S.busy = true;
     S.b = b;
+    S.n = n_slot;
     S.tag = tag;
     S.t_submit = now_s() - t0;
-    select_primes(b, n, &S.rns, &S.is32);
+    select_primes(b, S.n, &S.rns, &S.is32);
     S.is32 = transform_is32(S.rns, S.is32);
     // E = b^(2^n) and its bits (src/genefer.h:1476)
     mpz_t E;
     mpz_init(E);
-    mpz_ui_pow_ui(E, b, 1ul << n);
+    mpz_ui_pow_ui(E, b, 1ul << S.n);
     S.esize = mpz_sizeinbase(E, 2);
     S.ebits.assign((S.esize + 63) / 64, 0);
     size_t cnt = 0;
     mpz_export(S.ebits.data(), &cnt, -1, 8, 0, 0, E);
-    const bool full = (iters == 0 || iters >= S.esize);
+    const bool full = (iters == 0 || iters >= S.esize) || depth > 0;  // a proof batch runs full tests
     S.iters = full ? S.esize : iters;
     if (!full) {
         Item r; r.kind = IT_RUN; r.run = RUN_PRP; r.start = 0; r.len = S.iters;
@@ -824,22 +909,30 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
         const uint64_t L = uint64_t(2) << (ilog2_32((uint32_t)S.esize) / 2);
         uint64_t B = (S.esize - 1) / L + 1;
         B = (B + 63) / 64 * 64;
+        // Pietrzak-Li checkpoints (proof batches): r0 after bit i with i % B_PL == 0 goes to
+        // register 3 + i / B_PL, B_PL = B_PietrzakLi(esize, depth) (src/genefer.h:473-476, 536-546)
+        const uint64_t BPL = depth > 0 ? ((S.esize - 1) >> depth) + 1 : 0;
         // PRP loop: squaring k handles bit i = esize - 1 - k; after bit i with i % B == 0 and
-        // i / B != 0, the d(t) update r2 = r0; r0 *= r1; r1 = r0; r0 = r2 (src/genefer.h:529-535)
+        // i / B != 0, the d(t) update r2 = r0; r0 *= r1; r1 = r0; r0 = r2 (src/genefer.h:529-535),
+        // then the checkpoint copy
         uint64_t k = 0;
         while (k < S.esize) {
             const uint64_t i = S.esize - 1 - k;  // first bit of this run
-            const uint64_t last = (i / B) * B;    // last bit of the run: next multiple of B below
+            uint64_t last = (i / B) * B;          // last bit of the run: next GL or PL point below
+            if (BPL) last = std::max(last, (i / BPL) * BPL);
             const uint64_t len = i - last + 1;
             Item r; r.kind = IT_RUN; r.run = RUN_PRP; r.start = k; r.len = len;
             S.q.push_back(r);
             k += len;
-            if (last != 0) {
+            if (last != 0 && last % B == 0) {
                 Item c; c.kind = IT_COPY; c.a = 2; c.b = 0; S.q.push_back(c);
                 Item m; m.kind = IT_INITMUL; m.a = 1; S.q.push_back(m);
                 Item u; u.kind = IT_MUL; S.q.push_back(u);
                 Item c1; c1.kind = IT_COPY; c1.a = 1; c1.b = 0; S.q.push_back(c1);
                 Item c2; c2.kind = IT_COPY; c2.a = 0; c2.b = 2; S.q.push_back(c2);
+            }
+            if (BPL && last % BPL == 0) {
+                Item c; c.kind = IT_COPY; c.a = 3 + (uint32_t)(last / BPL); c.b = 0; S.q.push_back(c);
             }
         }
         Item rd; rd.kind = IT_READ_RES; S.q.push_back(rd);
@@ -863,12 +956,19 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
         { Item h; h.kind = IT_READ_H1; S.q.push_back(h); }
         { Item c; c.kind = IT_COPY; c.a = 0; c.b = 2; S.q.push_back(c); }
         { Item h; h.kind = IT_READ_H2; S.q.push_back(h); }
+        if (depth > 0) {  // the proof: mu_0 = checkpoint 0 (the PRP residue)
+            S.pl_pending = true;
+            S.pl_k = 0;
+            S.w.assign(size_t(1) << (depth - 1), Words());
+            { Item c; c.kind = IT_COPY; c.a = 0; c.b = 3; S.q.push_back(c); }
+            { Item rd; rd.kind = IT_READ_MU; S.q.push_back(rd); }
+        }
     }
     mpz_clear(E);
     // slot table and CRT constants; r0 = 1, r1 = 1 (prp: set(1); copy(1, 0))
     SlotInfo &si = slot_host[s];
-    si.b = b; si.rns = S.rns; si.is32 = S.is32; si.rounds = carry_rounds(b, n);
-    crt_host[s] = make_crt(S.rns, S.is32, n, b);
+    si.b = b; si.rns = S.rns; si.is32 = S.is32; si.rounds = carry_rounds(b, S.n); si.n = (int32_t)S.n;
+    crt_host[s] = make_crt(S.rns, S.is32, S.n, b);
     zl = std::max(zl, (uint32_t)s + 1);
     int rc = check_hip(hipMemcpyAsync(slots_dev + s, &si, sizeof(si), hipMemcpyHostToDevice, stream), "slot");
     if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(crt + s, crt_host + s, sizeof(CrtConst), hipMemcpyHostToDevice, stream), "crt");
@@ -880,20 +980,87 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out) {
     rc = check_hip(hipMemcpyAsync(act_dev, st, cap, hipMemcpyHostToDevice, stream), "flags");
     stage_release();
     if (rc != GFE_OK) return rc;
-    op_set(act_dev, 0, 1);
-    op_set(act_dev, 1, 1);
+    with_n(S.n, [&]() {
+        op_set(act_dev, 0, 1);
+        op_set(act_dev, 1, 1);
+    });
     if (slot_out) *slot_out = s;
     return check("submit");
 }
 
 int Batch::read_residue(int s, uint32_t reg, gint &g) {
-    int rc = check_hip(hipMemcpyAsync(readback, regs + (size_t)s * 3 * N + reg * N, N * 4, hipMemcpyDeviceToHost, stream),
+    int rc = check_hip(hipMemcpyAsync(readback, this->reg(s, reg), slot_N(s) * 4, hipMemcpyDeviceToHost, stream),
                        "readback");
     if (rc == GFE_OK) rc = check_hip(hipStreamSynchronize(stream), "readback sync");
     if (rc != GFE_OK) return rc;
-    memcpy(g.data(), readback, N * 4);
+    memcpy(g.data(), readback, slot_N(s) * 4);
     g.reset();
     return GFE_OK;
+}
+
+// r0 = r_src^e by square-and-multiply (power / powerz, src/genefer.h:441-465): rm = r_src, r0 = 1,
+// then for each bit of e from the top a squaring round and, for a one bit, a multiplying round.
+void Batch::queue_power(Slot &S, uint32_t src, const Words &e) {
+    { Item m; m.kind = IT_INITMUL; m.a = src; S.q.push_back(m); }
+    { Item st; st.kind = IT_SET; st.a = 0; st.b = 1; S.q.push_back(st); }
+    if (e.empty()) return;
+    Item r; r.kind = IT_RUN; r.run = RUN_POW; r.start = S.powseq.size();
+    for (size_t wi = e.size(); wi-- > 0;)
+        for (int bit = 31; bit >= 0; --bit) {
+            if (wi == e.size() - 1 && r.len == 0 && !((e[wi] >> bit) & 1u)) continue;  // leading zeros
+            S.powseq.push_back(0);
+            if ((e[wi] >> bit) & 1u) S.powseq.push_back(1);
+            r.len = S.powseq.size() - r.start;
+        }
+    S.q.push_back(r);
+}
+
+// Phase k >= 1 of the proof up to the read-back of mu_k (src/genefer.h:650-705): with i =
+// 2^(depth - k), r1 = ckpt_i^w_0 prod_j ckpt_(i + 2j)^w_j over j = i, 2i, .. < 2^depth / 2, r0 = r1.
+void Batch::queue_pl_phase(Slot &S, int k) {
+    const size_t L = size_t(1) << depth, i = size_t(1) << (depth - k);
+    { Item c; c.kind = IT_COPY; c.a = 0; c.b = 3 + (uint32_t)i; S.q.push_back(c); }
+    queue_power(S, 0, S.w[0]);
+    { Item c; c.kind = IT_COPY; c.a = 1; c.b = 0; S.q.push_back(c); }
+    for (size_t j = i; j < L / 2; j += i) {
+        { Item c; c.kind = IT_COPY; c.a = 0; c.b = 3 + (uint32_t)(i + 2 * j); S.q.push_back(c); }
+        queue_power(S, 0, S.w[j]);
+        { Item m; m.kind = IT_INITMUL; m.a = 1; S.q.push_back(m); }
+        { Item u; u.kind = IT_MUL; S.q.push_back(u); }
+        { Item c; c.kind = IT_COPY; c.a = 1; c.b = 0; S.q.push_back(c); }
+    }
+    { Item rd; rd.kind = IT_READ_MU; S.q.push_back(rd); }
+}
+
+// mu_k has been read back (r0): record its hash, then queue the rest of phase k -- v1 = v1 mu_k^q
+// with q = hash32(mu_k) (for k = 0: v1 = mu_0^q) and the exponent update w_(i/2 + j) = w_j q --
+// and phase k + 1, or after mu_depth the read-back of v1, the proof key.
+void Batch::pl_read_done(Slot &S, gint &g) {
+    const int k = S.pl_k;
+    S.mu_hash.push_back(g.gethash64());
+    const uint32_t q = g.gethash32();
+    queue_power(S, 0, Words{q});
+    if (k == 0) {
+        S.w[0] = Words{q};
+    } else {
+        { Item m; m.kind = IT_INITMUL; m.a = 2; S.q.push_back(m); }
+        { Item u; u.kind = IT_MUL; S.q.push_back(u); }
+        const size_t L = size_t(1) << depth, i = size_t(1) << (depth - k);
+        if (i > 1)
+            for (size_t j = 0; j < L / 2; j += i) {
+                S.w[i / 2 + j] = S.w[j];
+                words_mul(S.w[i / 2 + j], q);
+            }
+    }
+    { Item c; c.kind = IT_COPY; c.a = 2; c.b = 0; S.q.push_back(c); }
+    S.pl_k = k + 1;
+    if (k < depth) {
+        queue_pl_phase(S, k + 1);
+    } else {
+        { Item c; c.kind = IT_COPY; c.a = 0; c.b = 2; S.q.push_back(c); }
+        { Item rd; rd.kind = IT_READ_PKEY; S.q.push_back(rd); }
+        S.pl_pending = false;
+    }
 }
 
 // Enqueues the read-back of register 0 of slot s for result item `kind`.
@@ -904,7 +1071,7 @@ int Batch::read_issue(int s, int kind) {
     }
     const int k = rfree.back();
     rfree.pop_back();
-    int rc = check_hip(hipMemcpyAsync(rbuf[k], regs + (size_t)s * 3 * N, N * 4, hipMemcpyDeviceToHost, stream), "readback");
+    int rc = check_hip(hipMemcpyAsync(rbuf[k], reg(s, 0), slot_N(s) * 4, hipMemcpyDeviceToHost, stream), "readback");
     if (rc == GFE_OK) rc = check_hip(hipEventRecord(rev[k], stream), "readback event");
     if (rc != GFE_OK) return rc;
     reads.push_back(Read{s, kind, k});
@@ -927,8 +1094,8 @@ int Batch::read_complete(bool wait) {
     }
     reads.pop_front();
     Slot &S = slots[r.slot];
-    gint gs(N, S.b);
-    memcpy(gs.data(), rbuf[r.buf], N * 4);
+    gint gs(slot_N(r.slot), S.b);
+    memcpy(gs.data(), rbuf[r.buf], slot_N(r.slot) * 4);
     gs.reset();
     rfree.push_back(r.buf);
     if (r.kind == IT_READ_RES) {
@@ -938,6 +1105,10 @@ int Batch::read_complete(bool wait) {
         S.hash64 = S.is_prp ? 0 : gs.gethash64();
     } else if (r.kind == IT_READ_H1) {
         S.h1 = gs.gethash64();
+    } else if (r.kind == IT_READ_MU) {
+        pl_read_done(S, gs);
+    } else if (r.kind == IT_READ_PKEY) {
+        S.pkey = gs.gethash64();
     } else {
         S.gl_ok = (gs.gethash64() == S.h1) ? 1 : 0;
     }
@@ -956,18 +1127,23 @@ int Batch::harvest() {
     bool freed = false;
     for (uint32_t s = 0; s < cap; ++s) {
         Slot &S = slots[s];
-        if (S.busy && !S.done && S.q.empty() && S.reads_pending == 0) {
+        if (S.busy && !S.done && S.q.empty() && S.reads_pending == 0 && !S.pl_pending) {
             S.done = true;
             S.t_done = now_s() - t0;
             gfe_batch_result r;
             memset(&r, 0, sizeof(r));
-            r.tag = S.tag; r.b = S.b; r.n = n; r.res64 = S.res64; r.old64 = S.old64; r.hash64 = S.hash64;
+            r.tag = S.tag; r.b = S.b; r.n = S.n; r.res64 = S.res64; r.old64 = S.old64; r.hash64 = S.hash64;
             r.esize = S.esize; r.iters = S.iters; r.is_prp = S.is_prp; r.gl_ok = S.gl_ok;
             r.t_submit = S.t_submit; r.t_done = S.t_done;
+            r.depth = S.mu_hash.empty() ? 0 : depth;
+            r.pkey = S.pkey;
+            for (size_t k = 0; k < S.mu_hash.size() && k < GFE_BATCH_MAX_DEPTH + 1; ++k) r.mu_hash64[k] = S.mu_hash[k];
             finished.push_back(r);
             S.busy = false;
             S.ebits.clear(); S.ebits.shrink_to_fit();
             S.resbits.clear(); S.resbits.shrink_to_fit();
+            S.powseq.clear(); S.powseq.shrink_to_fit();
+            S.w.clear(); S.mu_hash.clear();
             freed = true;
         }
     }
@@ -982,13 +1158,13 @@ int Batch::step(int32_t *active_out) {
     bool issued = false;
     // 1. register operations, grouped by (kind, arguments) across slots
     for (;;) {
-        std::map<std::tuple<int, uint32_t, uint32_t>, std::vector<int>> groups;
+        std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, std::vector<int>> groups;
         for (uint32_t s = 0; s < cap; ++s) {
             Slot &S = slots[s];
             if (!S.busy || S.done || S.q.empty()) continue;
             const Item &it = S.q.front();
             if (it.kind == IT_RUN) continue;
-            groups[std::make_tuple(it.kind, it.a, it.b)].push_back((int)s);
+            groups[std::make_tuple(it.kind, it.a, it.b, S.n)].push_back((int)s);
         }
         if (groups.empty()) break;
         flush_carries();
@@ -996,7 +1172,8 @@ int Batch::step(int32_t *active_out) {
             const int kind = std::get<0>(kv.first);
             const uint32_t a = std::get<1>(kv.first), b2 = std::get<2>(kv.first);
             issued = true;
-            if (kind == IT_READ_RES || kind == IT_READ_H1 || kind == IT_READ_H2) {
+            if (kind == IT_READ_RES || kind == IT_READ_H1 || kind == IT_READ_H2 || kind == IT_READ_MU ||
+                kind == IT_READ_PKEY) {
                 for (int s : kv.second) {
                     const int rc = read_issue(s, kind);
                     if (rc != GFE_OK) return rc;
@@ -1011,12 +1188,14 @@ int Batch::step(int32_t *active_out) {
             int rc = check_hip(hipMemcpyAsync(act_dev, st, cap, hipMemcpyHostToDevice, stream), "flags");
             stage_release();
             if (rc != GFE_OK) return rc;
-            switch (kind) {
-                case IT_COPY: op_copy(act_dev, a, b2); break;
-                case IT_SET: op_set(act_dev, a, (int32_t)b2); break;
-                case IT_INITMUL: op_initmul(act_dev, a); break;
-                case IT_MUL: op_mul(act_dev); break;
-            }
+            with_n(std::get<3>(kv.first), [&]() {
+                switch (kind) {
+                    case IT_COPY: op_copy(act_dev, a, b2); break;
+                    case IT_SET: op_set(act_dev, a, (int32_t)b2); break;
+                    case IT_INITMUL: op_initmul(act_dev, a); break;
+                    case IT_MUL: op_mul(act_dev); break;
+                }
+            });
             rc = check("op");
             if (rc != GFE_OK) return rc;
             for (int s : kv.second) slots[s].q.pop_front();
@@ -1024,7 +1203,7 @@ int Batch::step(int32_t *active_out) {
     }
     // 2. one block of up to 64 squaring rounds
     int rounds = 0;
-    uint32_t nact = 0;
+    uint32_t nact = 0, nbig = 0;
     std::vector<uint64_t> len(cap, 0);
     for (uint32_t s = 0; s < cap; ++s) {
         Slot &S = slots[s];
@@ -1032,11 +1211,13 @@ int Batch::step(int32_t *active_out) {
         len[s] = std::min<uint64_t>(64, S.q.front().len - S.pos);
         rounds = std::max(rounds, (int)len[s]);
         ++nact;
+        if (S.n == nalloc) ++nbig;
     }
     if (rounds > 0) {
         uint8_t *st = stage_acquire();
-        uint8_t *act = st, *dup = st + (size_t)64 * cap;
-        memset(st, 0, (size_t)2 * 64 * cap);
+        uint8_t *act = st, *dup = st + (size_t)64 * cap, *mul = st + (size_t)128 * cap;
+        memset(st, 0, (size_t)3 * 64 * cap);
+        bool has_mul = false;
         for (uint32_t s = 0; s < cap; ++s) {
             if (!len[s]) continue;
             const Slot &S = slots[s];
@@ -1047,14 +1228,19 @@ int Batch::step(int32_t *active_out) {
                 uint8_t d = 0;
                 if (it.run == RUN_PRP) d = S.bit(it.start + k) ? 1 : 0;
                 else if (it.run == RUN_RES) d = S.resbits[k];
+                else if (it.run == RUN_POW && S.powseq[it.start + k]) {
+                    mul[r * cap + s] = 1;
+                    has_mul = true;
+                }
                 dup[r * cap + s] = d;
             }
         }
         int rc = check_hip(hipMemcpyAsync(act_dev, act, (size_t)rounds * cap, hipMemcpyHostToDevice, stream), "act");
         if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(dup_dev, dup, (size_t)rounds * cap, hipMemcpyHostToDevice, stream), "dup");
+        if (rc == GFE_OK) rc = check_hip(hipMemcpyAsync(mul_dev, mul, (size_t)rounds * cap, hipMemcpyHostToDevice, stream), "mul");
         stage_release();
         if (rc != GFE_OK) return rc;
-        rc = run_block(rounds, nact);
+        rc = run_block(rounds, nact, has_mul, nbig);
         if (rc != GFE_OK) return rc;
         if ((int)blk_q.size() == NBLK) {
             rc = check_hip(hipEventSynchronize(blk_ev[blk_q.front()]), "block sync");
@@ -1106,18 +1292,23 @@ struct gfe_batch {
 
 extern "C" {
 
-size_t gfe_batch_slot_bytes(uint32_t n) {
+size_t gfe_batch_slot_bytes(uint32_t n) { return gfe_batch_proof_slot_bytes(n, 0); }
+
+size_t gfe_batch_proof_slot_bytes(uint32_t n, uint32_t depth) {
     const size_t N = (size_t)1 << n;
-    return 9 * N * 4 + (N >> 3) * 8 + sizeof(gfe::CrtConst) + sizeof(gfe::SlotInfo);
+    const size_t regs = depth > 0 ? 3 + ((size_t)1 << depth) : 3;
+    return (regs + 6) * N * 4 + (N >> 3) * 8 + sizeof(gfe::CrtConst) + sizeof(gfe::SlotInfo);
 }
 
-int gfe_batch_create(uint32_t n, uint32_t capacity, int device, gfe_batch **out) {
+
+static int batch_create(uint32_t n, uint32_t capacity, uint32_t depth, bool mixed, int device, gfe_batch **out) {
     if (!out) return set_error(GFE_EINVAL, "null output");
     *out = nullptr;
     if (n < 12 || n > 23) return set_error(GFE_EINVAL, "n must be in [12, 23]");
     if (capacity < 1 || capacity > 65535) return set_error(GFE_EINVAL, "capacity must be in [1, 65535]");
+    if (depth > GFE_BATCH_MAX_DEPTH) return set_error(GFE_EINVAL, "depth must be in [0, " + std::to_string(GFE_BATCH_MAX_DEPTH) + "]");
     gfe_batch *b = new gfe_batch;
-    const int rc = b->bt.init(n, capacity, device);
+    const int rc = b->bt.init(n, capacity, device, (int)depth, mixed);
     if (rc != GFE_OK) {
         const std::string msg = gfe::g_last_error;
         delete b;
@@ -1125,6 +1316,18 @@ int gfe_batch_create(uint32_t n, uint32_t capacity, int device, gfe_batch **out)
     }
     *out = b;
     return GFE_OK;
+}
+
+int gfe_batch_create(uint32_t n, uint32_t capacity, int device, gfe_batch **out) {
+    return batch_create(n, capacity, 0, false, device, out);
+}
+
+int gfe_batch_create_proof(uint32_t n, uint32_t capacity, uint32_t depth, int device, gfe_batch **out) {
+    return batch_create(n, capacity, depth, false, device, out);
+}
+
+int gfe_batch_create_mixed(uint32_t capacity, int device, gfe_batch **out) {
+    return batch_create(13, capacity, 0, true, device, out);
 }
 
 int gfe_batch_destroy(gfe_batch *b) {
@@ -1141,10 +1344,14 @@ int gfe_batch_info(const gfe_batch *b, char *buf, size_t len) {
 }
 
 int gfe_batch_submit(gfe_batch *b, uint32_t base, uint64_t iters, uint64_t tag, int32_t *slot) {
+    return gfe_batch_submit_n(b, b ? b->bt.nalloc : 0, base, iters, tag, slot);
+}
+
+int gfe_batch_submit_n(gfe_batch *b, uint32_t n, uint32_t base, uint64_t iters, uint64_t tag, int32_t *slot) {
     if (!b) return set_error(GFE_EINVAL, "null batch");
     if (base < 1024 || base > 2000000000u) return set_error(GFE_EINVAL, "b must be in [1024, 2000000000]");
     if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
-    return b->bt.submit(base, iters, tag, slot);
+    return b->bt.submit(base, iters, tag, slot, n);
 }
 
 int gfe_batch_step(gfe_batch *b, int32_t *active) {
@@ -1185,6 +1392,9 @@ int gfe_batch_set_plan(gfe_batch *b, const int32_t plan[6]) {
     if (!b || !plan) return set_error(GFE_EINVAL, "set_plan");
     gfe::Plan P;
     if (!gfe::plan_from_array(plan, b->bt.n, &P)) return set_error(GFE_EINVAL, "plan not admissible for this n");
+    if (P.limb) return set_error(GFE_EINVAL, "a batch does not run the two-digit-limb transform");
+    if (b->bt.mixed && (!P.spec || !P.fuse_sq))
+        return set_error(GFE_EINVAL, "a mixed batch keeps compile-time passes and the single-workgroup squaring");
     if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
     b->bt.flush_carries();
     if (b->bt.zero_carries() != GFE_OK || hipStreamSynchronize(b->bt.stream) != hipSuccess)
@@ -1209,18 +1419,20 @@ int gfe_batch_backlog(gfe_batch *b) {
     return b->bt.backlog();
 }
 
-size_t gfe_batch_device_bytes(uint32_t n, uint32_t capacity) {
+size_t gfe_batch_device_bytes(uint32_t n, uint32_t capacity) { return gfe_batch_proof_device_bytes(n, capacity, 0); }
+
+size_t gfe_batch_proof_device_bytes(uint32_t n, uint32_t capacity, uint32_t depth) {
     const size_t N = (size_t)1 << n;
-    return (size_t)capacity * (gfe_batch_slot_bytes(n) + 2 * 64) + 6 * N * 4;
+    return (size_t)capacity * (gfe_batch_proof_slot_bytes(n, depth) + 3 * 64) + 6 * N * 4;
 }
 
 int gfe_batch_get_digits(gfe_batch *b, int32_t slot, uint32_t reg, int32_t *out) {
-    if (!b || !out || slot < 0 || (uint32_t)slot >= b->bt.cap || reg >= 3) return set_error(GFE_EINVAL, "get_digits");
+    if (!b || !out || slot < 0 || (uint32_t)slot >= b->bt.cap || reg >= b->bt.nregs) return set_error(GFE_EINVAL, "get_digits");
     if (hipSetDevice(b->bt.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
-    gfe::gint g(b->bt.N, b->bt.slots[slot].busy ? b->bt.slots[slot].b : 2);
+    gfe::gint g(b->bt.slot_N(slot), b->bt.slots[slot].busy ? b->bt.slots[slot].b : 2);
     const int rc = b->bt.read_residue(slot, reg, g);
     if (rc != GFE_OK) return rc;
-    memcpy(out, g.data(), b->bt.N * 4);
+    memcpy(out, g.data(), b->bt.slot_N(slot) * 4);
     return GFE_OK;
 }
 
@@ -1230,25 +1442,31 @@ int gfe_batch_time_rounds(gfe_batch *b, uint32_t rounds, double *ms) {
     if (hipSetDevice(B.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
     // every round: the busy slots square without doubling (64 flag rows, as a squaring block)
     uint8_t *st = B.stage_acquire();
-    memset(st, 0, (size_t)2 * 64 * B.cap);
-    uint32_t nact = 0;
-    for (uint32_t s = 0; s < B.cap; ++s) nact += B.slots[s].busy ? 1 : 0;
+    memset(st, 0, (size_t)3 * 64 * B.cap);
+    uint32_t nact = 0, nbig = 0;
+    for (uint32_t s = 0; s < B.cap; ++s) {
+        nact += B.slots[s].busy ? 1 : 0;
+        nbig += (B.slots[s].busy && B.slots[s].n == B.nalloc) ? 1 : 0;
+    }
     for (uint32_t r = 0; r < 64; ++r)
         for (uint32_t s = 0; s < B.cap; ++s) st[(size_t)r * B.cap + s] = B.slots[s].busy ? 1 : 0;
     int rc = gfe::check_hip(hipMemcpyAsync(B.act_dev, st, (size_t)64 * B.cap, hipMemcpyHostToDevice, B.stream), "act");
     if (rc == GFE_OK)
         rc = gfe::check_hip(hipMemcpyAsync(B.dup_dev, st + (size_t)64 * B.cap, (size_t)64 * B.cap, hipMemcpyHostToDevice, B.stream), "dup");
+    if (rc == GFE_OK)
+        rc = gfe::check_hip(hipMemcpyAsync(B.mul_dev, st + (size_t)128 * B.cap, (size_t)64 * B.cap, hipMemcpyHostToDevice, B.stream), "mul");
     B.stage_release();
     if (rc != GFE_OK) return rc;
     // captured blocks are built before the timed region
-    if (B.use_graphs && !B.plan.pipe && !B.fused_block(nact)) {
+    const bool four_step = B.mixed ? (nbig > 0 && nbig < 128) : !B.fused_block(nact);
+    if (B.use_graphs && !B.plan.pipe && four_step) {
         if (rounds >= 64 && !B.block_graph(64)) return set_error(GFE_EHIP, "squaring block graph capture failed");
         if ((rounds % 64) && !B.block_graph((int)(rounds % 64))) return set_error(GFE_EHIP, "squaring block graph capture failed");
     }
     (void)hipEventRecord(B.ev0, B.stream);
     for (uint32_t done = 0; done < rounds;) {
         const int r = (int)std::min<uint32_t>(64, rounds - done);
-        rc = B.run_block(r, nact);
+        rc = B.run_block(r, nact, false, nbig);
         if (rc != GFE_OK) return rc;
         done += (uint32_t)r;
     }
@@ -1266,16 +1484,17 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     Batch &B = b->bt;
     if (hipSetDevice(B.device) != hipSuccess) return set_error(GFE_EHIP, "hipSetDevice");
     if (reps == 0) reps = 4;
+    if (B.mixed) return set_error(GFE_EINVAL, "a mixed batch keeps its plan (not tuned)");
     bool any = false;
     for (auto &S : B.slots) any |= S.busy;
     if (!any) return set_error(GFE_EINVAL, "tune needs busy slots");
     const std::vector<gfe::Plan> plans = gfe::candidate_plans(B.n);
     B.flush_carries();
     // squarings on the busy slots' r0: save and restore register 0 of every slot
-    const size_t bytes = (size_t)B.cap * 3 * B.N * 4;
+    const size_t row = B.NA * 4, pitch = (size_t)B.nregs * B.NA * 4;
     int32_t *save = nullptr;
-    if (hipMalloc(&save, bytes) != hipSuccess) return set_error(GFE_ENOMEM, "tune save");
-    (void)hipMemcpyAsync(save, B.regs, bytes, hipMemcpyDeviceToDevice, B.stream);
+    if (hipMalloc(&save, (size_t)B.cap * row) != hipSuccess) return set_error(GFE_ENOMEM, "tune save");
+    (void)hipMemcpy2DAsync(save, row, B.regs, pitch, row, B.cap, hipMemcpyDeviceToDevice, B.stream);
     const gfe::Plan orig = B.plan;
     gfe::Plan best = orig;
     double best_ms = 1e30;
@@ -1304,7 +1523,7 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
     B.use_graphs = graphs;
     B.drop_graphs();
     (void)B.zero_carries();
-    (void)hipMemcpyAsync(B.regs, save, bytes, hipMemcpyDeviceToDevice, B.stream);
+    (void)hipMemcpy2DAsync(B.regs, pitch, save, row, row, B.cap, hipMemcpyDeviceToDevice, B.stream);
     (void)hipStreamSynchronize(B.stream);
     (void)hipFree(save);
     return B.check("tune");

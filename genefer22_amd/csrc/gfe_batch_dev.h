@@ -9,6 +9,7 @@ namespace gfe {
 struct SlotInfo {
     uint32_t b;
     int32_t rns, is32, rounds;   // rounds: carry rounds of the fused inverse column pass
+    int32_t n;                   // the candidate's n (a mixed batch holds several)
 };
 
 struct BArgs {
@@ -20,9 +21,11 @@ struct BArgs {
     const uint32_t *tw;      // [2][3][N]
     const uint8_t *active;   // [cap] or null (all)
     const uint8_t *dup;      // [cap] or null
+    const uint8_t *mulf;     // [cap] or null: squaring rounds of these slots multiply by zp instead
     PrimeArgs prS, prU;
     long long reg_stride;    // 3 N
-    int n, a, ng;
+    int n, a, ng;            // n: the launch's transform size; slots of another n exit at once
+    int sn;                  // log2 of the per-slot allocation: slot s's residues at s 3 2^sn
     uint32_t src_reg, dst_reg;
     int32_t value;
     int32_t absorb;          // squaring rounds: the column forward pass adds the pending carries
@@ -31,6 +34,7 @@ struct BArgs {
 __device__ __forceinline__ int bslot(const BArgs &A) {
     const int s = (int)blockIdx.z;
     if (A.active && !A.active[s]) return -1;
+    if (A.slots[s].n != A.n) return -1;
     return s;
 }
 

@@ -13,6 +13,7 @@
 #include "gfe_pass.h"
 #include "gfe_spec.h"
 #include "gfe_fused.h"
+#include "gfe_limb.h"
 
 namespace gfe {
 
@@ -27,6 +28,8 @@ struct Plan {
     int spec = 0;    // 1: column passes and the squaring row pass with compile-time geometry (gfe_pass.h)
     int fuse_sq = 0; // 1: the squaring in one workgroup per candidate (gfe_fused.hip, n <= 13); the
                      //    other fields still shape the multiplicand passes
+    int limb = 0;    // 1: the two-digit-limb transform (gfe_limb.h): length N/2 over four 31-bit primes,
+                     //    compile-time passes; a and the row fields then describe the N/2-point transform
 };
 
 Plan default_plan(uint32_t n);
@@ -49,7 +52,15 @@ uint32_t prime_qinv(uint32_t p);
 // Sets the kernels' LDS limits once per process (engine, compile-time and single-workgroup passes).
 void engine_set_lds_attributes();
 CrtConst make_crt(int rns, int is32, uint32_t n, uint32_t b = 0);
+LimbCrt make_limb_crt(uint32_t n, uint32_t b);
 int carry_rounds(uint32_t b, uint32_t n);
+// The two-digit-limb transform is exact for (b, n): an NTT3 candidate (the reference's rule),
+// b >= 2^23 (B = b^2 >= 2^46) and N l_max^2 < P/4 with l_max = (b/2 + 4)(b + 1), P the product of
+// the four primes.
+bool limb_ok(uint32_t b, uint32_t n);
+// Plans of the limb transform the autotuner times (n: digits exponent).
+std::vector<Plan> candidate_limb_plans(uint32_t n);
+Plan default_limb_plan(uint32_t n);
 void make_twiddle_tables(const int *sets, const int *primes, int count, uint32_t n, uint32_t *out, int max_threads = 8);
 int set_error(int code, const std::string &msg);
 extern thread_local std::string g_last_error;
@@ -67,6 +78,17 @@ struct Engine {
     int64_t *carry = nullptr;
     uint32_t *tw = nullptr;
     CrtConst *crt = nullptr;
+    // two-digit-limb transform (limb_ok): four twiddle tables of N/2 entries, its CRT constants,
+    // its primes and b 2^32 mod p of each
+    uint32_t *tw_limb = nullptr;
+    void *limb_crt = nullptr;
+    PrimeArgs pr4;
+    uint32_t bm4[4] = {0, 0, 0, 0};
+    bool can_limb = false;
+    int tn() const { return plan.limb ? (int)n - 1 : (int)n; }   // log2 of the transform length
+    int tp() const { return plan.limb ? 4 : rns; }               // primes of the transform
+    const uint32_t *cur_tw() const { return plan.limb ? tw_limb : tw; }
+    const PrimeArgs &cur_pr() const { return plan.limb ? pr4 : pr; }
     int rounds = 0;   // carry rounds of the fused inverse column pass
     int cus = 0;      // compute units of the device
     std::map<std::tuple<const void *, int, size_t>, unsigned> resident;  // per (kernel, threads, LDS)
@@ -77,7 +99,7 @@ struct Engine {
     // forward pass adds them on the way in (plans with compile-time passes); every other use of the
     // registers runs the carry kernel first (flush_carry).
     bool carry_pending = false;
-    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused && !plan.fuse_sq; }
+    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused && !plan.fuse_sq && !plan.limb; }
     void flush_carry();
 
     ~Engine();

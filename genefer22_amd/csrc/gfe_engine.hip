@@ -12,6 +12,7 @@
 // a fused plan k_col_fwd_all, k_row, k_col_inv_norm, k_fold.
 
 #include "gfe_engine.h"
+#include "gfe_limb.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -46,6 +47,24 @@ struct CarryArgs {
 __global__ void __launch_bounds__(256) k_carry_wg(CarryArgs A) {
     const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w < A.nwg) carry_wg_body(A.dst, A.carry, A.nwg, A.b, w);
+}
+
+struct LimbNormArgs {
+    const uint32_t *ws;     // [4][N/2] residues of the limb transform
+    int32_t *dst;           // register 0
+    int64_t *wl;            // 2 x workgroups: boundary carries
+    const LimbCrt *crt;
+    size_t M;
+    int dup;
+};
+
+__global__ void __launch_bounds__(256) k_norm_limb(LimbNormArgs A) {
+    norm_limb_body(*A.crt, A.ws, A.M, A.dst, A.wl, A.dup != 0, blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_carry_limb(int32_t *dst, int64_t *wl, unsigned nwg, uint32_t b) {
+    const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nwg) carry_limb_body(dst, wl, nwg, b, w);
 }
 
 struct FoldArgs {
@@ -90,7 +109,17 @@ static const uint32_t k_primes[6] = {127u * (1u << 24) + 1u, 63u * (1u << 25) + 
                                      125u * (1u << 25) + 1u, 243u * (1u << 24) + 1u, 235u * (1u << 24) + 1u};
 static const uint32_t k_gen[6] = {3, 5, 31, 3, 7, 3};
 
-uint32_t prime_of(int is32, int k) { return k_primes[(is32 ? 3 : 0) + k]; }
+// Prime k of a set: 0 = the 31-bit primes, 1 = the 32-bit primes, LIMB_SET = the four primes of the
+// two-digit-limb transform (the 31-bit primes and LIMB_P4).
+constexpr int LIMB_SET = 2;
+uint32_t prime_of(int is32, int k) {
+    if (is32 == LIMB_SET) return k < 3 ? k_primes[k] : LIMB_P4;
+    return k_primes[(is32 ? 3 : 0) + k];
+}
+static uint32_t gen_of(int set, int k) {
+    if (set == LIMB_SET) return k < 3 ? k_gen[k] : LIMB_G4;
+    return k_gen[(set ? 3 : 0) + k];
+}
 
 static uint32_t hmul(uint32_t a, uint32_t b, uint32_t p) { return (uint32_t)(((uint64_t)a * b) % p); }
 static uint32_t hpow(uint32_t a, uint64_t e, uint32_t p) {
@@ -113,7 +142,7 @@ static uint32_t pinv32(uint32_t p) {
 uint32_t mulw_const(uint32_t w, uint32_t p) { return (uint32_t)((((uint64_t)w) << 32) % p); }
 
 uint32_t psi_mont(int is32, int k, uint32_t n) {
-    const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k];
+    const uint32_t p = prime_of(is32, k), g = gen_of(is32, k);
     return mulw_const(hpow(g, (p - 1) >> (n + 1), p), p);
 }
 
@@ -150,7 +179,7 @@ static inline uint32_t hmont(uint32_t a, uint32_t b, uint32_t p, uint32_t q) {
 // the j = 0 factor of factored row twiddles).  The powers psi^e 2^32 mod p follow by Montgomery
 // products with psi 2^32 mod p, and e's bit reversal by a reversed increment.
 static void make_twiddles(int is32, int k, uint32_t n, uint32_t *out) {
-    const uint32_t p = prime_of(is32, k), g = k_gen[(is32 ? 3 : 0) + k], q = pinv32(p);
+    const uint32_t p = prime_of(is32, k), g = gen_of(is32, k), q = pinv32(p);
     const size_t N = (size_t)1 << n;
     const uint32_t psim = mulw_const(hpow(g, (p - 1) >> (n + 1), p), p);
     uint32_t w = mulw_const(1, p);
@@ -231,6 +260,54 @@ CrtConst make_crt(int rns, int is32, uint32_t n, uint32_t b) {
     return c;
 }
 
+// Balanced base-B piece of x: returns r in (-B/2, B/2] with x = r (mod B) and leaves (x - r) / B in x.
+static int64_t bal_piece(__int128 &x, __int128 B) {
+    __int128 r = x % B;
+    if (r > B / 2) r -= B;
+    else if (r <= -(B / 2)) r += B;
+    x = (x - r) / B;
+    return (int64_t)r;
+}
+
+// Constants of the limb transform's CRT and carry (gfe_limb.h) for (b, n): transform length
+// M = 2^(n-1).
+LimbCrt make_limb_crt(uint32_t n, uint32_t b) {
+    LimbCrt c;
+    memset(&c, 0, sizeof(c));
+    __int128 P = 1;
+    for (int k = 0; k < 4; ++k) P *= prime_of(LIMB_SET, k);
+    const __int128 B = (__int128)b * b;
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t p = prime_of(LIMB_SET, k);
+        c.p[k] = p;
+        c.q[k] = pinv32(p);
+        const __int128 Q = P / p;
+        const uint32_t u = hinv((uint32_t)(Q % p), p);  // (P/p)^-1 mod p
+        // raw inverse-transform output = M 2^-32 (coefficient); times 2^32 M^-1 (P/p)^-1
+        const uint32_t s = hmul((uint32_t)((((uint64_t)1) << 32) % p), hinv((uint32_t)(((uint64_t)1 << (n - 1)) % p), p), p);
+        c.zc[k] = mulw_const(hmul(s, u, p), p);
+        c.zc_q[k] = c.zc[k] * c.q[k];
+        c.inv_p[k] = 1.0 / (double)p;
+        __int128 x = Q;
+        c.w0[k] = bal_piece(x, B);
+        c.w1[k] = bal_piece(x, B);
+        c.w2[k] = (int64_t)x;
+        c.w0d[k] = (double)c.w0[k];
+        c.w1d[k] = (double)c.w1[k];
+    }
+    __int128 x = P;
+    c.P0 = bal_piece(x, B);
+    c.P1 = bal_piece(x, B);
+    c.P2 = (int64_t)x;
+    c.P0d = (double)c.P0;
+    c.P1d = (double)c.P1;
+    c.B = (uint64_t)B;
+    c.invB = 1.0 / (double)c.B;
+    c.b = b;
+    c.binv = 1.0 / (double)b;
+    return c;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Plans
 
@@ -282,6 +359,39 @@ Plan default_batch_plan(uint32_t n, uint32_t capacity) {
     return P;
 }
 
+bool limb_ok(uint32_t b, uint32_t n) {
+    int rns = 0, is32 = 0;
+    select_primes(b, n, &rns, &is32);
+    if (rns != 3 || n < 12 || n > 23 || b < (1u << 23)) return false;
+    unsigned __int128 P = 1;
+    for (int k = 0; k < 4; ++k) P *= prime_of(LIMB_SET, k);
+    const uint64_t lmax = ((uint64_t)(b / 2) + 4) * ((uint64_t)b + 1);
+    const unsigned __int128 l2 = (unsigned __int128)lmax * lmax;
+    return l2 < (P / 4) >> n;
+}
+
+// The limb transform's plans are the compile-time plans of its length N/2 = 2^(n-1).
+Plan default_limb_plan(uint32_t n) {
+    Plan P = default_plan(n - 1);
+    P.limb = 1;
+    if (!plan_valid(P, n)) {
+        P = Plan();
+        return P;  // limb = 0: no limb plan for this n
+    }
+    return P;
+}
+
+std::vector<Plan> candidate_limb_plans(uint32_t n) {
+    std::vector<Plan> out;
+    if (n < 12) return out;
+    for (Plan P : candidate_plans(n - 1)) {
+        if (!P.spec || P.fuse_sq) continue;
+        P.limb = 1;
+        if (plan_valid(P, n)) out.push_back(P);
+    }
+    return out;
+}
+
 static Plan runtime_default_plan(uint32_t n) {
     Plan P;
     P.a = (int)n / 2;
@@ -300,6 +410,12 @@ static Plan runtime_default_plan(uint32_t n) {
 }
 
 bool plan_valid(const Plan &P, uint32_t n) {
+    if (P.limb) {  // compile-time passes of the N/2-point transform
+        if (!P.spec || P.fuse_sq || n < 12) return false;
+        Plan Q = P;
+        Q.limb = 0;
+        return plan_valid(Q, n - 1);
+    }
     if (P.fuse_sq) {
         if ((int)n < FUSED_MIN_N || (int)n > FUSED_MAX_N || P.fused || P.pipe) return false;
         Plan Q = P;
@@ -375,19 +491,20 @@ std::vector<Plan> candidate_plans(uint32_t n) {
 
 void plan_to_array(const Plan &P, int32_t out[6]) {
     out[0] = P.a; out[1] = P.col_loge; out[2] = P.col_ng; out[3] = P.row_loge; out[4] = P.row_ng;
-    out[5] = P.row_tw | (P.fused << 1) | (P.pipe << 2) | (P.spec << 3) | (P.fuse_sq << 4);
+    out[5] = P.row_tw | (P.fused << 1) | (P.pipe << 2) | (P.spec << 3) | (P.fuse_sq << 4) | (P.limb << 5);
 }
 
 bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out) {
     Plan P;
     const int fl = in[5] & 15;
-    if (in[5] < 0 || in[5] > 31 || (fl & 6) == 6 || ((fl & 8) && fl != 8)) return false;
+    if (in[5] < 0 || in[5] > 63 || (fl & 6) == 6 || ((fl & 8) && fl != 8)) return false;
     P.a = in[0]; P.col_loge = in[1]; P.col_ng = in[2]; P.row_loge = in[3]; P.row_ng = in[4];
     P.row_tw = fl & 1; P.fused = (fl >> 1) & 1; P.pipe = (fl >> 2) & 1; P.spec = (fl >> 3) & 1;
     P.fuse_sq = (in[5] >> 4) & 1;
+    P.limb = (in[5] >> 5) & 1;
     P.norm_s = 8;
     auto pow2 = [](int v) { return v >= 1 && (v & (v - 1)) == 0; };
-    if (P.a < 1 || P.a >= (int)n || P.col_loge < 3 || P.col_loge > 5 || P.row_loge < 3 || P.row_loge > 5) return false;
+    if (P.a < 1 || P.a >= (int)n - P.limb || P.col_loge < 3 || P.col_loge > 5 || P.row_loge < 3 || P.row_loge > 5) return false;
     if (!pow2(P.col_ng) || !pow2(P.row_ng)) return false;
     if (!plan_valid(P, n)) return false;
     *out = P;
@@ -396,9 +513,11 @@ bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out) {
 
 std::string plan_str(const Plan &P, uint32_t n) {
     std::ostringstream ss;
-    ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (n - P.a)
+    const uint32_t tn = n - (uint32_t)P.limb;
+    if (P.limb) ss << "limbs of two digits (2^" << tn << " limbs, four 31-bit primes); ";
+    ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (tn - P.a)
        << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg" << (P.row_tw ? ", factored twiddles" : "")
-       << "), " << (P.fused ? "all-prime columns with CRT + carry" : "norm 8 digits/thread")
+       << "), " << (P.fused ? "all-prime columns with CRT + carry" : P.limb ? "norm 4 limbs/thread" : "norm 8 digits/thread")
        << (P.pipe ? ", pipelined columns and norm" : "") << (P.spec ? ", compile-time passes" : "")
        << (P.fuse_sq ? ", squaring in one workgroup" : "");
     return ss.str();
@@ -535,6 +654,8 @@ Engine::~Engine() {
         if (stream) (void)hipStreamSynchronize(stream);
         (void)hipFree(regs); (void)hipFree(ws); (void)hipFree(zp); (void)hipFree(carry);
         (void)hipFree(tw); (void)hipFree(crt);
+        if (tw_limb) (void)hipFree(tw_limb);
+        if (limb_crt) (void)hipFree(limb_crt);
         if (stream) (void)hipStreamDestroy(stream);
         for (auto &ev : events) (void)hipEventDestroy(ev);
     }
@@ -574,6 +695,28 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
     const CrtConst c = make_crt(rns, is32, n, b);
     if (hipMemcpy(crt, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "crt upload");
     for (int k = 0; k < 3; ++k) { pr.p[k] = prime_of(is32, k); pr.q[k] = pinv32(pr.p[k]); }
+    pr.p[3] = pr.q[3] = 0;
+    can_limb = limb_ok(b, n) && default_limb_plan(n).limb;
+    if (can_limb) {
+        // the two-digit-limb transform: four tables of N/2 twiddles and its CRT constants
+        const size_t M = N / 2;
+        if (hipMalloc(&tw_limb, 4 * M * 4) != hipSuccess || hipMalloc(&limb_crt, sizeof(LimbCrt)) != hipSuccess)
+            return set_error(GFE_ENOMEM, "device allocation failed");
+        std::vector<uint32_t> htw(4 * M);
+        const int sets[4] = {LIMB_SET, LIMB_SET, LIMB_SET, LIMB_SET}, primes[4] = {0, 1, 2, 3};
+        make_twiddle_tables(sets, primes, 4, n - 1, htw.data());
+        if (hipMemcpy(tw_limb, htw.data(), 4 * M * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(GFE_EHIP, "twiddle upload");
+        const LimbCrt lc = make_limb_crt(n, b);
+        if (hipMemcpy(limb_crt, &lc, sizeof(lc), hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(GFE_EHIP, "crt upload");
+        for (int k = 0; k < 4; ++k) {
+            pr4.p[k] = prime_of(LIMB_SET, k);
+            pr4.q[k] = pinv32(pr4.p[k]);
+            bm4[k] = mulw_const(b % pr4.p[k], pr4.p[k]);
+        }
+        plan = default_limb_plan(n);
+    }
     if (hipMemsetAsync(regs, 0, (size_t)num_regs * N * 4, stream) != hipSuccess) return set_error(GFE_EHIP, "memset");
     for (int i = 0; i < 12; ++i) { hipEvent_t ev; if (hipEventCreate(&ev) != hipSuccess) return set_error(GFE_EHIP, "event"); events.push_back(ev); }
     if (hipStreamSynchronize(stream) != hipSuccess) return set_error(GFE_EHIP, "init sync");
@@ -600,7 +743,7 @@ size_t Engine::lds_col() const {
     return (size_t)plan.col_ng * ((1u << plan.a) + ((1u << plan.a) >> plan.col_loge)) * 4;
 }
 size_t Engine::lds_row() const {
-    const uint32_t G = 1u << (n - plan.a);
+    const uint32_t G = 1u << (tn() - plan.a);
     return (size_t)plan.row_ng * (G + (G >> plan.row_loge)) * 4;
 }
 
@@ -621,6 +764,17 @@ unsigned Engine::resident_blocks(const void *fn, int threads, size_t lds) {
 void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
     const bool absorb = carry_pending && src == 0 && defer_carry();
     if (carry_pending && !absorb) flush_carry();
+    if (plan.limb) {
+        ColArgs A;
+        A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw_limb; A.pr = pr4;
+        A.n = tn(); A.a = plan.a; A.ng = plan.col_ng;
+        for (int k = 0; k < 4; ++k) A.bm[k] = bm4[k];
+        const uint32_t C = 1u << (tn() - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(spec_col_fwd_limb(plan.col_loge, plan.a), dim3(C / plan.col_ng, 4, 1), dim3(T),
+                           col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A);
+        return;
+    }
     if (plan.spec) {
         ColArgs A;
         A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw; A.pr = pr;
@@ -669,12 +823,12 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
 void Engine::launch_col_inv(int dup) {
     if (plan.spec) {
         ColArgs A;
-        A.src = nullptr; A.ws = ws; A.tw = tw; A.pr = pr;
-        A.n = (int)n; A.a = plan.a; A.ng = plan.col_ng;
-        const uint32_t C = 1u << (n - plan.a);
+        A.src = nullptr; A.ws = ws; A.tw = cur_tw(); A.pr = cur_pr();
+        A.n = tn(); A.a = plan.a; A.ng = plan.col_ng;
+        const uint32_t C = 1u << (tn() - plan.a);
         const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
-        hipLaunchKernelGGL(spec_col_inv(is32, plan.col_loge, plan.a), dim3(C / plan.col_ng, rns, 1), dim3(T),
-                           col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A);
+        hipLaunchKernelGGL(spec_col_inv(plan.limb ? 0 : is32, plan.col_loge, plan.a), dim3(C / plan.col_ng, tp(), 1),
+                           dim3(T), col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A);
         return;
     }
     const KernelSet K = kernels_for(rns, is32, plan);
@@ -708,23 +862,32 @@ void Engine::launch_col_inv(int dup) {
 }
 
 void Engine::launch_row(int mode, uint32_t *data) {
-    const KernelSet K = kernels_for(rns, is32, plan);
+    const KernelSet K = kernels_for(rns, plan.limb ? 0 : is32, plan);
     RowArgs A;
-    A.ws = data; A.zp = zp; A.tw = tw; A.pr = pr;
-    A.n = (int)n; A.a = plan.a; A.ng = plan.row_ng;
-    const uint32_t R = 1u << plan.a, G = 1u << (n - plan.a);
+    A.ws = data; A.zp = zp; A.tw = cur_tw(); A.pr = cur_pr();
+    A.n = tn(); A.a = plan.a; A.ng = plan.row_ng;
+    const uint32_t R = 1u << plan.a, G = 1u << (tn() - plan.a);
     const int T = plan.row_ng * G / (1 << plan.row_loge);
     if (plan.spec && mode == ROW_SQUARE) {
-        hipLaunchKernelGGL(spec_row_square(is32, plan.row_loge, (int)n - plan.a), dim3(R / plan.row_ng, rns, 1), dim3(T),
-                           row_pass_lds_words((int)n - plan.a, plan.row_loge, plan.row_ng) * 4, stream, A);
+        hipLaunchKernelGGL(spec_row_square(plan.limb ? 0 : is32, plan.row_loge, tn() - plan.a), dim3(R / plan.row_ng, tp(), 1),
+                           dim3(T), row_pass_lds_words(tn() - plan.a, plan.row_loge, plan.row_ng) * 4, stream, A);
         return;
     }
     RowFn f = (mode == ROW_SQUARE) ? K.row_sq : (mode == ROW_MUL) ? K.row_mul : K.row_fwd;
-    hipLaunchKernelGGL(f, dim3(R / plan.row_ng, rns, 1), dim3(T), lds_row(), stream, A);
+    hipLaunchKernelGGL(f, dim3(R / plan.row_ng, tp(), 1), dim3(T), lds_row(), stream, A);
 }
 
 // Normalisation of register 0 (fused plan: the cross-segment carries of k_col_inv_norm).
 void Engine::launch_norm(int dup) {
+    if (plan.limb) {
+        LimbNormArgs A;
+        A.ws = ws; A.dst = regs; A.wl = carry; A.crt = static_cast<const LimbCrt *>(limb_crt);
+        A.M = N / 2; A.dup = dup;
+        const unsigned nwg = (unsigned)(A.M / LIMB_RANGE);
+        hipLaunchKernelGGL(k_norm_limb, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
+        hipLaunchKernelGGL(k_carry_limb, dim3((nwg + 255) / 256, 1, 1), dim3(256), 0, stream, regs, carry, nwg, b);
+        return;
+    }
     if (plan.fused) {
         FoldArgs A;
         A.dst = regs; A.seg = carry; A.nseg = N / (size_t)plan.col_ng; A.ng = plan.col_ng; A.b = b;
@@ -952,7 +1115,7 @@ int gfe_set_digits(gfe_engine *e, uint32_t reg, const int32_t *in) {
 size_t gfe_mem_size(const gfe_engine *e) {
     if (!e) return 0;
     const Engine &g = e->eng;
-    return (size_t)g.num_regs * g.N * 4 + 3 * (size_t)g.rns * g.N * 4 + g.N / 2048 * 8;
+    return (size_t)g.num_regs * g.N * 4 + 3 * (size_t)g.rns * g.N * 4 + g.N / 2048 * 8 + (g.can_limb ? 2 * g.N * 4 : 0);
 }
 
 size_t gfe_cache_size(const gfe_engine *e) {
@@ -1142,7 +1305,9 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
         x ^= x << 13; x ^= x >> 7; x ^= x << 17;
         trial[i] = (int32_t)((int64_t)(x % g.b) - half + ((g.b & 1) ? 0 : 1));
     }
-    const std::vector<gfe::Plan> plans = gfe::candidate_plans(g.n);
+    std::vector<gfe::Plan> plans = gfe::candidate_plans(g.n);
+    if (g.can_limb)
+        for (const gfe::Plan &P : gfe::candidate_limb_plans(g.n)) plans.push_back(P);
     const gfe::Plan orig = g.plan;
     // ms for `count` squarings under plan P (after one untimed), < 0 on a launch error
     auto time_plan = [&](const gfe::Plan &P, uint32_t count, double *ms) -> int {
@@ -1196,6 +1361,17 @@ int gfe_default_plan(uint32_t n, uint32_t capacity, int32_t plan[6]) {
     return GFE_OK;
 }
 
+int gfe_engine_plan(uint32_t b, uint32_t n, int32_t plan[6]) {
+    if (!plan || n < 12 || n > 23 || b < 1024 || b > 2000000000u) return set_error(GFE_EINVAL, "engine_plan");
+    gfe::Plan P = gfe::default_plan(n);
+    if (gfe::limb_ok(b, n)) {
+        const gfe::Plan L = gfe::default_limb_plan(n);
+        if (L.limb) P = L;
+    }
+    gfe::plan_to_array(P, plan);
+    return GFE_OK;
+}
+
 int gfe_plan_admissible(uint32_t n, const int32_t plan[6]) {
     gfe::Plan P;
     if (!plan || n < 12 || n > 23 || !gfe::plan_from_array(plan, n, &P))
@@ -1225,6 +1401,7 @@ int gfe_set_plan(gfe_engine *e, const int32_t plan[6]) {
     if (!plan) return set_error(GFE_EINVAL, "set_plan");
     gfe::Plan P;
     if (!gfe::plan_from_array(plan, e->eng.n, &P)) return set_error(GFE_EINVAL, "plan not admissible for this n");
+    if (P.limb && !e->eng.can_limb) return set_error(GFE_EINVAL, "the limb transform is not exact for this (b, n)");
     e->eng.flush_carry();
     if (hipStreamSynchronize(e->eng.stream) != hipSuccess) return set_error(GFE_EHIP, "set_plan sync");
     e->eng.plan = P;

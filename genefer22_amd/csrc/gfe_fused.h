@@ -22,6 +22,8 @@ struct FusedArgs {
     uint32_t b;               // engine only
     int rns, is32;            // engine only
     PrimeArgs prS, prU;
+    uint32_t tws;             // entries between the primes' twiddle tables
+    int only_n;               // mixed kernel: run the slots of this n only (0: all)
 };
 
 typedef void (*FusedFn)(FusedArgs);
@@ -29,6 +31,8 @@ typedef void (*FusedFn)(FusedArgs);
 // Kernel for N = 2^logn digits (null outside FUSED_MIN_N..FUSED_MAX_N); launch with grid.x =
 // slots, blockDim.x = fused_threads(logn) and fused_lds_bytes(logn) bytes of dynamic LDS.
 FusedFn fused_square_fn(int logn);
+// One launch for the slots of n = 12 and 13 of a mixed batch (512 threads, fused_lds_bytes(13)).
+FusedFn fused_square_mixed_fn();
 size_t fused_lds_bytes(int logn);
 unsigned fused_threads(int logn);
 void fused_set_lds_attributes();

@@ -32,9 +32,11 @@ __host__ __device__ inline size_t fused_lds_bytes_n(int logn) {
 // chunks interleaved so that they share the LDS exchanges' barriers (and give each thread RNS times
 // the independent butterflies).  Prime k's residues end in the padded region res3 + k RS (element
 // j at j + (j >> FLOGE)).
+// tw3: the primes' twiddle tables, `tws` entries apart (a table of a larger N serves: its first N
+// entries are the table of N).
 template <class M, int LOGN, int RNS, int K0>
 __device__ __forceinline__ void fused_ntt(const M (&md3)[3], uint32_t *res3, const int32_t *dig,
-                                          const uint32_t *__restrict__ tw3) {
+                                          const uint32_t *__restrict__ tw3, uint32_t tws) {
     constexpr int FLOGE = fused_loge(LOGN), FE = 1 << FLOGE;
     using GM = Geo<LOGN, FLOGE>;
     constexpr uint32_t N = 1u << LOGN, RS = N + N / FE;
@@ -43,7 +45,7 @@ __device__ __forceinline__ void fused_ntt(const M (&md3)[3], uint32_t *res3, con
 #pragma unroll
     for (int k = 0; k < RNS; ++k) md[k] = md3[K0 + k];
     uint32_t *res = res3 + K0 * RS;
-    const uint32_t *tw = tw3 + K0 * N;
+    const uint32_t *tw = tw3 + K0 * tws;
     const uint32_t s = threadIdx.x;
     uint32_t x[RNS][FE], w[RNS][FE];
     static_for<0, GM::NCH>([&](auto cc) {
@@ -52,7 +54,7 @@ __device__ __forceinline__ void fused_ntt(const M (&md3)[3], uint32_t *res3, con
         const uint32_t hi = s >> YL, base = (hi << (LOGN - lam0)) + (s & ((1u << YL) - 1u));
         const uint32_t lb = base + (base >> FLOGE);
 #pragma unroll
-        for (int k = 0; k < RNS; ++k) load_fwd_tw<FLOGE>(w[k], tw + k * N, (1u << lam0) + hi, NE);
+        for (int k = 0; k < RNS; ++k) load_fwd_tw<FLOGE>(w[k], tw + k * tws, (1u << lam0) + hi, NE);
         if constexpr (ci == 0) {
 #pragma unroll
             for (int rr = 0; rr < FE; ++rr) {
@@ -80,7 +82,7 @@ __device__ __forceinline__ void fused_ntt(const M (&md3)[3], uint32_t *res3, con
     constexpr int LAST = GM::NCH - 1;
 #pragma unroll
     for (int k = 0; k < RNS; ++k) {
-        load_inv_tw<FLOGE>(w[k], tw + k * N, (1u << GM::lam0(LAST)) + s, GM::lam0(LAST), GM::ne(LAST));
+        load_inv_tw<FLOGE>(w[k], tw + k * tws, (1u << GM::lam0(LAST)) + s, GM::lam0(LAST), GM::ne(LAST));
 #pragma unroll
         for (int rr = 0; rr < FE; ++rr) x[k][rr] = md[k].mont(x[k][rr], x[k][rr]);
     }
@@ -91,7 +93,7 @@ __device__ __forceinline__ void fused_ntt(const M (&md3)[3], uint32_t *res3, con
         const uint32_t lb = base + (base >> FLOGE);
         if constexpr (ci != LAST) {
 #pragma unroll
-            for (int k = 0; k < RNS; ++k) load_inv_tw<FLOGE>(w[k], tw + k * N, (1u << lam0) + hi, lam0, NE);
+            for (int k = 0; k < RNS; ++k) load_inv_tw<FLOGE>(w[k], tw + k * tws, (1u << lam0) + hi, lam0, NE);
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < RNS; ++k)
@@ -188,12 +190,12 @@ __device__ __forceinline__ void fused_block(const FusedArgs &A, const M (&md)[3]
         any = true;
         __syncthreads();  // digits of the previous round (or the load) complete
         if constexpr (LOGN >= 13) {  // one prime at a time: 16 points per thread x 3 primes exceed the registers
-            fused_ntt<M, LOGN, 1, 0>(md, res, dig, tw);
-            fused_ntt<M, LOGN, 1, 1>(md, res, dig, tw);
-            if (rns == 3) fused_ntt<M, LOGN, 1, 2>(md, res, dig, tw);
+            fused_ntt<M, LOGN, 1, 0>(md, res, dig, tw, A.tws);
+            fused_ntt<M, LOGN, 1, 1>(md, res, dig, tw, A.tws);
+            if (rns == 3) fused_ntt<M, LOGN, 1, 2>(md, res, dig, tw, A.tws);
         } else {
-            if (rns == 3) fused_ntt<M, LOGN, 3, 0>(md, res, dig, tw);
-            else fused_ntt<M, LOGN, 2, 0>(md, res, dig, tw);
+            if (rns == 3) fused_ntt<M, LOGN, 3, 0>(md, res, dig, tw, A.tws);
+            else fused_ntt<M, LOGN, 2, 0>(md, res, dig, tw, A.tws);
         }
         __syncthreads();
         if (rns == 3) fused_crt<3, M, LOGN>(c, res, dig, cs, b, dup);
@@ -206,18 +208,15 @@ __device__ __forceinline__ void fused_block(const FusedArgs &A, const M (&md)[3]
         *reinterpret_cast<int4 *>(gdig + t * FE + j) = make_int4(dig[o + j], dig[o + j + 1], dig[o + j + 2], dig[o + j + 3]);
 }
 
-// grid.x = slots (batch) or 1 (single engine); blockDim.x = fused_threads(n).
+// The block of rounds of one workgroup's slot (batch) or of the engine's candidate at N = 2^LOGN.
 template <int LOGN>
-__global__ void __launch_bounds__((1 << LOGN) >> fused_loge(LOGN)) kf_square(FusedArgs A) {
-    extern __shared__ uint32_t lds[];
-    const int slot = (int)blockIdx.x;
+__device__ __forceinline__ void fused_slot(const FusedArgs &A, int slot, uint32_t *lds) {
     uint32_t b;
     int rns, is32;
     const CrtConst *c;
     int32_t *gdig;
     if (A.slots) {
         const SlotInfo si = A.slots[slot];
-        if (si.rns == 0) return;
         b = si.b; rns = si.rns; is32 = si.is32;
         c = A.crt + slot;
         gdig = A.regs + (size_t)slot * A.reg_stride;
@@ -226,7 +225,7 @@ __global__ void __launch_bounds__((1 << LOGN) >> fused_loge(LOGN)) kf_square(Fus
         c = A.crt;
         gdig = A.regs;
     }
-    const uint32_t *tw = A.tw + (A.slots ? (size_t)is32 * 3 * ((size_t)1 << LOGN) : 0);
+    const uint32_t *tw = A.tw + (A.slots ? (size_t)is32 * 3 * A.tws : 0);
     if (is32) {
         const ModU md[3] = {{A.prU.p[0], A.prU.q[0]}, {A.prU.p[1], A.prU.q[1]}, {A.prU.p[2], A.prU.q[2]}};
         fused_block<ModU, LOGN>(A, md, rns, b, *c, gdig, tw, slot, lds);
@@ -235,6 +234,33 @@ __global__ void __launch_bounds__((1 << LOGN) >> fused_loge(LOGN)) kf_square(Fus
         fused_block<ModS, LOGN>(A, md, rns, b, *c, gdig, tw, slot, lds);
     }
 }
+
+// grid.x = slots (batch) or 1 (single engine); blockDim.x = fused_threads(n).  Batch slots of
+// another n (or empty, rns = 0) exit at once.
+template <int LOGN>
+__global__ void __launch_bounds__((1 << LOGN) >> fused_loge(LOGN)) kf_square(FusedArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int slot = (int)blockIdx.x;
+    if (A.slots && (A.slots[slot].rns == 0 || A.slots[slot].n != LOGN)) return;
+    fused_slot<LOGN>(A, slot, lds);
+}
+
+// Mixed batches: one launch runs the block for slots of n = 12 and n = 13 alike (both take 512
+// threads; the LDS is sized for n = 13), each workgroup at its slot's n; with only_n != 0 the
+// slots of that n alone.
+__global__ void __launch_bounds__(512) kf_square_mixed(FusedArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int slot = (int)blockIdx.x;
+    const SlotInfo &si = A.slots[slot];
+    if (si.rns == 0 || (A.only_n && si.n != A.only_n)) return;
+    if (si.n == 12) fused_slot<12>(A, slot, lds);
+    else if (si.n == 13) fused_slot<13>(A, slot, lds);
+}
+
+static_assert(((1 << 12) >> fused_loge(12)) == 512 && ((1 << 13) >> fused_loge(13)) == 512,
+              "the mixed kernel's slots of n = 12 and 13 take the same workgroup");
+
+FusedFn fused_square_mixed_fn() { return kf_square_mixed; }
 
 FusedFn fused_square_fn(int logn) {
     switch (logn) {
@@ -255,6 +281,7 @@ void fused_set_lds_attributes() {
         for (int n = FUSED_MIN_N; n <= FUSED_MAX_N; ++n)
             (void)hipFuncSetAttribute((const void *)fused_square_fn(n), hipFuncAttributeMaxDynamicSharedMemorySize,
                                       160 * 1024);
+        (void)hipFuncSetAttribute((const void *)kf_square_mixed, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
     });
 }

@@ -30,9 +30,11 @@ namespace gfe {
 
 constexpr int MAX_RNS = 3;
 
+// Primes of one transform: the two or three of the prime set, or the four of the two-digit-limb
+// transform (gfe_limb.h).
 struct PrimeArgs {
-    uint32_t p[MAX_RNS];
-    uint32_t q[MAX_RNS];
+    uint32_t p[4];
+    uint32_t q[4];
 };
 
 // Forward CT levels mu = 0..ne-1 of a 2^LOGE-point sub-transform held in x[].  twf(mu, bi) is
@@ -469,10 +471,12 @@ enum RowMode { ROW_SQUARE = 0, ROW_MUL = 1, ROW_FWD = 2 };
 // 3 2^l - 1 - m = 2^l + (R-1-row) 2^(l-a) + (2^(l-a) - 1 - j) factors the same way.
 //
 // `data` and `zp` point at the prime's N residues.
+// With MODE == ROW_SQUARE, `mul` (uniform over the workgroup) takes the product by zp instead of the
+// square (a multiplying round of a batched power).
 template <class M, int LOGE, int MODE, int TWF>
 __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ zp,
                                          const uint32_t *__restrict__ tw, int n, int a, int NG, unsigned blk,
-                                         uint32_t *lds) {
+                                         uint32_t *lds, bool mul = false) {
     constexpr int E = 1 << LOGE;
     const int g = n - a, G = 1 << g;
     const int spr = G >> LOGE;  // sub-transforms per row
@@ -538,7 +542,7 @@ __device__ __forceinline__ void row_body(const M &md, uint32_t *__restrict__ dat
     // the last forward chunk's element set is the first inverse chunk's: its twiddles load while
     // the pointwise square runs
     if (!TWF) inv_tw(w, lam0, hi, ne);
-    if (MODE == ROW_SQUARE) {
+    if (MODE == ROW_SQUARE && !mul) {
 #pragma unroll
         for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], x[rr]);
     } else {
@@ -767,6 +771,7 @@ struct ColArgs {
     int n, a, ng;
     const int64_t *wc = nullptr;  // forward: workgroup carries still to add into src (carry_wg_body), or null
     uint32_t b = 0;
+    uint32_t bm[4] = {0, 0, 0, 0};  // limb forward pass: b 2^32 mod p of each prime
 };
 
 // 32 points per thread need more than the 128 registers a 1024-thread workgroup leaves: such plans

@@ -1,0 +1,202 @@
+// gfe_limb.h -- the two-digit-limb transform: CRT and carry of limbs over four 31-bit primes.
+//
+// b^N + 1 = B^M + 1 with B = b^2 and M = N / 2, so a residue is also the M limbs
+// l_j = d_2j + b d_2j+1 in base B, and a squaring mod b^N + 1 is a negacyclic convolution of length
+// M of the limbs.  With balanced digits |l_j| <= (b/2 + 4)(b + 1) and a (doubled) coefficient is at
+// most N l_max^2; four primes p1 p2 p3 p4 (P ~ 2^123.6) keep that below P / 4 for b up to about
+// 4.4e7 at n = 22 (limb_ok on the host).  Against the three-prime transform of length N this
+// halves the transform length and multiplies the primes by 4/3: 36 % fewer butterflies and a third
+// less transform data per squaring.
+//
+// CRT (explicit form).  With z_k = x (P/p_k)^-1 mod p_k (one Montgomery product per prime: the
+// inverse transform's raw output times a constant that also undoes the transform scale and the
+// pointwise product's 2^-32), x = sum_k z_k (P/p_k) - t P where t = round(sum_k z_k / p_k), since
+// |x| < P/4 puts the fractional part within 1/4 of an integer.  P/p_k and P are split in balanced
+// base-B pieces on the host (w0 + w1 B + w2 B^2), so x = h0 + h1 B + h2 B^2 with |h0|, |h1| <= B/2
+// follows from two balanced divisions by B of sums known exactly mod 2^64 and as double estimates
+// (error < 2^35 << B, B >= 2^46), and h2 (|h2| <= N/4 + 1) exactly.  The limb of the product is then
+// L_j = h0_j + h1_(j-1) + h2_(j-2) (|L_j| <~ B + N/4), split into two balanced digits with a carry
+// of a few units: every quantity stays within 64 bits.
+#pragma once
+
+#include "gfe_kernels.h"
+
+namespace gfe {
+
+constexpr int LIMB_PRIMES = 4;
+constexpr uint32_t LIMB_P4 = 27u * (1u << 26) + 1u;  // 1811939329, generator 13; 2^26 | p4 - 1
+constexpr uint32_t LIMB_G4 = 13u;
+constexpr int LIMB_S = 4;                             // limbs per normalisation thread
+constexpr uint32_t LIMB_RANGE = 256 * LIMB_S;         // limbs per normalisation workgroup (2048 digits)
+
+struct LimbCrt {
+    uint32_t p[4], q[4];
+    uint32_t zc[4], zc_q[4];   // mulp operands of 2^32 M^-1 (P/p_k)^-1 mod p_k and their w q factors
+    double inv_p[4];           // 1 / p_k
+    int64_t w0[4], w1[4], w2[4];  // P/p_k = w0 + w1 B + w2 B^2 (balanced pieces)
+    double w0d[4], w1d[4];
+    int64_t P0, P1, P2;        // P = P0 + P1 B + P2 B^2
+    double P0d, P1d;
+    uint64_t B;                // b^2
+    double invB;
+    uint32_t b;
+    int32_t pad;
+    double binv;
+};
+
+// Exact int64 of an integer-valued double, |v| < 2^52.
+__device__ __forceinline__ int64_t i64_of(double v) {
+    const double hi = floor(v * 0x1p-32);
+    const uint32_t lo = (uint32_t)fma(hi, -0x1p32, v);
+    return (int64_t)(((uint64_t)(uint32_t)(int32_t)hi << 32) | lo);
+}
+
+// Balanced division by B of the exact integer s, given as s mod 2^64 and an estimate sd with
+// |sd - s| << B: s = q B + h, h in (-B/2, B/2].
+__device__ __forceinline__ int64_t split_B(uint64_t s64, double sd, const LimbCrt &c, int64_t *q) {
+    int64_t qi = i64_of(rint(sd * c.invB));
+    int64_t h = (int64_t)(s64 - (uint64_t)qi * c.B);
+    const int64_t half = (int64_t)(c.B >> 1);
+    if (h > half) { h -= (int64_t)c.B; qi += 1; }
+    else if (h <= -half) { h += (int64_t)c.B; qi -= 1; }
+    *q = qi;
+    return h;
+}
+
+// Base-B pieces of the limb coefficient whose raw inverse-transform residues are a[0..3] (times 2
+// when dup).
+__device__ __forceinline__ void limb_pieces(const LimbCrt &c, const uint32_t (&a)[4], bool dup, int64_t &h0,
+                                            int64_t &h1, int64_t &h2) {
+    uint32_t z[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const ModS m{c.p[k], c.q[k]};
+        z[k] = m.mulp(a[k], c.zc[k], c.zc_q[k]);
+    }
+    double ts = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ts = fma((double)z[k], c.inv_p[k], ts);
+    const int32_t t = (int32_t)rint(ts);
+    uint64_t s0 = (uint64_t)(-(int64_t)t) * (uint64_t)c.P0;
+    double s0d = -(double)t * c.P0d;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        s0 += (uint64_t)z[k] * (uint64_t)c.w0[k];
+        s0d = fma((double)z[k], c.w0d[k], s0d);
+    }
+    int64_t q0;
+    h0 = split_B(s0, s0d, c, &q0);
+    uint64_t s1 = (uint64_t)q0 - (uint64_t)(int64_t)t * (uint64_t)c.P1;
+    double s1d = fma(-(double)t, c.P1d, (double)q0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        s1 += (uint64_t)z[k] * (uint64_t)c.w1[k];
+        s1d = fma((double)z[k], c.w1d[k], s1d);
+    }
+    int64_t q1;
+    h1 = split_B(s1, s1d, c, &q1);
+    int64_t s2 = q1 - (int64_t)t * c.P2;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s2 += (int64_t)z[k] * c.w2[k];
+    h2 = s2;
+    if (dup) {  // 2x = 2 h0 + 2 h1 B + 2 h2 B^2: pieces up to B stay within the limb bounds below
+        h0 *= 2; h1 *= 2; h2 *= 2;
+    }
+}
+
+// Two balanced digits of the limb value v and the carry out: v = d0 + b d1 + B c (div_bal is exact
+// for |v| < 2^63; its double estimate only seeds the quotient).
+__device__ __forceinline__ void limb_digits(int64_t v, uint32_t b, double binv, int32_t &d0, int32_t &d1, int64_t &c) {
+    int64_t q;
+    d0 = div_bal((uint64_t)v, (double)v, b, binv, &q);
+    d1 = div_bal((uint64_t)q, (double)q, b, binv, &c);
+}
+
+// Adds the carries of a limb normalisation range boundary into the next range's first S digits:
+// e0 at digit 0, e1 at digit 2 (the limb after), digit by digit while a carry remains.
+template <int S>
+__device__ __forceinline__ void absorb_carry2(int32_t (&d)[S], int64_t e0, int64_t e1, uint32_t b, double binv) {
+    int64_t c = e0;
+#pragma unroll
+    for (int j = 0; j < S - 1; ++j) {
+        if (j == 2) c += e1;
+        if (c != 0) {
+            const int64_t f = (int64_t)d[j] + c;
+            d[j] = div_bal((uint64_t)f, (double)f, b, binv, &c);
+        }
+    }
+    d[S - 1] += (int32_t)c;
+}
+
+// Normalisation of limbs [blk 1024, (blk + 1) 1024): scale + CRT to base-B pieces, the pieces of
+// the limbs below added in (the previous thread's through LDS; the previous workgroup's by
+// carry_limb_body after the kernel), two digits per limb with the small carry, the thread carries
+// handed up through LDS.  wl[2 blk], wl[2 blk + 1] receive what the next range's first two limbs
+// still need: e0 (into its limb 0) and e1 (into its limb 1).  blockDim.x must be 256.
+__device__ __forceinline__ void norm_limb_body(const LimbCrt &c, const uint32_t *__restrict__ ws, size_t M,
+                                               int32_t *__restrict__ dst, int64_t *__restrict__ wl, bool dup,
+                                               unsigned blk) {
+    constexpr int S = LIMB_S;
+    __shared__ int64_t xs[3][256];
+    const int tid = threadIdx.x;
+    const size_t j0 = ((size_t)blk * 256 + tid) * S;
+    uint32_t a[4][S];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(ws + (size_t)k * M + j0);
+        a[k][0] = v.x; a[k][1] = v.y; a[k][2] = v.z; a[k][3] = v.w;
+    }
+    int64_t h0[S], h1[S], h2[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const uint32_t r[4] = {a[0][j], a[1][j], a[2][j], a[3][j]};
+        limb_pieces(c, r, dup, h0[j], h1[j], h2[j]);
+    }
+    xs[0][tid] = h1[S - 1];
+    xs[1][tid] = h2[S - 2];
+    xs[2][tid] = h2[S - 1];
+    __syncthreads();
+    int64_t pb1 = 0, pb2 = 0, pl2 = 0;  // previous thread: h1 of its last limb, h2 of its last two
+    if (tid > 0) { pb1 = xs[0][tid - 1]; pb2 = xs[1][tid - 1]; pl2 = xs[2][tid - 1]; }
+    const uint32_t b = c.b;
+    const double binv = c.binv;
+    int32_t d[2 * S];
+    int64_t cr = 0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int64_t lj = h0[j] + (j == 0 ? pb1 + pb2 : h1[j - 1] + (j == 1 ? pl2 : h2[j - 2]));
+        limb_digits(lj + cr, b, binv, d[2 * j], d[2 * j + 1], cr);
+    }
+    // the thread's carry-out (a few units) goes into the next thread's digits
+    __syncthreads();
+    xs[0][tid] = cr;
+    __syncthreads();
+    if (tid > 0) absorb_carry<2 * S>(d, xs[0][tid - 1], b, binv);
+    int32_t *o = dst + 2 * j0;
+#pragma unroll
+    for (int j = 0; j < 2 * S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);
+    if (tid == 255) {
+        wl[2 * (size_t)blk] = h1[S - 1] + h2[S - 2] + cr;
+        wl[2 * (size_t)blk + 1] = h2[S - 1];
+    }
+}
+
+// Range w's first 8 digits receive the boundary carries of range w - 1 (of the last range, negated,
+// for w = 0: B^M = -1); the pair is cleared once applied.
+__device__ __forceinline__ void carry_limb_body(int32_t *dc, int64_t *wl, unsigned nwg, uint32_t b, unsigned w) {
+    const unsigned src = (w == 0) ? nwg - 1 : w - 1;
+    int64_t e0 = wl[2 * (size_t)src], e1 = wl[2 * (size_t)src + 1];
+    if (e0 == 0 && e1 == 0) return;
+    wl[2 * (size_t)src] = 0;
+    wl[2 * (size_t)src + 1] = 0;
+    if (w == 0) { e0 = -e0; e1 = -e1; }
+    int32_t *p = dc + (size_t)w * 2 * LIMB_RANGE;
+    int32_t e[8];
+    const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);
+    e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w; e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
+    absorb_carry2<8>(e, e0, e1, b, 1.0 / (double)b);
+    *reinterpret_cast<int4 *>(p) = make_int4(e[0], e[1], e[2], e[3]);
+    *reinterpret_cast<int4 *>(p + 4) = make_int4(e[4], e[5], e[6], e[7]);
+}
+
+}  // namespace gfe

@@ -145,9 +145,12 @@ __device__ __forceinline__ uint32_t swap_bits45(uint32_t s) {
     return s ^ (d * 0x30u);
 }
 
+// With MODE == ROW_SQUARE, `mul` (uniform over the workgroup) turns the pointwise square into the
+// product by the multiplicand zp: a multiplying round of a batched power (r0 = r0 rm).
 template <class M, int LOGE, int LOGG, int MODE>
 __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ zp,
-                                         const uint32_t *__restrict__ tw, int a, int NG, unsigned blk, uint32_t *lds) {
+                                         const uint32_t *__restrict__ tw, int a, int NG, unsigned blk, uint32_t *lds,
+                                         bool mul = false) {
     using GM = Geo<LOGG, LOGE>;
     constexpr int E = 1 << LOGE, G = 1 << LOGG, SPR = G >> LOGE, PAD = row_pad(LOGG, LOGE);
     const uint32_t t = threadIdx.x, q = t / SPR, s = t % SPR;
@@ -190,7 +193,7 @@ __device__ __forceinline__ void row_pass(const M &md, uint32_t *__restrict__ dat
     } else {
         load_inv_tw<LOGE>(w, tw, (1u << (a + GM::lam0(LAST))) + (row << GM::lam0(LAST)) + s, a + GM::lam0(LAST),
                           GM::ne(LAST));
-        if constexpr (MODE == ROW_SQUARE) {
+        if (MODE == ROW_SQUARE && !mul) {
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) x[rr] = md.mont(x[rr], x[rr]);
         } else {
@@ -297,12 +300,14 @@ __device__ __forceinline__ bool col_absorb_carries(int32_t (&dg)[1 << LOGE], con
 }
 
 // Forward levels 0..a-1 of one prime: signed digits `src` -> residues `dst` (both N values).  With
-// `wc`, the carries of the last normalisation are added on the way in (col_absorb_carries).
-template <class M, int LOGE, int LOGG>
+// `wc`, the carries of the last normalisation are added on the way in (col_absorb_carries).  LIMB:
+// element i is the limb d_2i + b d_2i+1 of two digits (the two-digit-limb transform, gfe_limb.h),
+// formed mod p with bm = b 2^32 mod p; `wc` must then be null.
+template <class M, int LOGE, int LOGG, bool LIMB = false>
 __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                              const uint32_t *__restrict__ tw, uint32_t C, int NG, unsigned blk,
                                              uint32_t *lds, const int64_t *__restrict__ wc = nullptr,
-                                             uint32_t b = 0) {
+                                             uint32_t b = 0, uint32_t bm = 0) {
     using GM = Geo<LOGG, LOGE>;
     constexpr int E = 1 << LOGE;
     const uint32_t t = threadIdx.x, q = t & (uint32_t)(NG - 1), s = t / (uint32_t)NG;
@@ -315,7 +320,15 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
         constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
         const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
         load_fwd_tw<LOGE>(w, tw, (1u << lam0) + hi, NE);
-        if constexpr (ci == 0) {
+        if constexpr (ci == 0 && LIMB) {
+            const uint32_t off = (base * C + c) * 8u;
+            const int2 *s2 = reinterpret_cast<const int2 *>(src);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) {
+                const int2 v = *at_bytes(s2 + (size_t)((uint32_t)rr << YL) * C, off);
+                x[rr] = md.add(to_res(v.x, md.p), md.mulw(to_res(v.y, md.p), bm));
+            }
+        } else if constexpr (ci == 0) {
             const uint32_t off = (base * C + c) * 4u;
             int32_t dg[E];
 #pragma unroll

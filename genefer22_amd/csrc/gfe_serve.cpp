@@ -30,15 +30,19 @@ double steady_s() {
 
 // Default slots of a pool of n: about 2^26 digits per pool (enough candidates to fill the device
 // with one pool), between 16 and 1024 slots.
-uint32_t default_slots(uint32_t n) {
+uint32_t default_slots(uint32_t n, uint32_t depth = 0) {
     const uint64_t s = (n >= 26) ? 1 : (uint64_t(1) << (26 - n));
-    return (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(16, s));
+    const uint32_t q = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(16, s));
+    // proof pools: a slot holds 3 + 2^depth registers instead of 3; keep the pool's registers near
+    // a quick-test pool's (at least 4 slots)
+    return depth == 0 ? q : std::max<uint32_t>(4, q * 3 / (3 + (1u << depth)));
 }
 
 struct Request {
     uint64_t tag;
     uint32_t n, b;
     uint64_t iters;
+    uint32_t depth;   // Pietrzak-Li proof depth (0: quick test)
     double t_arrival;
 };
 
@@ -50,6 +54,8 @@ struct Admitted {
 struct Pool {
     gfe_batch *bt = nullptr;
     uint32_t n = 0, cap = 0;
+    uint32_t depth = 0;   // proof pools hold 3 + 2^depth registers per slot
+    bool mixed = false;   // a mixed pool (n = 13 allocation) takes the quick tests of n = 12 and 13
     uint64_t bytes = 0;
     uint32_t busy = 0;
 };
@@ -77,14 +83,26 @@ struct gfe_server {
         st.pools = (uint32_t)pools.size();
     }
 
-    // Creates a pool of n with up to `want` slots within the budget, releasing idle pools of
-    // other n when needed.  *idx receives its index, or -1 when the budget has no room.
-    int create_pool(uint32_t n, uint32_t want, long *idx) {
+    // Creates a pool of n (proof depth `depth`) with up to `want` slots within the budget, releasing
+    // idle pools of other kinds when needed.  *idx receives its index, or -1 when the budget has no
+    // room.
+    // Quick tests of n <= 13 share one kind of pool: a mixed pool, whose squaring blocks are one
+    // launch for both n (gfe_batch_create_mixed).
+    static bool mixed_kind(uint32_t n, uint32_t depth) { return depth == 0 && n <= 13; }
+    static bool pool_takes(const Pool &P, uint32_t n, uint32_t depth) {
+        return mixed_kind(n, depth) ? P.mixed : (!P.mixed && P.n == n && P.depth == depth);
+    }
+
+    int create_pool(uint32_t n, uint32_t depth, uint32_t want, long *idx) {
         *idx = -1;
-        auto fits = [&](uint32_t c) { return st.bytes_reserved + gfe_batch_device_bytes(n, c) <= st.bytes_budget; };
+        const bool mixed = mixed_kind(n, depth);
+        if (mixed) n = 13;
+        auto fits = [&](uint32_t c) {
+            return st.bytes_reserved + gfe_batch_proof_device_bytes(n, c, depth) <= st.bytes_budget;
+        };
         if (!fits(1)) {
             for (size_t i = pools.size(); i-- > 0;)
-                if (pools[i].busy == 0 && pools[i].n != n) {
+                if (pools[i].busy == 0 && (pools[i].n != n || pools[i].depth != depth || pools[i].mixed != mixed)) {
                     release(i);
                     if (fits(1)) break;
                 }
@@ -94,9 +112,11 @@ struct gfe_server {
         while (cap > 1 && !fits(cap)) cap = std::max<uint32_t>(1, cap / 2);
         Pool P;
         P.n = n;
+        P.depth = depth;
+        P.mixed = mixed;
         P.cap = cap;
-        P.bytes = gfe_batch_device_bytes(n, cap);
-        const int rc = gfe_batch_create(n, cap, device, &P.bt);
+        P.bytes = gfe_batch_proof_device_bytes(n, cap, depth);
+        const int rc = mixed ? gfe_batch_create_mixed(cap, device, &P.bt) : gfe_batch_create_proof(n, cap, depth, device, &P.bt);
         if (rc != GFE_OK) return rc;
         pools.push_back(P);
         st.bytes_reserved += P.bytes;
@@ -112,16 +132,16 @@ struct gfe_server {
             const Request &rq = *it;
             long pi = -1;
             for (size_t i = 0; i < pools.size(); ++i)
-                if (pools[i].n == rq.n && pools[i].busy < pools[i].cap) { pi = (long)i; break; }
+                if (pool_takes(pools[i], rq.n, rq.depth) && pools[i].busy < pools[i].cap) { pi = (long)i; break; }
             if (pi < 0) {
-                const int rc = create_pool(rq.n, default_slots(rq.n), &pi);
+                const int rc = create_pool(rq.n, rq.depth, default_slots(rq.n, rq.depth), &pi);
                 if (rc != GFE_OK) return rc;
             }
             if (pi < 0) { ++it; continue; }  // waits for memory
             Pool &P = pools[(size_t)pi];
             int32_t slot = -1;
             const uint64_t id = seq++;
-            const int rc = gfe_batch_submit(P.bt, rq.b, rq.iters, id, &slot);
+            const int rc = gfe_batch_submit_n(P.bt, rq.n, rq.b, rq.iters, id, &slot);
             if (rc == GFE_EFULL) { ++it; continue; }
             if (rc != GFE_OK) return rc;
             inflight[id] = Admitted{rq, clock()};
@@ -151,6 +171,8 @@ struct gfe_server {
                 memset(&o, 0, sizeof(o));
                 o.tag = a.rq.tag; o.n = r.n; o.b = r.b; o.res64 = r.res64; o.old64 = r.old64; o.hash64 = r.hash64;
                 o.esize = r.esize; o.iters = r.iters; o.is_prp = r.is_prp; o.gl_ok = r.gl_ok;
+                o.depth = r.depth; o.pkey = r.pkey;
+                memcpy(o.mu_hash64, r.mu_hash64, sizeof(o.mu_hash64));
                 o.t_arrival = a.rq.t_arrival; o.t_admit = a.t_admit; o.t_done = t;
                 done.push_back(o);
                 P.busy--;
@@ -224,19 +246,24 @@ int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity) {
     if (!s) return gfe::set_error(GFE_EINVAL, "null server");
     if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
     long pi = -1;
-    const int rc = s->create_pool(n, capacity ? capacity : default_slots(n), &pi);
+    const int rc = s->create_pool(n, 0, capacity ? capacity : default_slots(n), &pi);
     if (rc != GFE_OK) return rc;
     if (pi < 0) return gfe::set_error(GFE_ENOMEM, "serve: pool does not fit in the budget");
     return GFE_OK;
 }
 
 int gfe_server_submit(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint64_t tag) {
+    return gfe_server_submit_proof(s, n, b, iters, 0, tag);
+}
+
+int gfe_server_submit_proof(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint32_t depth, uint64_t tag) {
     if (!s) return gfe::set_error(GFE_EINVAL, "null server");
     if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
     if (b < 1024 || b > 2000000000u) return gfe::set_error(GFE_EINVAL, "b must be in [1024, 2000000000]");
-    if (gfe_batch_device_bytes(n, 1) > s->st.bytes_budget)
+    if (depth > GFE_BATCH_MAX_DEPTH) return gfe::set_error(GFE_EINVAL, "serve: proof depth out of range");
+    if (gfe_batch_proof_device_bytes(n, 1, depth) > s->st.bytes_budget)
         return gfe::set_error(GFE_ENOMEM, "serve: one candidate of this n exceeds the budget");
-    s->queue.push_back(Request{tag, n, b, iters, s->clock()});
+    s->queue.push_back(Request{tag, n, b, depth ? 0 : iters, depth, s->clock()});
     s->st.submitted++;
     s->st.queued++;
     return GFE_OK;
@@ -266,8 +293,8 @@ int gfe_server_get_stats(const gfe_server *s, gfe_server_stats *out) {
     return GFE_OK;
 }
 
-int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters, const double *arrival,
-                     uint32_t count, int device, uint64_t hbm_budget, const uint32_t *prepare_n,
+int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters, const uint32_t *depth,
+                     const double *arrival, uint32_t count, int device, uint64_t hbm_budget, const uint32_t *prepare_n,
                      uint32_t prepare_count, gfe_serve_result *results, gfe_server_stats *stats) {
     if (!n || !b || !iters || !arrival || !results || count == 0) return gfe::set_error(GFE_EINVAL, "replay arguments");
     for (uint32_t k = 1; k < count; ++k)
@@ -282,7 +309,7 @@ int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters
     while (rc == GFE_OK && finished < count) {
         const double now = s->clock() - start;
         while (next < count && arrival[next] <= now && rc == GFE_OK) {
-            rc = gfe_server_submit(s, n[next], b[next], iters[next], next);
+            rc = gfe_server_submit_proof(s, n[next], b[next], iters[next], depth ? depth[next] : 0, next);
             ++next;
         }
         if (rc != GFE_OK) break;

@@ -15,6 +15,7 @@ typedef void (*SpecBatchRowFn)(BArgs);
 // Null when the shape is not instantiated (spec_col_ok / spec_row_ok).
 SpecColFn spec_col_fwd(int is32, int loge, int logg);
 SpecColFn spec_col_inv(int is32, int loge, int logg);
+SpecColFn spec_col_fwd_limb(int loge, int logg);  // two-digit-limb transform, 31-bit primes
 SpecRowFn spec_row_square(int is32, int loge, int logg);
 SpecBatchColFn spec_batch_col_fwd(int loge, int logg);
 SpecBatchColFn spec_batch_col_inv(int loge, int logg);  // second argument unused

@@ -16,13 +16,13 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kbs_col_fwd(BArgs A, i
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
     const int32_t *src = A.regs + s * A.reg_stride + A.src_reg * N;
-    uint32_t *dst = (to_zp ? A.zp : A.ws) + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    uint32_t *dst = (to_zp ? A.zp : A.ws) + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
     const uint32_t C = (uint32_t)(N >> LOGG);
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-    const int64_t *wc = A.absorb ? A.carry + (size_t)s * (N >> 3) : nullptr;
+    const int64_t *wc = A.absorb ? A.carry + (size_t)s * (NS >> 3) : nullptr;
     if (si.is32)
         col_fwd_pass<ModU, LOGE, LOGG>(ModU{A.prU.p[prime], A.prU.q[prime]}, src, dst, tw, C, A.ng, blk, lds, wc, si.b);
     else
@@ -37,9 +37,9 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) kbs_col_inv(BArgs A, i
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *data = A.ws + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
+    uint32_t *data = A.ws + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
     const uint32_t C = (uint32_t)(N >> LOGG);
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
     if (si.is32) col_inv_pass<ModU, LOGE, LOGG>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, tw, C, A.ng, blk, lds);
@@ -57,12 +57,13 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024, LOGE >= 5 ? 3 : 1) kbs
     const SlotInfo si = A.slots[s];
     const int prime = blockIdx.y;
     if (prime >= si.rns) return;
-    const size_t N = (size_t)1 << A.n;
-    uint32_t *data = A.ws + ((size_t)s * 3 + prime) * N;
-    const uint32_t *zp = A.zp + ((size_t)s * 3 + prime) * N;
-    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * N;
-    if (si.is32) row_pass<ModU, LOGE, LOGG, ROW_SQUARE>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, zp, tw, A.a, A.ng, blockIdx.x, lds);
-    else row_pass<ModS, LOGE, LOGG, ROW_SQUARE>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.a, A.ng, blockIdx.x, lds);
+    const size_t N = (size_t)1 << A.n, NS = (size_t)1 << A.sn;
+    uint32_t *data = A.ws + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *zp = A.zp + (size_t)s * 3 * NS + prime * N;
+    const uint32_t *tw = A.tw + ((size_t)si.is32 * 3 + prime) * NS;
+    const bool mul = A.mulf && A.mulf[s];
+    if (si.is32) row_pass<ModU, LOGE, LOGG, ROW_SQUARE>(ModU{A.prU.p[prime], A.prU.q[prime]}, data, zp, tw, A.a, A.ng, blockIdx.x, lds, mul);
+    else row_pass<ModS, LOGE, LOGG, ROW_SQUARE>(ModS{A.prS.p[prime], A.prS.q[prime]}, data, zp, tw, A.a, A.ng, blockIdx.x, lds, mul);
 }
 
 template <int KIND, int LOGE, int LOGG = SPEC_COL_MIN>

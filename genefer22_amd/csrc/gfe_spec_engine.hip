@@ -17,6 +17,18 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd(ColArgs A) 
                                 xcd_block(blockIdx.x, gridDim.x), lds, A.wc, A.b);
 }
 
+// Forward column pass of the two-digit-limb transform (gfe_limb.h): element i = limb of digits
+// 2i, 2i + 1; grid (blocks, 4 primes), A.n = log2 of the limb count.
+template <int LOGE, int LOGG>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd_limb(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    const int prime = blockIdx.y;
+    const ModS md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t M = (size_t)1 << A.n;
+    col_fwd_pass<ModS, LOGE, LOGG, true>(md, A.src, A.ws + prime * M, A.tw + prime * M, (uint32_t)(M >> LOGG), A.ng,
+                                         xcd_block(blockIdx.x, gridDim.x), lds, nullptr, 0, A.bm[prime]);
+}
+
 template <class M, int LOGE, int LOGG>
 __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_inv(ColArgs A) {
     extern __shared__ uint32_t lds[];
@@ -53,6 +65,19 @@ static SpecColFn col_lookup(int logg) {
     }
 }
 
+template <int LOGE, int LOGG = SPEC_COL_MIN>
+static SpecColFn col_limb_lookup(int logg) {
+    if constexpr (LOGG > SPEC_COL_MAX) {
+        return nullptr;
+    } else {
+        if (logg == LOGG) {
+            if constexpr (LOGG >= LOGE) return ks_col_fwd_limb<LOGE, LOGG>;
+            else return nullptr;
+        }
+        return col_limb_lookup<LOGE, LOGG + 1>(logg);
+    }
+}
+
 template <class M, int LOGE, int LOGG = SPEC_ROW_MIN>
 static SpecRowFn row_lookup(int logg) {
     if constexpr (LOGG > SPEC_ROW_MAX) {
@@ -79,6 +104,15 @@ SpecColFn spec_col_fwd(int is32, int loge, int logg) {
 SpecColFn spec_col_inv(int is32, int loge, int logg) {
     return is32 ? col_fn<1, ModU>(loge, logg) : col_fn<1, ModS>(loge, logg);
 }
+SpecColFn spec_col_fwd_limb(int loge, int logg) {
+    switch (loge) {
+        case 3: return col_limb_lookup<3>(logg);
+        case 4: return col_limb_lookup<4>(logg);
+        case 5: return col_limb_lookup<5>(logg);
+        default: return nullptr;
+    }
+}
+
 SpecRowFn spec_row_square(int is32, int loge, int logg) {
     if (loge == 4) return is32 ? row_lookup<ModU, 4>(logg) : row_lookup<ModS, 4>(logg);
     if (loge == 5) return is32 ? row_lookup<ModU, 5>(logg) : row_lookup<ModS, 5>(logg);
@@ -95,6 +129,8 @@ void spec_set_lds_attributes() {
                     if (SpecColFn f = spec_col_fwd(is32, loge, g))
                         (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                     if (SpecColFn f = spec_col_inv(is32, loge, g))
+                        (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                    if (SpecColFn f = is32 ? nullptr : spec_col_fwd_limb(loge, g))
                         (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                 }
                 for (int g = SPEC_ROW_MIN; g <= SPEC_ROW_MAX; ++g)

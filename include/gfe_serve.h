@@ -5,8 +5,9 @@
  * (src/genefer.h:1363-1573).  Each request runs the reference's PRP work for one candidate
  * (src/genefer.h:479-615): its full quick test, or the first `iters` squarings of its PRP loop.
  *
- * Requests are queued in arrival order.  A request is admitted into a pool (a gfe_batch of its n)
- * that has a free slot; when no pool of its n has one, a new pool is created if its device bytes
+ * Requests are queued in arrival order.  A request is admitted into a pool (a gfe_batch of its n;
+ * for the quick tests of n = 12 and 13 one mixed batch, gfe_batch_create_mixed, that squares both
+ * in one launch per block; for proof requests a proof batch of its n and depth) that has a free slot; when no pool of its n has one, a new pool is created if its device bytes
  * (gfe_batch_device_bytes) fit in the HBM budget, after releasing idle pools of other n if needed.
  * Otherwise the request waits.  Every pool runs on its own stream; a pump enqueues one block of
  * squaring rounds on each pool whose device backlog is below two blocks, so the pools' work
@@ -20,6 +21,8 @@
 
 #include <stddef.h>
 #include <stdint.h>
+
+#include "gfe_batch.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -36,6 +39,10 @@ typedef struct gfe_serve_result {
     double t_arrival;          /* seconds on the server clock: submitted */
     double t_admit;            /* admitted into a pool slot */
     double t_done;             /* result available to the caller */
+    int32_t depth;             /* proof requests: the proof's depth, else 0 */
+    int32_t pad;
+    uint64_t pkey;             /* proof requests: the proof key (as gfe_batch_result) */
+    uint64_t mu_hash64[GFE_BATCH_MAX_DEPTH + 1];  /* gethash64 of mu_0 .. mu_depth */
 } gfe_serve_result;
 
 typedef struct gfe_server_stats {
@@ -55,6 +62,11 @@ double gfe_server_clock(const gfe_server *s);
 int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity);
 /* Queues a request (arrival time = now). */
 int gfe_server_submit(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint64_t tag);
+/* Queues a proof request: the full test and the Pietrzak-Li proof of `depth` (1 ..
+ * GFE_BATCH_MAX_DEPTH; 0 = gfe_server_submit), run in a proof pool of its (n, depth) whose slots
+ * keep the checkpoints in HBM; admitted by that pool's larger footprint
+ * (gfe_batch_proof_device_bytes).  `iters` is ignored for depth > 0. */
+int gfe_server_submit_proof(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint32_t depth, uint64_t tag);
 /* Admits queued requests, then enqueues one block on every pool that has work and a short
  * backlog.  *busy receives the number of admitted, unfinished requests. */
 int gfe_server_pump(gfe_server *s, int32_t *busy);
@@ -62,11 +74,12 @@ int gfe_server_pump(gfe_server *s, int32_t *busy);
 int gfe_server_poll(gfe_server *s, gfe_serve_result *out, uint32_t max, uint32_t *count);
 int gfe_server_get_stats(const gfe_server *s, gfe_server_stats *out);
 
-/* Open-loop replay: request k (n[k], b[k], iters[k]) is submitted when the server clock reaches
- * arrival[k] seconds (non-decreasing); runs until every request has finished.  results[k] is the
+/* Open-loop replay: request k (n[k], b[k], iters[k], proof depth depth[k] -- depth may be null:
+ * no proofs) is submitted when the server clock reaches arrival[k] seconds (non-decreasing); runs
+ * until every request has finished.  results[k] is the
  * result of request k; stats (optional) receives the final statistics. */
-int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters, const double *arrival,
-                     uint32_t count, int device, uint64_t hbm_budget, const uint32_t *prepare_n,
+int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters, const uint32_t *depth,
+                     const double *arrival, uint32_t count, int device, uint64_t hbm_budget, const uint32_t *prepare_n,
                      uint32_t prepare_count, gfe_serve_result *results, gfe_server_stats *stats);
 
 #ifdef __cplusplus

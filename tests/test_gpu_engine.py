@@ -257,7 +257,7 @@ def test_largest_n_from_full_size_digits(gfe, oracle, b, n):
 def test_set_rejects_values_beyond_the_balanced_bound(gfe, oracle):
     """Registers hold balanced digits: set(a) accepts a <= b/2 + 4 and refuses larger a (a value
     that would wrap int32 or exceed a 31-bit prime's residues would otherwise come back wrong)."""
-    b, n = 1000, 12
+    b, n = 1026, 12
     t = gfe.Transform(b, n)
     t.set(b // 2 + 4)
     d = t.get_digits(0)
