@@ -99,7 +99,7 @@ struct Engine {
     // forward pass adds them on the way in (plans with compile-time passes); every other use of the
     // registers runs the carry kernel first (flush_carry).
     bool carry_pending = false;
-    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused && !plan.fuse_sq && !plan.limb; }
+    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused && !plan.fuse_sq; }
     void flush_carry();
 
     ~Engine();
@@ -116,7 +116,9 @@ struct Engine {
     void launch_col_inv(int dup);
     void launch_row(int mode, uint32_t *data);
     void launch_norm(int dup);
-    void launch_carry();
+    void launch_carry();          // the pending carries of the current plan's normalisation
+    void launch_carry_digits();   // workgroup carries of k_norm / k_mulscalar
+    void launch_carry_limbs();    // boundary carries of k_norm_limb
     void square(int dup);
     void launch_fused(int dup);
     void mul();

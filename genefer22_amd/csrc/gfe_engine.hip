@@ -769,6 +769,10 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
         A.src = regs + (size_t)src * N; A.ws = dst; A.tw = tw_limb; A.pr = pr4;
         A.n = tn(); A.a = plan.a; A.ng = plan.col_ng;
         for (int k = 0; k < 4; ++k) A.bm[k] = bm4[k];
+        if (absorb) {  // the caller overwrites register 0 (square, mul)
+            A.wc = carry; A.b = b;
+            carry_pending = false;
+        }
         const uint32_t C = 1u << (tn() - plan.a);
         const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
         hipLaunchKernelGGL(spec_col_fwd_limb(plan.col_loge, plan.a), dim3(C / plan.col_ng, 4, 1), dim3(T),
@@ -885,7 +889,8 @@ void Engine::launch_norm(int dup) {
         A.M = N / 2; A.dup = dup;
         const unsigned nwg = (unsigned)(A.M / LIMB_RANGE);
         hipLaunchKernelGGL(k_norm_limb, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
-        hipLaunchKernelGGL(k_carry_limb, dim3((nwg + 255) / 256, 1, 1), dim3(256), 0, stream, regs, carry, nwg, b);
+        if (defer_carry()) carry_pending = true;
+        else launch_carry_limbs();
         return;
     }
     if (plan.fused) {
@@ -915,6 +920,16 @@ void Engine::flush_carry() {
 }
 
 void Engine::launch_carry() {
+    if (plan.limb) launch_carry_limbs();
+    else launch_carry_digits();
+}
+
+void Engine::launch_carry_limbs() {
+    const unsigned nwg = (unsigned)(N / 2 / LIMB_RANGE);
+    hipLaunchKernelGGL(k_carry_limb, dim3((nwg + 255) / 256, 1, 1), dim3(256), 0, stream, regs, carry, nwg, b);
+}
+
+void Engine::launch_carry_digits() {
     CarryArgs A;
     A.dst = regs; A.carry = carry; A.b = b;
     A.nwg = (unsigned)(N / (8 * 256));
@@ -960,7 +975,7 @@ void Engine::mul_scalar(int32_t a) {
     A.dst = regs; A.carry = carry; A.b = b; A.a = a;
     const unsigned nwg = (unsigned)(N / (8 * 256));
     hipLaunchKernelGGL(k_mulscalar, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
-    launch_carry();
+    launch_carry_digits();
 }
 
 void Engine::copy(uint32_t dst, uint32_t src) {

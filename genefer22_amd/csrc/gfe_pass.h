@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "gfe_kernels.h"
+#include "gfe_limb.h"
 
 namespace gfe {
 
@@ -299,10 +300,52 @@ __device__ __forceinline__ bool col_absorb_carries(int32_t (&dg)[1 << LOGE], con
     return true;
 }
 
+// As col_absorb_carries for the two-digit-limb transform (gfe_limb.h): range g = i / 1024 of the
+// limbs starts at limb 1024 g, and the boundary carries of range g - 1 (wl[2 (g-1)], wl[2 (g-1) + 1],
+// the last range's negated for g = 0: B^M = -1) enter its first 8 digits (carry_limb_body's work),
+// i.e. the limbs of columns c0..c0+3 of that row, held by lanes q < 4 as digit pairs d0, d1.
+template <int LOGE, int LOGG>
+__device__ __forceinline__ bool col_absorb_limb_carries(int32_t (&d0)[1 << LOGE], int32_t (&d1)[1 << LOGE],
+                                                        const int32_t *__restrict__ src, const int64_t *__restrict__ wl,
+                                                        uint32_t b, uint32_t C, uint32_t c0, uint32_t q, uint32_t base,
+                                                        int YL, uint32_t nrange, uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    const bool starts = (C >= LIMB_RANGE) ? (c0 % LIMB_RANGE) == 0 : c0 == 0;
+    if (!starts) return false;  // uniform over the workgroup
+    const uint32_t every = (C >= LIMB_RANGE) ? 1u : LIMB_RANGE / C;
+    int32_t *P = reinterpret_cast<int32_t *>(lds);
+    const double binv = 1.0 / (double)b;
+    for (uint32_t r = threadIdx.x; r < (1u << LOGG); r += blockDim.x) {
+        if (r % every) continue;
+        const uint32_t g = (uint32_t)(((size_t)r * C + c0) / LIMB_RANGE);
+        const size_t w = 2 * (size_t)((g == 0) ? nrange - 1 : g - 1);
+        const int64_t e0 = (g == 0) ? -wl[w] : wl[w], e1 = (g == 0) ? -wl[w + 1] : wl[w + 1];
+        const int32_t *p = src + 2 * ((size_t)r * C + c0);
+        const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);
+        int32_t e8[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        if (e0 != 0 || e1 != 0) absorb_carry2<8>(e8, e0, e1, b, binv);
+        *reinterpret_cast<int4 *>(P + r * 8) = make_int4(e8[0], e8[1], e8[2], e8[3]);
+        *reinterpret_cast<int4 *>(P + r * 8 + 4) = make_int4(e8[4], e8[5], e8[6], e8[7]);
+    }
+    __syncthreads();
+    if (q < 4) {
+#pragma unroll
+        for (int rr = 0; rr < E; ++rr) {
+            const uint32_t r = base + ((uint32_t)rr << YL);
+            if (r % every == 0) {
+                d0[rr] = P[r * 8 + 2 * q];
+                d1[rr] = P[r * 8 + 2 * q + 1];
+            }
+        }
+    }
+    return true;
+}
+
 // Forward levels 0..a-1 of one prime: signed digits `src` -> residues `dst` (both N values).  With
 // `wc`, the carries of the last normalisation are added on the way in (col_absorb_carries).  LIMB:
 // element i is the limb d_2i + b d_2i+1 of two digits (the two-digit-limb transform, gfe_limb.h),
-// formed mod p with bm = b 2^32 mod p; `wc` must then be null.
+// formed mod p with bm = b 2^32 mod p; `wc` then holds the limb normalisation's boundary carries
+// (col_absorb_limb_carries).
 template <class M, int LOGE, int LOGG, bool LIMB = false>
 __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                              const uint32_t *__restrict__ tw, uint32_t C, int NG, unsigned blk,
@@ -323,11 +366,18 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
         if constexpr (ci == 0 && LIMB) {
             const uint32_t off = (base * C + c) * 8u;
             const int2 *s2 = reinterpret_cast<const int2 *>(src);
+            int32_t d0[E], d1[E];
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) {
                 const int2 v = *at_bytes(s2 + (size_t)((uint32_t)rr << YL) * C, off);
-                x[rr] = md.add(to_res(v.x, md.p), md.mulw(to_res(v.y, md.p), bm));
+                d0[rr] = v.x;
+                d1[rr] = v.y;
             }
+            if (wc)
+                patched = col_absorb_limb_carries<LOGE, LOGG>(d0, d1, src, wc, b, C, blk * (uint32_t)NG, q, base, YL,
+                                                              (uint32_t)(((size_t)C << LOGG) / LIMB_RANGE), lds);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = md.add(to_res(d0[rr], md.p), md.mulw(to_res(d1[rr], md.p), bm));
         } else if constexpr (ci == 0) {
             const uint32_t off = (base * C + c) * 4u;
             int32_t dg[E];

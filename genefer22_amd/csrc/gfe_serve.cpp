@@ -245,6 +245,8 @@ double gfe_server_clock(const gfe_server *s) { return s ? s->clock() : 0.0; }
 int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity) {
     if (!s) return gfe::set_error(GFE_EINVAL, "null server");
     if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
+    for (const Pool &P : s->pools)
+        if (gfe_server::pool_takes(P, n, 0)) return GFE_OK;  // n <= 13: the mixed pool already serves n
     long pi = -1;
     const int rc = s->create_pool(n, 0, capacity ? capacity : default_slots(n), &pi);
     if (rc != GFE_OK) return rc;

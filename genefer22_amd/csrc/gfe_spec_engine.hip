@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd_limb(ColArg
     const ModS md{A.pr.p[prime], A.pr.q[prime]};
     const size_t M = (size_t)1 << A.n;
     col_fwd_pass<ModS, LOGE, LOGG, true>(md, A.src, A.ws + prime * M, A.tw + prime * M, (uint32_t)(M >> LOGG), A.ng,
-                                         xcd_block(blockIdx.x, gridDim.x), lds, nullptr, 0, A.bm[prime]);
+                                         xcd_block(blockIdx.x, gridDim.x), lds, A.wc, A.b, A.bm[prime]);
 }
 
 template <class M, int LOGE, int LOGG>

@@ -58,7 +58,8 @@ int gfe_server_create(int device, uint64_t hbm_budget, gfe_server **out);
 int gfe_server_destroy(gfe_server *s);
 /* Seconds since the server was created (steady clock). */
 double gfe_server_clock(const gfe_server *s);
-/* Creates a pool for n ahead of its first request (`capacity` slots, 0 = default for n). */
+/* Creates a pool for n ahead of its first request (`capacity` slots, 0 = default for n); for
+ * n <= 13 the mixed pool, unless one exists. */
 int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity);
 /* Queues a request (arrival time = now). */
 int gfe_server_submit(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint64_t tag);
