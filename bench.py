@@ -21,7 +21,8 @@ One JSON line on stdout (progress on stderr):
   * serving    : p50/p99 time-to-result of seeded Poisson requests of mixed n (12..20) through the
                  serving loop, and candidates/hour (each truncated to its first 4096 squarings)
                  against the serial plain path: the 2000-request C5 workload at about 80 % of capacity
-                 (serving) and 100 requests at a light rate (serving_light: latency near service time);
+                 (serving) and 40 requests at 2/s (serving_light: requests seldom overlap, so
+                 time-to-result is a lone request's service time in the batched path);
   * parity     : RES64 after P PRP iterations on the GPU vs the reference's own CPU transform;
   * cpu_baseline: the reference's CPU transform (compiled from its sources into oracle/_ref) timed on
                  this host's cores on a bounded sample of the same workload;
@@ -398,8 +399,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000, help="requests of the C5 serving replay")
     ap.add_argument("--serve-rate", type=float, default=24.0,
                     help="arrivals/s of the C5 replay (about 80 %% of the serving loop's capacity for this mix)")
-    ap.add_argument("--light-requests", type=int, default=100, help="requests of the light-load serving replay")
-    ap.add_argument("--light-rate", type=float, default=8.0, help="arrivals/s of the light-load serving replay")
+    ap.add_argument("--light-requests", type=int, default=40, help="requests of the light-load serving replay")
+    ap.add_argument("--light-rate", type=float, default=2.0,
+                    help="arrivals/s of the light-load serving replay (about a fifth of one plain engine's capacity "
+                         "for this mix, so that requests seldom overlap and time-to-result shows a lone request's)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--job-iters", type=int, default=100_000,
                     help="squarings of the timed n=22 PRP job with Gerbicz-Li updates every 2000 (0: skip)")

@@ -941,6 +941,7 @@ void Engine::launch_fused(int dup) {
     memset(&A, 0, sizeof(A));
     A.regs = regs; A.tw = tw; A.crt = crt; A.slots = nullptr; A.act = nullptr; A.dupf = nullptr;
     A.rounds = 1; A.dup0 = dup; A.b = b; A.rns = rns; A.is32 = is32; A.prS = pr; A.prU = pr;
+    A.tws = (uint32_t)N;  // the primes' tables lie N entries apart
     hipLaunchKernelGGL(fused_square_fn((int)n), dim3(1), dim3(fused_threads((int)n)), fused_lds_bytes((int)n), stream, A);
 }
 

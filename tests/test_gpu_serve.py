@@ -30,8 +30,9 @@ def test_serve_replay_matches_reference_fixtures(gfe):
 
 
 def test_serve_alone_equals_mix_under_a_small_budget(gfe):
-    """With a budget that holds one small pool at a time, requests wait for memory, idle pools are
-    released for other n, and every result equals the candidate's run alone."""
+    """With a budget that holds one small pool at a time, requests wait for memory, an idle pool is
+    released for another n (the quick tests of n = 12 and 13 share one mixed pool, so the n = 14
+    requests need at least one release), and every result equals the candidate's run alone."""
     mix = [(12, 1000000000), (13, 1534), (12, 30000002), (14, 1999999266), (13, 777777778), (12, 4098),
            (14, 100000002), (12, 1999999990)]
     iters = 700
@@ -39,7 +40,7 @@ def test_serve_alone_equals_mix_under_a_small_budget(gfe):
     reqs = [(n, b, iters, 0.002 * k) for k, (n, b) in enumerate(mix)]
     results, stats = gfe.serve_replay(reqs, hbm_budget=budget)
     assert stats["bytes_peak"] <= budget
-    assert stats["pools_released"] >= 2
+    assert stats["pools_released"] >= 1
     for (n, b), g in zip(mix, results):
         alone = gfe.run_prp(b, n, iters=iters, b_gl=-1)
         assert (g["res64"], g["old64"], g["hash64"]) == (alone["res64"], alone["old64"], alone["hash64"]), (n, b)
