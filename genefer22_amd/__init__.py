@@ -100,6 +100,8 @@ ABI = [
     ("gfe_engine_plan", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, I32P]),
     ("gfe_candidate_plans", ctypes.c_int, [ctypes.c_uint32, I32P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("gfe_host_twiddles", ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U32P]),
+    ("gfe_square_noalloc", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
+                                          ctypes.c_uint64, ctypes.c_void_p]),
     ("gfe_run_prp_resumable", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
                                              ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(Result)]),
     ("gfe_run_proof", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U64P, ctypes.POINTER(Result)]),

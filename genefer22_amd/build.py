@@ -21,7 +21,7 @@ GMP_LIB = os.environ.get("GFE_GMP_LIB", "/usr/lib/x86_64-linux-gnu/libgmp.so.10"
 GMP_LINK = ["-L" + os.path.dirname(GMP_LIB), "-l:" + os.path.basename(GMP_LIB)]
 
 SOURCES = ["gfe_engine.hip", "gfe_batch.hip", "gfe_spec_engine.hip", "gfe_spec_batch.hip", "gfe_fused.hip", "genefer_driver.cpp",
-           "gfe_serve.cpp"]
+           "gfe_serve.cpp", "gfe_noalloc.hip"]
 OBJDIR = os.path.join(HERE, "_build")
 
 FLAGS = [

@@ -133,6 +133,26 @@ int gfe_candidate_plans(uint32_t n, int32_t *plans, uint32_t max, uint32_t *coun
  * threads (0: on the calling thread only); the result does not depend on it. */
 int gfe_host_twiddles(uint32_t n, int is32, int max_threads, uint32_t *out);
 
+/* No counterpart in the reference: squarings that allocate nothing, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * Squares the N = 2^n balanced digits at `digits` -- device memory of `device`, read and written in
+ * place, digits as gfe_get_digits returns them (|d| <= b/2 + 4) -- `count` times, squaring i
+ * doubled when bit i mod 64 of dup_bits is set, queued on `stream` (a hipStream_t; NULL = the null
+ * stream), and returns without waiting.  The squarings run the digit transform's default plan
+ * (gfe_default_plan(n, 0)): the digits equal those of `count` gfe_square_dup calls on an engine of
+ * the same (b, n) on that plan followed by gfe_get_digits, and their value equals the engine's on
+ * any plan (where an engine starts on the two-digit-limb transform, its balanced digits may differ
+ * from these while the residue is the same).  The transform work space, twiddles,
+ * carries and CRT constants live in one static device array of GFE_NOALLOC_BYTES that the library
+ * holds on every device it runs on until the process ends, which limits n to
+ * 12 <= n <= GFE_NOALLOC_MAX_N (GFE_EINVAL beyond); no device or host memory is allocated or freed.
+ * Every call rebuilds the twiddles and constants of its (b, n) in that array, so a call does not
+ * depend on an earlier one; calls that overlap on one device race on the array. */
+#define GFE_NOALLOC_MAX_N 16
+#define GFE_NOALLOC_BYTES 1638656 /* 2 x 3 x 2^16 x 4 (work space, twiddles) + 2^16 (carries) + 256 */
+int gfe_square_noalloc(uint32_t b, uint32_t n, int device, int32_t *digits, uint32_t count, uint64_t dup_bits,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif
