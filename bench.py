@@ -57,6 +57,8 @@ VALU_PEAK_LANE_INSTR = 256 * 64 * 2.4e9
 MEASURED = {
     ("row", 8, 5, 1, 8): {"kernel": "gfe::ks_row_sq<gfe::ModS, 5, 14>", "fetch_mib": 144.9, "write_mib": 48.0,
                           "valu_instr_per_butterfly": 11.9, "profile": "profiles/n22_spec_plan_kernels.md"},
+    ("row", 7, 5, 1, 40): {"kernel": "gfe::ks_row_sq<gfe::ModS, 5, 14>", "fetch_mib": 96.8, "write_mib": 32.0,
+                           "valu_instr_per_butterfly": 11.92, "profile": "profiles/n22_limb_plan_kernels.md"},
 }
 
 
