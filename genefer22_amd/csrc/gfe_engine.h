@@ -58,6 +58,11 @@ int carry_rounds(uint32_t b, uint32_t n);
 // b >= 2^23 (B = b^2 >= 2^46) and N l_max^2 < P/4 with l_max = (b/2 + 4)(b + 1), P the product of
 // the four primes.
 bool limb_ok(uint32_t b, uint32_t n);
+// The limb transform with its inverse column pass fused to the normalisation (flags 2 | 8 | 32):
+// the range-start limbs enter the next transform unnormalised, so its bound counts them larger.
+bool limb_fused_ok(uint32_t b, uint32_t n, int ng);
+// The plan an engine of (b, n) starts with (gfe_engine_plan).
+Plan engine_plan_for(uint32_t b, uint32_t n);
 // Plans of the limb transform the autotuner times (n: digits exponent).
 std::vector<Plan> candidate_limb_plans(uint32_t n);
 Plan default_limb_plan(uint32_t n);
@@ -99,7 +104,10 @@ struct Engine {
     // forward pass adds them on the way in (plans with compile-time passes); every other use of the
     // registers runs the carry kernel first (flush_carry).
     bool carry_pending = false;
-    bool defer_carry() const { return plan.spec && !plan.pipe && !plan.fused && !plan.fuse_sq; }
+    // what the pending carries are: 0 k_norm's workgroup carries, 1 k_norm_limb's boundary pairs,
+    // 2 the range slots of ks_col_inv_norm_limb (ranges of 2^pend_rlog limbs)
+    int pend_kind = 0, pend_rlog = 0;
+    bool defer_carry() const { return plan.spec && !plan.pipe && (!plan.fused || plan.limb) && !plan.fuse_sq; }
     void flush_carry();
 
     ~Engine();
@@ -119,6 +127,7 @@ struct Engine {
     void launch_carry();          // the pending carries of the current plan's normalisation
     void launch_carry_digits();   // workgroup carries of k_norm / k_mulscalar
     void launch_carry_limbs();    // boundary carries of k_norm_limb
+    void launch_carry_slots();    // range carries of ks_col_inv_norm_limb
     void square(int dup);
     void launch_fused(int dup);
     void mul();

@@ -67,6 +67,12 @@ __global__ void __launch_bounds__(256) k_carry_limb(int32_t *dst, int64_t *wl, u
     if (w < nwg) carry_limb_body(dst, wl, nwg, b, w);
 }
 
+__global__ void __launch_bounds__(256) k_carry_limb_slots(int32_t *dst, const int4 *slots, unsigned nrange, int rlog,
+                                                          uint32_t b) {
+    const unsigned w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < nrange) carry_limb_slot_body(dst, slots, rlog, b, w);
+}
+
 struct FoldArgs {
     int32_t *dst;
     const int64_t *seg;
@@ -381,12 +387,46 @@ Plan default_limb_plan(uint32_t n) {
     return P;
 }
 
+// Fused plans leave the range carries of the last normalisation in the first two limbs of each
+// range of ng limbs (col_fwd_pass): limb 0's digits take (u0, u1) with |u0|, |u1| <= b/2, limb 1's
+// low digit u2 + e1 with |u2| <= 2 and |e1| <= h2max = P / (4 B^2) + 1.  A coefficient of the
+// doubled product is at most 2 sum_i |l_i|^2 (Cauchy-Schwarz), which over M / ng ranges is
+// 2 (M / ng) (L0^2 + L1^2 + (ng - 2) L^2) with L the balanced limb, L0 = (b + 4)(b + 1) and
+// L1 = L + h2max + 2; the explicit CRT needs it below P / 4.
+bool limb_fused_ok(uint32_t b, uint32_t n, int ng) {
+    if (!limb_ok(b, n) || ng < 8) return false;
+    unsigned __int128 P = 1;
+    for (int k = 0; k < 4; ++k) P *= prime_of(LIMB_SET, k);
+    const unsigned __int128 B = (unsigned __int128)b * b;
+    const unsigned __int128 h2max = P / (4 * B * B) + 1;
+    const unsigned __int128 L = ((unsigned __int128)(b / 2) + 4) * ((unsigned __int128)b + 1);
+    const unsigned __int128 L0 = ((unsigned __int128)b + 4) * ((unsigned __int128)b + 1);
+    const unsigned __int128 L1 = L + h2max + 2;
+    const unsigned __int128 per_range = L0 * L0 + L1 * L1 + (unsigned __int128)(ng - 2) * L * L;
+    const unsigned __int128 ranges = ((unsigned __int128)1 << (n - 1)) / (unsigned)ng;
+    return 2 * ranges * per_range < P / 4;
+}
+
+// An engine's first plan: the limb transform where it is exact (with the inverse column pass and
+// normalisation fused where that bound holds), else the digit transform's default.
+Plan engine_plan_for(uint32_t b, uint32_t n) {
+    Plan P = default_plan(n);
+    if (!limb_ok(b, n)) return P;
+    const Plan L = default_limb_plan(n);
+    if (!L.limb) return P;
+    Plan F = L;
+    F.fused = 1;
+    return (plan_valid(F, n) && limb_fused_ok(b, n, F.col_ng)) ? F : L;
+}
+
 std::vector<Plan> candidate_limb_plans(uint32_t n) {
     std::vector<Plan> out;
     if (n < 12) return out;
     for (Plan P : candidate_plans(n - 1)) {
         if (!P.spec || P.fuse_sq) continue;
         P.limb = 1;
+        if (plan_valid(P, n)) out.push_back(P);
+        P.fused = 1;
         if (plan_valid(P, n)) out.push_back(P);
     }
     return out;
@@ -411,10 +451,12 @@ static Plan runtime_default_plan(uint32_t n) {
 
 bool plan_valid(const Plan &P, uint32_t n) {
     if (P.limb) {  // compile-time passes of the N/2-point transform
-        if (!P.spec || P.fuse_sq || n < 12) return false;
+        if (!P.spec || P.fuse_sq || P.pipe || P.row_tw || n < 12) return false;
         Plan Q = P;
         Q.limb = 0;
-        return plan_valid(Q, n - 1);
+        Q.fused = 0;
+        if (!plan_valid(Q, n - 1)) return false;
+        return !P.fused || limb_col_norm_ok(P.a, P.col_loge, P.col_ng);
     }
     if (P.fuse_sq) {
         if ((int)n < FUSED_MIN_N || (int)n > FUSED_MAX_N || P.fused || P.pipe) return false;
@@ -497,7 +539,8 @@ void plan_to_array(const Plan &P, int32_t out[6]) {
 bool plan_from_array(const int32_t in[6], uint32_t n, Plan *out) {
     Plan P;
     const int fl = in[5] & 15;
-    if (in[5] < 0 || in[5] > 63 || (fl & 6) == 6 || ((fl & 8) && fl != 8)) return false;
+    const bool limb_fused = (in[5] & 32) && fl == 10;  // the limb transform's fused inverse columns
+    if (in[5] < 0 || in[5] > 63 || (fl & 6) == 6 || ((fl & 8) && fl != 8 && !limb_fused)) return false;
     P.a = in[0]; P.col_loge = in[1]; P.col_ng = in[2]; P.row_loge = in[3]; P.row_ng = in[4];
     P.row_tw = fl & 1; P.fused = (fl >> 1) & 1; P.pipe = (fl >> 2) & 1; P.spec = (fl >> 3) & 1;
     P.fuse_sq = (in[5] >> 4) & 1;
@@ -517,7 +560,12 @@ std::string plan_str(const Plan &P, uint32_t n) {
     if (P.limb) ss << "limbs of two digits (2^" << tn << " limbs, four 31-bit primes); ";
     ss << "cols 2^" << P.a << " (E=" << (1 << P.col_loge) << ", " << P.col_ng << "/wg) x rows 2^" << (tn - P.a)
        << " (E=" << (1 << P.row_loge) << ", " << P.row_ng << "/wg" << (P.row_tw ? ", factored twiddles" : "")
-       << "), " << (P.fused ? "all-prime columns with CRT + carry" : P.limb ? "norm 4 limbs/thread" : "norm 8 digits/thread")
+       << "), "
+       << (P.limb && P.fused ? "inverse columns fused with the normalisation (ranges of " + std::to_string(P.col_ng) +
+                                   " limbs, " + std::to_string((1 << P.col_loge) / 4) + " limbs/thread)"
+           : P.fused       ? std::string("all-prime columns with CRT + carry")
+           : P.limb        ? std::string("norm 4 limbs/thread")
+                           : std::string("norm 8 digits/thread"))
        << (P.pipe ? ", pipelined columns and norm" : "") << (P.spec ? ", compile-time passes" : "")
        << (P.fuse_sq ? ", squaring in one workgroup" : "");
     return ss.str();
@@ -715,7 +763,7 @@ int Engine::init(uint32_t b_, uint32_t n_, uint32_t num_regs_, int device_) {
             pr4.q[k] = pinv32(pr4.p[k]);
             bm4[k] = mulw_const(b % pr4.p[k], pr4.p[k]);
         }
-        plan = default_limb_plan(n);
+        plan = engine_plan_for(b, n);
     }
     if (hipMemsetAsync(regs, 0, (size_t)num_regs * N * 4, stream) != hipSuccess) return set_error(GFE_EHIP, "memset");
     for (int i = 0; i < 12; ++i) { hipEvent_t ev; if (hipEventCreate(&ev) != hipSuccess) return set_error(GFE_EHIP, "event"); events.push_back(ev); }
@@ -762,7 +810,10 @@ unsigned Engine::resident_blocks(const void *fn, int threads, size_t lds) {
 
 // Forward column pass from register `src` into `dst` (ws or zp).
 void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
-    const bool absorb = carry_pending && src == 0 && defer_carry();
+    // the pending carries enter this pass when it reads register 0 for a squaring or product and
+    // knows their form (the digit transform: k_norm's; the limb transform: either limb form)
+    const bool absorb = carry_pending && src == 0 && defer_carry() && (pend_kind != 0) == (plan.limb != 0) &&
+                        (pend_kind != 2 || plan.col_ng == (1 << pend_rlog));
     if (carry_pending && !absorb) flush_carry();
     if (plan.limb) {
         ColArgs A;
@@ -770,7 +821,13 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
         A.n = tn(); A.a = plan.a; A.ng = plan.col_ng;
         for (int k = 0; k < 4; ++k) A.bm[k] = bm4[k];
         if (absorb) {  // the caller overwrites register 0 (square, mul)
-            A.wc = carry; A.b = b;
+            if (pend_kind == 2) {
+                A.ls = reinterpret_cast<const int4 *>(carry);
+                A.rlog = pend_rlog;
+            } else {
+                A.wc = carry;
+            }
+            A.b = b;
             carry_pending = false;
         }
         const uint32_t C = 1u << (tn() - plan.a);
@@ -825,6 +882,18 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
 
 // Inverse column pass; with a fused plan it also normalises register 0 (dup: doubled).
 void Engine::launch_col_inv(int dup) {
+    if (plan.spec && plan.limb && plan.fused) {  // with the normalisation: register 0 and the range carries
+        LimbColNormArgs A;
+        A.ws = ws; A.tw = tw_limb; A.pr = pr4; A.dst = regs; A.slots = reinterpret_cast<int4 *>(carry);
+        A.crt = static_cast<const LimbCrt *>(limb_crt);
+        A.n = tn(); A.a = plan.a; A.ng = plan.col_ng; A.dup = dup;
+        const uint32_t C = 1u << (tn() - plan.a);
+        const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        hipLaunchKernelGGL(spec_col_inv_norm_limb(plan.col_loge, plan.a), dim3(C / plan.col_ng, 1, 1), dim3(4 * T),
+                           limb_col_norm_lds_bytes(plan.a, plan.col_loge, plan.col_ng), stream, A);
+        carry_pending = false;  // register 0 was rewritten; launch_norm records its carries
+        return;
+    }
     if (plan.spec) {
         ColArgs A;
         A.src = nullptr; A.ws = ws; A.tw = cur_tw(); A.pr = cur_pr();
@@ -883,12 +952,19 @@ void Engine::launch_row(int mode, uint32_t *data) {
 
 // Normalisation of register 0 (fused plan: the cross-segment carries of k_col_inv_norm).
 void Engine::launch_norm(int dup) {
+    if (plan.limb && plan.fused) {  // done by the inverse column pass: its range carries are pending
+        pend_kind = 2;
+        pend_rlog = __builtin_ctz((unsigned)plan.col_ng);
+        carry_pending = true;
+        return;
+    }
     if (plan.limb) {
         LimbNormArgs A;
         A.ws = ws; A.dst = regs; A.wl = carry; A.crt = static_cast<const LimbCrt *>(limb_crt);
         A.M = N / 2; A.dup = dup;
         const unsigned nwg = (unsigned)(A.M / LIMB_RANGE);
         hipLaunchKernelGGL(k_norm_limb, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
+        pend_kind = 1;
         if (defer_carry()) carry_pending = true;
         else launch_carry_limbs();
         return;
@@ -909,6 +985,7 @@ void Engine::launch_norm(int dup) {
     } else {
         hipLaunchKernelGGL(K.norm, dim3(nwg, 1, 1), dim3(256), 0, stream, A);
     }
+    pend_kind = 0;
     if (defer_carry()) carry_pending = true;
     else launch_carry();
 }
@@ -920,8 +997,15 @@ void Engine::flush_carry() {
 }
 
 void Engine::launch_carry() {
-    if (plan.limb) launch_carry_limbs();
+    if (pend_kind == 2) launch_carry_slots();
+    else if (pend_kind == 1) launch_carry_limbs();
     else launch_carry_digits();
+}
+
+void Engine::launch_carry_slots() {
+    const unsigned nrange = (unsigned)((N / 2) >> pend_rlog);
+    hipLaunchKernelGGL(k_carry_limb_slots, dim3((nrange + 255) / 256, 1, 1), dim3(256), 0, stream, regs,
+                       reinterpret_cast<const int4 *>(carry), nrange, pend_rlog, b);
 }
 
 void Engine::launch_carry_limbs() {
@@ -1288,6 +1372,17 @@ int gfe_time_squarings(gfe_engine *e, uint32_t count, uint64_t dup_bits, double 
         int rc = GFE_OK;
         if (g.plan.fuse_sq) {
             rc = time_k([&]() { g.launch_fused(0); }, &kernel_ms[0]);
+        } else if (g.plan.limb && g.plan.fused) {
+            // the fused inverse column pass reads the row pass's output and writes register 0, so it is
+            // timed first, on the product of one squaring; the forward passes then only read register 0
+            g.flush_carry();
+            g.launch_col_fwd(0, g.ws);
+            g.launch_row(gfe::ROW_SQUARE, g.ws);
+            rc = time_k([&]() { g.launch_col_inv(0); }, &kernel_ms[2]);
+            g.launch_norm(0);
+            g.flush_carry();
+            if (rc == GFE_OK) rc = time_k([&]() { g.launch_col_fwd(0, g.ws); }, &kernel_ms[0]);
+            if (rc == GFE_OK) rc = time_k([&]() { g.launch_row(gfe::ROW_SQUARE, g.ws); }, &kernel_ms[1]);
         } else {
             g.flush_carry();
             g.launch_col_fwd(0, g.ws);
@@ -1323,7 +1418,8 @@ int gfe_tune(gfe_engine *e, uint32_t reps) {
     }
     std::vector<gfe::Plan> plans = gfe::candidate_plans(g.n);
     if (g.can_limb)
-        for (const gfe::Plan &P : gfe::candidate_limb_plans(g.n)) plans.push_back(P);
+        for (const gfe::Plan &P : gfe::candidate_limb_plans(g.n))
+            if (!P.fused || gfe::limb_fused_ok(g.b, g.n, P.col_ng)) plans.push_back(P);
     const gfe::Plan orig = g.plan;
     // ms for `count` squarings under plan P (after one untimed), < 0 on a launch error
     auto time_plan = [&](const gfe::Plan &P, uint32_t count, double *ms) -> int {
@@ -1379,12 +1475,7 @@ int gfe_default_plan(uint32_t n, uint32_t capacity, int32_t plan[6]) {
 
 int gfe_engine_plan(uint32_t b, uint32_t n, int32_t plan[6]) {
     if (!plan || n < 12 || n > 23 || b < 1024 || b > 2000000000u) return set_error(GFE_EINVAL, "engine_plan");
-    gfe::Plan P = gfe::default_plan(n);
-    if (gfe::limb_ok(b, n)) {
-        const gfe::Plan L = gfe::default_limb_plan(n);
-        if (L.limb) P = L;
-    }
-    gfe::plan_to_array(P, plan);
+    gfe::plan_to_array(gfe::engine_plan_for(b, n), plan);
     return GFE_OK;
 }
 
@@ -1418,6 +1509,8 @@ int gfe_set_plan(gfe_engine *e, const int32_t plan[6]) {
     gfe::Plan P;
     if (!gfe::plan_from_array(plan, e->eng.n, &P)) return set_error(GFE_EINVAL, "plan not admissible for this n");
     if (P.limb && !e->eng.can_limb) return set_error(GFE_EINVAL, "the limb transform is not exact for this (b, n)");
+    if (P.limb && P.fused && !gfe::limb_fused_ok(e->eng.b, e->eng.n, P.col_ng))
+        return set_error(GFE_EINVAL, "the fused limb normalisation is not exact for this (b, n) and column group");
     e->eng.flush_carry();
     if (hipStreamSynchronize(e->eng.stream) != hipSuccess) return set_error(GFE_EHIP, "set_plan sync");
     e->eng.plan = P;

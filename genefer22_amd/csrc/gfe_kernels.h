@@ -772,6 +772,8 @@ struct ColArgs {
     const int64_t *wc = nullptr;  // forward: workgroup carries still to add into src (carry_wg_body), or null
     uint32_t b = 0;
     uint32_t bm[4] = {0, 0, 0, 0};  // limb forward pass: b 2^32 mod p of each prime
+    const int4 *ls = nullptr;       // limb forward pass: carries into ranges of 2^rlog limbs, or null
+    int rlog = 0;
 };
 
 // 32 points per thread need more than the 128 registers a 1024-thread workgroup leaves: such plans

@@ -199,4 +199,111 @@ __device__ __forceinline__ void carry_limb_body(int32_t *dc, int64_t *wl, unsign
     *reinterpret_cast<int4 *>(p + 4) = make_int4(e[4], e[5], e[6], e[7]);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Fused inverse column pass and normalisation (ks_col_inv_norm_limb): a workgroup owns NG
+// consecutive columns of the 2^a x C matrix of limbs for all four primes, so after its inverse
+// column transforms (col_inv_to_lds) it holds 2^a rows of NG consecutive limbs each -- limbs
+// k C + blk NG + j, j < NG, for row k -- and normalises every row as a range of its own.  What a
+// range hands on to the next (e0 into its limb 0, e1 into its limb 1: the pieces of the range's
+// last limbs and the digit carry) is split into digits, e0 = u0 + u1 b + u2 B, and left in
+// slots[next] = (u0, u1, u2 + e1), negated for the range after the last (B^M = -1).  The next
+// forward column pass adds these to the first two limbs' digit pairs without normalising them
+// (col_fwd_pass, ls): such a limb is at most (b + 4)(b + 1) or a balanced limb plus a few units
+// more than the largest h2 -- about twice a balanced limb -- which the host's bound for these plans
+// counts (limb_fused_ok).  Every other use of the register runs carry_limb_slot_body first.
+
+struct LimbColNormArgs {
+    const uint32_t *ws;   // [4][M] residues after the row pass (read only)
+    const uint32_t *tw;   // [4][M] twiddle tables
+    PrimeArgs pr;
+    int32_t *dst;         // register 0: 2M digits
+    int4 *slots;          // M / ng: the carry into each range of ng limbs
+    const LimbCrt *crt;
+    int n, a, ng;         // n = log2 M
+    int dup;
+};
+
+// The normalisation half: rows of NG limbs whose four primes' residues are at
+// NL[p rows NG + k NG + j]; thread t takes S consecutive limbs of row t / (NG / S), pieces and the
+// digit carry pass between the threads of one row through xs (3 blockDim.x int64 of LDS, apart
+// from NL), as in norm_limb_body.  blockDim.x = rows NG / S.
+template <int S>
+__device__ __forceinline__ void norm_limb_rows(const LimbCrt &c, const uint32_t *NL, uint32_t NG, uint32_t rows,
+                                               uint32_t C, unsigned blk, int32_t *__restrict__ dst,
+                                               int4 *__restrict__ slots, uint32_t nrange, bool dup, int64_t *xs) {
+    static_assert(S >= 2 && S % 2 == 0, "two or more limbs per thread");
+    const uint32_t tid = threadIdx.x, nt = blockDim.x, tpr = NG / S;
+    const uint32_t k = tid / tpr, jt = tid % tpr, j0 = jt * S;
+    uint32_t a[4][S];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t *src = NL + (size_t)p * rows * NG + k * NG + j0;
+        if constexpr (S % 4 == 0) {
+#pragma unroll
+            for (int j = 0; j < S; j += 4) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(src + j);
+                a[p][j] = v.x; a[p][j + 1] = v.y; a[p][j + 2] = v.z; a[p][j + 3] = v.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < S; j += 2) {
+                const uint2 v = *reinterpret_cast<const uint2 *>(src + j);
+                a[p][j] = v.x; a[p][j + 1] = v.y;
+            }
+        }
+    }
+    int64_t h0[S], h1[S], h2[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const uint32_t r[4] = {a[0][j], a[1][j], a[2][j], a[3][j]};
+        limb_pieces(c, r, dup, h0[j], h1[j], h2[j]);
+    }
+    xs[tid] = h1[S - 1];
+    xs[nt + tid] = h2[S - 2];
+    xs[2 * nt + tid] = h2[S - 1];
+    __syncthreads();
+    int64_t pb1 = 0, pb2 = 0, pl2 = 0;  // the row's previous thread: h1 of its last limb, h2 of its last two
+    if (jt > 0) { pb1 = xs[tid - 1]; pb2 = xs[nt + tid - 1]; pl2 = xs[2 * nt + tid - 1]; }
+    const uint32_t b = c.b;
+    const double binv = c.binv;
+    int32_t d[2 * S];
+    int64_t cr = 0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int64_t lj = h0[j] + (j == 0 ? pb1 + pb2 : h1[j - 1] + (j == 1 ? pl2 : h2[j - 2]));
+        limb_digits(lj + cr, b, binv, d[2 * j], d[2 * j + 1], cr);
+    }
+    __syncthreads();
+    xs[tid] = cr;
+    __syncthreads();
+    if (jt > 0) absorb_carry<2 * S>(d, xs[tid - 1], b, binv);
+    int32_t *o = dst + 2 * ((size_t)k * C + (size_t)blk * NG + j0);
+#pragma unroll
+    for (int j = 0; j < 2 * S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);
+    if (jt == tpr - 1) {
+        const int64_t e0 = h1[S - 1] + h2[S - 2] + cr, e1 = h2[S - 1];
+        int64_t q, u2;
+        const int32_t u0 = div_bal((uint64_t)e0, (double)e0, b, binv, &q);
+        const int32_t u1 = div_bal((uint64_t)q, (double)q, b, binv, &u2);
+        uint32_t w = k * (C / NG) + blk + 1;
+        int32_t sg = 1;
+        if (w == nrange) { w = 0; sg = -1; }
+        slots[w] = make_int4(sg * u0, sg * u1, sg * (int32_t)(u2 + e1), 0);
+    }
+}
+
+// Range w of 2^rlog limbs (digits from 2w 2^rlog) receives slots[w] = (u0, u1, v): u0 + u1 b at its
+// digit 0 and v at its digit 2, normalised over its first 8 digits as carry_limb_body does.
+__device__ __forceinline__ void carry_limb_slot_body(int32_t *dc, const int4 *slots, int rlog, uint32_t b, unsigned w) {
+    const int4 v = slots[w];
+    if (v.x == 0 && v.y == 0 && v.z == 0) return;
+    int32_t *p = dc + ((size_t)w << (rlog + 1));
+    int32_t e[8];
+    const int4 v0 = *reinterpret_cast<const int4 *>(p), v1 = *reinterpret_cast<const int4 *>(p + 4);
+    e[0] = v0.x; e[1] = v0.y; e[2] = v0.z; e[3] = v0.w; e[4] = v1.x; e[5] = v1.y; e[6] = v1.z; e[7] = v1.w;
+    absorb_carry2<8>(e, (int64_t)v.x + (int64_t)v.y * (int64_t)b, (int64_t)v.z, b, 1.0 / (double)b);
+    *reinterpret_cast<int4 *>(p) = make_int4(e[0], e[1], e[2], e[3]);
+    *reinterpret_cast<int4 *>(p + 4) = make_int4(e[4], e[5], e[6], e[7]);
+}
+
 }  // namespace gfe

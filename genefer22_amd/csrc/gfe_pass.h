@@ -341,16 +341,48 @@ __device__ __forceinline__ bool col_absorb_limb_carries(int32_t (&d0)[1 << LOGE]
     return true;
 }
 
+// The range carries of the fused limb normalisation (norm_limb_rows), for a column workgroup whose
+// NG columns are one range of every row (NG = 2^rlog, the plan that wrote them): range k C / NG + blk
+// of row k starts at the group's first column, its carry ls[.] = (u0, u1, v, 0) goes to the digit
+// pair of lane q = 0 as (u0, u1) and to lane q = 1 as (v, 0).  The workgroup stages one carry per row
+// in LDS (the loads go out with the digits'), and after one barrier the two lanes add theirs, read
+// from LDS, to every row they hold.  The staging area (2^LOGG x 16 bytes) is released by the barrier
+// that precedes the pass's first LDS exchange.
+template <int LOGE, int LOGG>
+__device__ __forceinline__ bool col_add_range_carries(int32_t (&d0)[1 << LOGE], int32_t (&d1)[1 << LOGE],
+                                                      const int4 *__restrict__ ls, int rlog, uint32_t C, uint32_t c0,
+                                                      uint32_t q, uint32_t base, int YL, uint32_t *lds) {
+    constexpr int E = 1 << LOGE;
+    int4 *P = reinterpret_cast<int4 *>(lds);
+    const uint32_t G = C >> rlog, g = c0 >> rlog;
+    for (uint32_t r = threadIdx.x; r < (1u << LOGG); r += blockDim.x) P[r] = ls[(size_t)r * G + g];
+    __syncthreads();
+    if (q < 2) {
+        const int2 *P2 = reinterpret_cast<const int2 *>(lds) + q;
+#pragma unroll
+        for (int rr = 0; rr < E; ++rr) {
+            const int2 v = P2[2 * (base + ((uint32_t)rr << YL))];
+            d0[rr] += v.x;
+            d1[rr] += v.y;
+        }
+    }
+    return true;
+}
+
 // Forward levels 0..a-1 of one prime: signed digits `src` -> residues `dst` (both N values).  With
 // `wc`, the carries of the last normalisation are added on the way in (col_absorb_carries).  LIMB:
 // element i is the limb d_2i + b d_2i+1 of two digits (the two-digit-limb transform, gfe_limb.h),
 // formed mod p with bm = b 2^32 mod p; `wc` then holds the limb normalisation's boundary carries
-// (col_absorb_limb_carries).
+// (col_absorb_limb_carries), or `ls` the carries of the fused limb normalisation into its ranges of
+// 2^rlog limbs (norm_limb_rows): the first two limbs of range r take (ls[r].x, ls[r].y) and
+// (ls[r].z, 0) on their digit pairs, unnormalised (the transform's exactness bound for these
+// plans counts those limbs at twice the size, limb_fused_ok on the host).
 template <class M, int LOGE, int LOGG, bool LIMB = false>
 __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                              const uint32_t *__restrict__ tw, uint32_t C, int NG, unsigned blk,
                                              uint32_t *lds, const int64_t *__restrict__ wc = nullptr,
-                                             uint32_t b = 0, uint32_t bm = 0) {
+                                             uint32_t b = 0, uint32_t bm = 0, const int4 *__restrict__ ls = nullptr,
+                                             int rlog = 0) {
     using GM = Geo<LOGG, LOGE>;
     constexpr int E = 1 << LOGE;
     const uint32_t t = threadIdx.x, q = t & (uint32_t)(NG - 1), s = t / (uint32_t)NG;
@@ -376,6 +408,7 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
             if (wc)
                 patched = col_absorb_limb_carries<LOGE, LOGG>(d0, d1, src, wc, b, C, blk * (uint32_t)NG, q, base, YL,
                                                               (uint32_t)(((size_t)C << LOGG) / LIMB_RANGE), lds);
+            if (ls) patched = col_add_range_carries<LOGE, LOGG>(d0, d1, ls, rlog, C, blk * (uint32_t)NG, q, base, YL, lds);
 #pragma unroll
             for (int rr = 0; rr < E; ++rr) x[rr] = md.add(to_res(d0[rr], md.p), md.mulw(to_res(d1[rr], md.p), bm));
         } else if constexpr (ci == 0) {
@@ -447,6 +480,50 @@ __device__ __forceinline__ void col_inv_pass(const M &md, uint32_t *__restrict__
     });
 }
 
+// Inverse levels a-1..0 of one prime as col_inv_pass, run by thread t of a group of threads that
+// has its own exchange region L (a workgroup holding several primes' column groups side by side),
+// with the output left in LDS: element (k, q) of the NG columns at out[k NG + q], row-major, for a
+// normalisation that walks the rows.  `out` may overlap any group's exchange region: a workgroup
+// barrier separates every group's last exchange reads from these writes, so all the groups of the
+// workgroup must call this with the same LOGE, LOGG and NG.
+template <class M, int LOGE, int LOGG>
+__device__ __forceinline__ void col_inv_to_lds(const M &md, const uint32_t *__restrict__ data, const uint32_t *__restrict__ tw,
+                                               uint32_t C, int NG, unsigned blk, uint32_t *L, uint32_t t, uint32_t *out) {
+    using GM = Geo<LOGG, LOGE>;
+    constexpr int E = 1 << LOGE;
+    const uint32_t q = t & (uint32_t)(NG - 1), s = t / (uint32_t)NG;
+    const uint32_t c = blk * (uint32_t)NG + q;
+    L += q * col_pass_stride(LOGG, LOGE, NG);
+    uint32_t x[E], w[E];
+    static_for_down<0, GM::NCH>([&](auto cc) {
+        constexpr int ci = decltype(cc)::value;
+        constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
+        const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+        load_inv_tw<LOGE>(w, tw, (1u << lam0) + hi, lam0, NE);
+        if constexpr (ci == GM::NCH - 1) {
+            const uint32_t off = (base * C + c) * 4u;
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = *at_bytes(data + (size_t)((uint32_t)rr << YL) * C, off);
+        } else {
+            __syncthreads();
+            const uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+        }
+        inv_chunk<M, LOGE, NE>(md, x, w);
+        if constexpr (ci == 0) {
+            __syncthreads();
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) out[(base + ((uint32_t)rr << YL)) * (uint32_t)NG + q] = x[rr];
+        } else {
+            if constexpr (ci < GM::NCH - 1) __syncthreads();  // the first chunk follows no LDS read
+            uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+        }
+    });
+}
+
 // Instantiated shapes of the compile-time passes (host and device agree on these ranges).
 constexpr int SPEC_ROW_MIN = 6, SPEC_ROW_MAX = 14;   // row levels
 constexpr int SPEC_COL_MIN = 5, SPEC_COL_MAX = 11;   // column levels
@@ -457,6 +534,26 @@ __host__ __device__ inline bool spec_row_ok(int logg, int loge) {
 __host__ __device__ inline bool spec_col_ok(int logg, int loge, int ng) {
     return logg >= SPEC_COL_MIN && logg <= SPEC_COL_MAX && loge >= 3 && loge <= 5 && logg >= loge && ng >= 8 &&
            ng <= 64;
+}
+
+// The limb transform's fused inverse column pass and normalisation (ks_col_inv_norm_limb) runs the
+// four primes' groups of T = ng 2^logg / 2^loge threads side by side: 4 T threads, whole waves per
+// prime, within the workgroup limit (512 at 32 points per thread), so logg - loge <= 5.  LDS: the
+// four exchange regions, reused for the rows (4 ng 2^logg words) and the normalisation's
+// neighbour exchange (3 int64 per thread).
+constexpr int LIMB_FUSED_MAX_SPAN = 5;
+
+__host__ __device__ inline size_t limb_col_norm_lds_bytes(int logg, int loge, int ng) {
+    const size_t T = ((size_t)ng << logg) >> loge;
+    const size_t ex = 4 * col_pass_lds_words(logg, loge, ng);
+    const size_t rows = 4 * ((size_t)ng << logg) + 6 * 4 * T;
+    return 4 * (ex > rows ? ex : rows);
+}
+
+__host__ __device__ inline bool limb_col_norm_ok(int logg, int loge, int ng) {
+    if (!spec_col_ok(logg, loge, ng) || logg - loge > LIMB_FUSED_MAX_SPAN) return false;
+    const int T = (ng << logg) >> loge;
+    return T % 64 == 0 && 4 * T <= (loge >= 5 ? 512 : 1024) && limb_col_norm_lds_bytes(logg, loge, ng) <= 160 * 1024;
 }
 
 }  // namespace gfe

@@ -4,10 +4,12 @@
 
 #include "gfe_batch_dev.h"
 #include "gfe_kernels.h"
+#include "gfe_limb.h"
 
 namespace gfe {
 
 typedef void (*SpecColFn)(ColArgs);
+typedef void (*SpecLimbNormFn)(LimbColNormArgs);
 typedef void (*SpecRowFn)(RowArgs);
 typedef void (*SpecBatchColFn)(BArgs, int);
 typedef void (*SpecBatchRowFn)(BArgs);
@@ -16,6 +18,9 @@ typedef void (*SpecBatchRowFn)(BArgs);
 SpecColFn spec_col_fwd(int is32, int loge, int logg);
 SpecColFn spec_col_inv(int is32, int loge, int logg);
 SpecColFn spec_col_fwd_limb(int loge, int logg);  // two-digit-limb transform, 31-bit primes
+// The limb transform's inverse column pass fused with its normalisation (grid: column groups,
+// 4 ng 2^logg / 2^loge threads, limb_col_norm_lds_bytes of LDS; null outside limb_col_norm_ok).
+SpecLimbNormFn spec_col_inv_norm_limb(int loge, int logg);
 SpecRowFn spec_row_square(int is32, int loge, int logg);
 SpecBatchColFn spec_batch_col_fwd(int loge, int logg);
 SpecBatchColFn spec_batch_col_inv(int loge, int logg);  // second argument unused

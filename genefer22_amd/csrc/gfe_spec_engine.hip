@@ -26,7 +26,31 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd_limb(ColArg
     const ModS md{A.pr.p[prime], A.pr.q[prime]};
     const size_t M = (size_t)1 << A.n;
     col_fwd_pass<ModS, LOGE, LOGG, true>(md, A.src, A.ws + prime * M, A.tw + prime * M, (uint32_t)(M >> LOGG), A.ng,
-                                         xcd_block(blockIdx.x, gridDim.x), lds, A.wc, A.b, A.bm[prime]);
+                                         xcd_block(blockIdx.x, gridDim.x), lds, A.wc, A.b, A.bm[prime], A.ls, A.rlog);
+}
+
+// Inverse column pass of the two-digit-limb transform fused with its normalisation (gfe_limb.h):
+// one workgroup per group of ng columns runs the four primes' inverse column transforms side by side
+// (threads [p T, (p + 1) T) for prime p, T = ng 2^LOGG / 2^LOGE, each prime on its own exchange
+// region), leaves them in LDS and normalises the group's 2^LOGG rows of ng limbs, 2^LOGE / 4 limbs
+// per thread.  The residues are read once and never written back; register 0 and the range carries
+// are the only outputs.
+template <int LOGE, int LOGG>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_inv_norm_limb(LimbColNormArgs A) {
+    extern __shared__ uint32_t lds[];
+    constexpr int S = (1 << LOGE) / 4;
+    const uint32_t T = ((uint32_t)A.ng << LOGG) >> LOGE;
+    const uint32_t prime = threadIdx.x / T, t = threadIdx.x - prime * T;
+    const ModS md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t M = (size_t)1 << A.n;
+    const uint32_t C = (uint32_t)(M >> LOGG);
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t exw = (uint32_t)col_pass_lds_words(LOGG, LOGE, A.ng), cells = (uint32_t)A.ng << LOGG;
+    col_inv_to_lds<ModS, LOGE, LOGG>(md, A.ws + prime * M, A.tw + prime * M, C, A.ng, blk, lds + prime * exw, t,
+                                     lds + prime * cells);
+    __syncthreads();
+    norm_limb_rows<S>(*A.crt, lds, (uint32_t)A.ng, 1u << LOGG, C, blk, A.dst, A.slots, (uint32_t)(M / (uint32_t)A.ng),
+                      A.dup != 0, reinterpret_cast<int64_t *>(lds + 4 * cells));
 }
 
 template <class M, int LOGE, int LOGG>
@@ -78,6 +102,19 @@ static SpecColFn col_limb_lookup(int logg) {
     }
 }
 
+template <int LOGE, int LOGG = SPEC_COL_MIN>
+static SpecLimbNormFn col_inv_norm_limb_lookup(int logg) {
+    if constexpr (LOGG > SPEC_COL_MAX || LOGG > LOGE + LIMB_FUSED_MAX_SPAN) {
+        return nullptr;
+    } else {
+        if (logg == LOGG) {
+            if constexpr (LOGG >= LOGE) return ks_col_inv_norm_limb<LOGE, LOGG>;
+            else return nullptr;
+        }
+        return col_inv_norm_limb_lookup<LOGE, LOGG + 1>(logg);
+    }
+}
+
 template <class M, int LOGE, int LOGG = SPEC_ROW_MIN>
 static SpecRowFn row_lookup(int logg) {
     if constexpr (LOGG > SPEC_ROW_MAX) {
@@ -113,6 +150,15 @@ SpecColFn spec_col_fwd_limb(int loge, int logg) {
     }
 }
 
+SpecLimbNormFn spec_col_inv_norm_limb(int loge, int logg) {
+    switch (loge) {
+        case 3: return col_inv_norm_limb_lookup<3>(logg);
+        case 4: return col_inv_norm_limb_lookup<4>(logg);
+        case 5: return col_inv_norm_limb_lookup<5>(logg);
+        default: return nullptr;
+    }
+}
+
 SpecRowFn spec_row_square(int is32, int loge, int logg) {
     if (loge == 4) return is32 ? row_lookup<ModU, 4>(logg) : row_lookup<ModS, 4>(logg);
     if (loge == 5) return is32 ? row_lookup<ModU, 5>(logg) : row_lookup<ModS, 5>(logg);
@@ -131,6 +177,8 @@ void spec_set_lds_attributes() {
                     if (SpecColFn f = spec_col_inv(is32, loge, g))
                         (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                     if (SpecColFn f = is32 ? nullptr : spec_col_fwd_limb(loge, g))
+                        (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                    if (SpecLimbNormFn f = is32 ? nullptr : spec_col_inv_norm_limb(loge, g))
                         (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                 }
                 for (int g = SPEC_ROW_MIN; g <= SPEC_ROW_MAX; ++g)

@@ -106,7 +106,10 @@ int gfe_tune(gfe_engine *e, uint32_t reps);
  * workgroup, flags: bit 0 factored row twiddles, bit 1 all-prime column passes with the CRT and
  * carry fused into the inverse one, bit 2 persistent column passes, bit 3 compile-time passes,
  * bit 4 the squaring in one workgroup (n <= 13), bit 5 the two-digit-limb transform (see
- * gfe_engine_plan; refused where it is not exact for the engine's b)}.  No reference counterpart (the reference's tuned split is
+ * gfe_engine_plan; refused where it is not exact for the engine's b); with bit 5, bits 1 and 3
+ * together fuse the limb transform's inverse column pass with its normalisation (ranges of
+ * columns-per-workgroup limbs whose carries the next squaring adds; refused where the larger
+ * range-start limbs this leaves would break exactness for the engine's b)}.  No reference counterpart (the reference's tuned split is
  * internal to its engines): fixes a plan for reproducible benchmarks and for tests that cover
  * every kernel variant.  gfe_set_plan returns GFE_EINVAL for a plan not admissible at this n. */
 int gfe_get_plan(const gfe_engine *e, int32_t plan[6]);
@@ -121,7 +124,8 @@ int gfe_default_plan(uint32_t n, uint32_t capacity, int32_t plan[6]);
 /* The plan gfe_create gives an engine of (b, n): gfe_default_plan(n, 0), or for the (b, n) the
  * two-digit-limb transform covers (flags bit 5: limbs d_2j + b d_2j+1 of base b^2, N/2-point
  * transforms over four 31-bit primes; a and the row fields then describe the N/2-point transform)
- * its default plan.  No device needed. */
+ * its default plan, with the inverse column pass fused to the normalisation (flags 42) where that
+ * stays exact.  No device needed. */
 int gfe_engine_plan(uint32_t b, uint32_t n, int32_t plan[6]);
 int gfe_plan_admissible(uint32_t n, const int32_t plan[6]);
 int gfe_candidate_plans(uint32_t n, int32_t *plans, uint32_t max, uint32_t *count);

@@ -1,7 +1,8 @@
 """GPU parity of the two-digit-limb transform (limbs d_2j + b d_2j+1 over four 31-bit primes,
 genefer22_amd/csrc/gfe_limb.h) against the oracle's squaring, which knows nothing of limbs:
-canonical residues equal after PRP squarings, products, squareMul, every admissible limb plan, and
-switching between the limb and the digit transform mid-run.  Digits stay within the balanced
+canonical residues equal after PRP squarings, products, squareMul, every admissible limb plan (the
+inverse column pass fused with the normalisation or not), and switching between the limb and the
+digit transform mid-run.  Digits stay within the balanced
 bound.  The C4 fixtures (n = 22, reference CPU transform) run through the limb plan by default
 (tests/test_gpu_golden.py)."""
 import numpy as np
@@ -121,18 +122,54 @@ def test_every_limb_plan(gfe, oracle, b, n):
     assert tried >= 10
 
 
+@pytest.mark.parametrize("b,n", [(100000000, 14), (60000000, 15)])
+def test_every_fused_limb_plan(gfe, oracle, b, n):
+    """Every admissible plan with the inverse column pass fused to the limb normalisation (flags
+    2 | 8 | 32: ranges of col_ng limbs, their carries added by the next forward column pass) gives
+    the oracle's residue, and a squaring's digits stay balanced once the carries are in."""
+    iters = 10
+    bits = prp_bits(b, n, iters)
+    o = oracle.Engine(b, n)
+    o.set(1)
+    for bit in bits:
+        o.square_dup(bit)
+    want = canon(oracle, o.get_digits(0), n, b)
+    t = gfe.Transform(b, n)
+    assert t.get_plan()[5] == 42, t.info()
+    plans = [p[:5] + (42,) for p in gfe.candidate_plans(n - 1) if p[5] == 8]
+    tried = 0
+    for plan in plans:
+        if not gfe.plan_admissible(n, plan):
+            continue
+        t.set_plan(plan)
+        t.set(1)
+        for bit in bits:
+            t.square_dup(bit)
+        got = t.get_digits(0)
+        assert np.abs(got).max() <= b // 2 + 4, plan
+        assert np.array_equal(canon(oracle, got, n, b), want), plan
+        tried += 1
+    assert tried >= 8
+
+
 def test_switching_between_limb_and_digit_transforms(gfe, oracle):
+    """The fused limb plan, the plain limb plan and the digit transform in turn: each hands the
+    next its pending carries in their own form (range slots, boundary pairs, workgroup carries)."""
     b, n = 100000000, 13
     bits = prp_bits(b, n, 60)
     t = gfe.Transform(b, n)
-    limb_plan, digit_plan = t.get_plan(), gfe.default_plan(n)
-    assert limb_plan[5] & LIMB and not digit_plan[5] & LIMB
+    fused_plan, digit_plan = t.get_plan(), gfe.default_plan(n)
+    assert fused_plan[5] == 42 and not digit_plan[5] & LIMB
+    limb_plan = fused_plan[:5] + (40,)
     o = oracle.Engine(b, n)
     t.set(1); o.set(1)
     for k, bit in enumerate(bits):
         if k % 7 == 0:
-            t.set_plan(digit_plan if (k // 7) % 2 else limb_plan)
+            t.set_plan((fused_plan, digit_plan, limb_plan)[(k // 7) % 3])
         t.square_dup(bit); o.square_dup(bit)
+        if k % 13 == 5:
+            t.copy(1, 0)
+            assert np.array_equal(canon(oracle, t.get_digits(1), n, b), canon(oracle, o.get_digits(0), n, b))
     assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b))
 
 
@@ -140,11 +177,18 @@ def test_limb_plan_refused_outside_its_bound(gfe):
     """b = 1999999266 at n = 12 exceeds the limb bound; the batch never runs limb plans."""
     limb12 = gfe.Transform(50000000, 12).get_plan()
     limb13 = gfe.Transform(100000000, 13).get_plan()
-    assert limb12[5] & LIMB and limb13[5] & LIMB
+    assert limb12[5] == 42 and limb13[5] == 42
     t = gfe.Transform(1999999266, 12)
     assert not t.get_plan()[5] & LIMB
     with pytest.raises(gfe.GfeError):
         t.set_plan(limb12)
+    # b = 249999998 at n = 12: limbs, but the fused normalisation's larger range-start limbs would
+    # leave its bound
+    u = gfe.Transform(249999998, 12)
+    assert u.get_plan()[5] == 40
+    with pytest.raises(gfe.GfeError):
+        u.set_plan(u.get_plan()[:5] + (42,))
+    u.close()
     bt = gfe.Batch(13, 4)
     with pytest.raises(gfe.GfeError):
         bt.set_plan(limb13)

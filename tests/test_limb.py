@@ -230,3 +230,126 @@ def test_normalisation_gives_the_square(b, n, dup, seed):
     want = (r * r * (2 if dup else 1)) % mod
     assert value(out, b) % mod == want
     assert max(abs(x) for x in out) <= half + 4
+
+
+# --- the fused inverse column pass and normalisation (norm_limb_rows, carry_limb_slot_body, and the
+# range carries col_fwd_pass adds to the next transform's limbs)
+
+def limb_fused_ok(b, n, ng):
+    """Host rule of gfe_engine.hip (limb_fused_ok): range-start limbs counted at their larger size."""
+    if not limb_ok(b, n) or ng < 8:
+        return False
+    B = b * b
+    h2max = P // (4 * B * B) + 1
+    L = (b // 2 + 4) * (b + 1)
+    L0 = (b + 4) * (b + 1)
+    L1 = L + h2max + 2
+    ranges = (1 << (n - 1)) // ng
+    return 2 * ranges * (L0 * L0 + L1 * L1 + (ng - 2) * L * L) < P // 4
+
+
+def normalise_fused(coeffs, b, n, dup, ng, S):
+    """norm_limb_rows over every range of ng limbs (threads of S limbs): the digits without the
+    range carries, and slots[w] = the carry into range w as (u0, u1, v)."""
+    c = make_consts(b, n)
+    M = 1 << (n - 1)
+    z = [[x % p * pow(P // p, -1, p) % p for p in PRIMES] for x in coeffs]
+    hs = [pieces(zj, c, dup) for zj in z]
+    digits = [0] * (2 * M)
+    nrange = M // ng
+    slots = [None] * nrange
+    tpr = ng // S
+    for w in range(nrange):
+        outs, carries = [], []
+        for jt in range(tpr):
+            j0 = w * ng + jt * S
+            h = hs[j0:j0 + S]
+            if jt > 0:
+                prev = hs[j0 - S:j0]
+                pb1, pb2, pl2 = prev[S - 1][1], prev[S - 2][2], prev[S - 1][2]
+            else:
+                pb1 = pb2 = pl2 = 0
+            d, cr = [], 0
+            for j in range(S):
+                lj = h[j][0] + (pb1 + pb2 if j == 0 else h[j - 1][1] + (pl2 if j == 1 else h[j - 2][2]))
+                assert abs(lj + cr) < 2 ** 62
+                d0, q = div_bal(lj + cr, b)
+                d1, cr = div_bal(q, b)
+                d += [d0, d1]
+            outs.append(d)
+            carries.append(cr)
+        for jt in range(tpr):
+            if jt > 0:
+                absorb(outs[jt], carries[jt - 1], b)
+            j0 = w * ng + jt * S
+            digits[2 * j0:2 * j0 + 2 * S] = outs[jt]
+        last = hs[(w + 1) * ng - S:(w + 1) * ng]
+        e0 = last[S - 1][1] + last[S - 2][2] + carries[tpr - 1]
+        e1 = last[S - 1][2]
+        u0, q = div_bal(e0, b)
+        u1, u2 = div_bal(q, b)
+        nxt, sg = (w + 1, 1) if w + 1 < nrange else (0, -1)
+        slots[nxt] = (sg * u0, sg * u1, sg * (u2 + e1))
+        assert abs(u2) <= 2 and abs(u2 + e1) < 2 ** 31
+    return digits, slots
+
+
+def flush_slots(digits, slots, b, ng):
+    """carry_limb_slot_body over every range."""
+    out = list(digits)
+    for w, (u0, u1, v) in enumerate(slots):
+        seg = out[2 * ng * w:2 * ng * w + 8]
+        absorb(seg, u0 + u1 * b, b, v)
+        out[2 * ng * w:2 * ng * w + 8] = seg
+    return out
+
+
+def fused_limbs(digits, slots, b, ng):
+    """The limbs the next forward column pass forms: digit pairs, range carries added unnormalised."""
+    limbs = [digits[2 * j] + b * digits[2 * j + 1] for j in range(len(digits) // 2)]
+    for w, (u0, u1, v) in enumerate(slots):
+        limbs[w * ng] += u0 + b * u1
+        limbs[w * ng + 1] += v
+    return limbs
+
+
+def test_fused_rule():
+    assert limb_fused_ok(20204692, 22, 16)  # C4's first seeded b
+    assert limb_fused_ok(35000000, 22, 16)
+    assert not limb_fused_ok(50000000, 22, 16)   # outside the limb transform altogether
+    assert limb_ok(249999998, 12) and not limb_fused_ok(249999998, 12, 8)  # limbs, but not fused
+    # the fused bound is stricter than the plain one: a b near the plain limit fails it
+    edge = [b for b in range(43000000, 45000000, 50000) if limb_ok(b, 22)]
+    assert edge and not limb_fused_ok(edge[-1], 22, 8)
+
+
+@pytest.mark.parametrize("b,n,dup,ng,S,seed", [(50000000, 12, True, 16, 4, 4), (200000000, 12, False, 8, 2, 5),
+                                               (100000000, 13, True, 32, 8, 6), (60000000, 12, False, 16, 4, 7)])
+def test_fused_normalisation_and_carries(b, n, dup, ng, S, seed):
+    """Digits of the square through the fused normalisation: flushed, they are balanced and equal
+    the square; as limbs with the range carries added (the next column pass), they keep the value
+    and stay within the sizes limb_fused_ok counts; a second squaring from those limbs is exact."""
+    if not limb_fused_ok(b, n, ng):
+        pytest.skip("outside the fused bound")
+    N, M = 1 << n, 1 << (n - 1)
+    rng = random.Random(seed)
+    half = b // 2
+    d = [rng.choice([-half - 4, half + 4]) if rng.random() < 0.3 else rng.randint(-half - 4, half + 4) for _ in range(N)]
+    mod = b ** N + 1
+    r = value(d, b) % mod
+    want = (r * r * (2 if dup else 1)) % mod
+    digits, slots = normalise_fused(negacyclic_square([d[2 * j] + b * d[2 * j + 1] for j in range(M)]), b, n, dup, ng, S)
+    assert max(abs(x) for x in digits) <= half + 4
+    flushed = flush_slots(digits, slots, b, ng)
+    assert value(flushed, b) % mod == want
+    assert max(abs(x) for x in flushed) <= half + 4
+    limbs = fused_limbs(digits, slots, b, ng)
+    B = b * b
+    assert sum(l * B ** j for j, l in enumerate(limbs)) % mod == want
+    L = (half + 4) * (b + 1)
+    h2max = P // (4 * B * B) + 1
+    for j, l in enumerate(limbs):
+        bound = (b + 4) * (b + 1) if j % ng == 0 else L + h2max + 2 if j % ng == 1 else L
+        assert abs(l) <= bound, (j, l, bound)
+    digits2, slots2 = normalise_fused(negacyclic_square(limbs), b, n, False, ng, S)
+    assert value(flush_slots(digits2, slots2, b, ng), b) % mod == want * want % mod

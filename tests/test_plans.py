@@ -150,27 +150,33 @@ def test_host_twiddle_tables(G, is32):
 
 def test_engine_plan_takes_the_limb_transform_where_it_is_exact(G):
     """gfe_engine_plan: the two-digit-limb transform (flag 32) exactly where the limb rule admits
-    (b, n) -- the rule restated in tests/test_limb.py -- and the digit transform's default
-    elsewhere; C4's bases always get it."""
-    from test_limb import limb_ok
+    (b, n) -- the rule restated in tests/test_limb.py -- with its inverse column pass fused to the
+    normalisation (flag 2) exactly where the fused rule admits it too, and the digit transform's
+    default elsewhere; C4's bases always get the fused limb plan."""
+    from test_limb import limb_fused_ok, limb_ok
     from genefer22_amd import workloads as W
     for b in W.config_bases(W.C4, 8):
         p = G.engine_plan(b, 22)
-        assert p[5] == 40 and G.plan_admissible(22, p), (b, p)
+        assert p[5] == 42 and G.plan_admissible(22, p), (b, p)
         assert p[:5] + (8,) == G.default_plan(21), p  # the 2^21-point transform's compile-time plan
     rs = np.random.RandomState(3)
     for n in range(12, 24):
-        for b in [1026, 1 << 23 | 2, 8388610, 30000000, 60000000, 249999998, 1999999266] + \
+        for b in [1026, 1 << 23 | 2, 8388610, 30000000, 60000000, 200000000, 249999998, 1999999266] + \
                  [int(x) * 2 for x in rs.randint(512, 1_000_000_000, 12)]:
             p = G.engine_plan(b, n)
             want = limb_ok(b, n)
             assert bool(p[5] & 32) == want, (b, n, p)
-            if not want:
+            if want:
+                assert bool(p[5] & 2) == limb_fused_ok(b, n, p[2]), (b, n, p)
+            else:
                 assert p == G.default_plan(n), (b, n, p)
 
 
 def test_limb_plans_need_the_compile_time_passes(G):
     base = G.default_plan(21)
     assert G.plan_admissible(22, base[:5] + (40,))
-    for fl in (32, 33, 34, 36, 56):  # runtime passes, fused / persistent columns, one-workgroup squaring
-        assert not G.plan_admissible(22, base[:5] + (fl,)), fl
+    for fl in (32, 33, 34, 36, 56, 41, 43, 44, 46):  # runtime passes, fused / persistent columns,
+        assert not G.plan_admissible(22, base[:5] + (fl,)), fl  # one-workgroup squaring, factored rows
+    assert G.plan_admissible(22, base[:5] + (42,))  # compile-time passes, inverse columns fused with the norm
+    # the fused kernel runs the four primes' column groups in one workgroup: 4 T <= 1024 threads
+    assert not G.plan_admissible(22, (7, 3, 32, 5, 1, 42)) and G.plan_admissible(22, (7, 3, 32, 5, 1, 40))
