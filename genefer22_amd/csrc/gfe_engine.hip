@@ -300,6 +300,11 @@ LimbCrt make_limb_crt(uint32_t n, uint32_t b) {
         c.w2[k] = (int64_t)x;
         c.w0d[k] = (double)c.w0[k];
         c.w1d[k] = (double)c.w1[k];
+        c.w0l[k] = (uint32_t)(uint64_t)c.w0[k];
+        c.w0h[k] = (uint32_t)((uint64_t)c.w0[k] >> 32);
+        c.w1l[k] = (uint32_t)(uint64_t)c.w1[k];
+        c.w1h[k] = (uint32_t)((uint64_t)c.w1[k] >> 32);
+        c.w2s[k] = (int32_t)c.w2[k];  // |w2| <= 2 for b >= 2^23 (limb_ok)
     }
     __int128 x = P;
     c.P0 = bal_piece(x, B);

@@ -34,6 +34,8 @@ struct LimbCrt {
     uint32_t zc[4], zc_q[4];   // mulp operands of 2^32 M^-1 (P/p_k)^-1 mod p_k and their w q factors
     double inv_p[4];           // 1 / p_k
     int64_t w0[4], w1[4], w2[4];  // P/p_k = w0 + w1 B + w2 B^2 (balanced pieces)
+    uint32_t w0l[4], w0h[4], w1l[4], w1h[4];  // low and high words of w0, w1 (two's complement)
+    int32_t w2s[4];            // w2 (|w2| <= 2: P/p_k < 2^93 <= 2 B^2)
     double w0d[4], w1d[4];
     int64_t P0, P1, P2;        // P = P0 + P1 B + P2 B^2
     double P0d, P1d;
@@ -44,23 +46,26 @@ struct LimbCrt {
     double binv;
 };
 
-// Exact int64 of an integer-valued double, |v| < 2^52.
+// The integer-valued double v (|v| < 2^51) as an int64: v + 1.5 2^52 has v's two's complement in
+// the low bits of its significand, so one add and an integer subtract recover it.
 __device__ __forceinline__ int64_t i64_of(double v) {
-    const double hi = floor(v * 0x1p-32);
-    const uint32_t lo = (uint32_t)fma(hi, -0x1p32, v);
-    return (int64_t)(((uint64_t)(uint32_t)(int32_t)hi << 32) | lo);
+    return (int64_t)(__double_as_longlong(v + 0x1.8p52) - 0x4338000000000000ll);
 }
 
-// Balanced division by B of the exact integer s, given as s mod 2^64 and an estimate sd with
-// |sd - s| << B: s = q B + h, h in (-B/2, B/2].
+// Division by B of the exact integer s, given as s mod 2^64 and an estimate sd with |sd - s| < 2^38:
+// s = q B + h with q = rint(sd / B), so |h| <= B/2 + 2^38 (B >= 2^46: within B/2 (1 + 2^-8)), not
+// balanced further -- every later step is exact for any h of that size.
 __device__ __forceinline__ int64_t split_B(uint64_t s64, double sd, const LimbCrt &c, int64_t *q) {
-    int64_t qi = i64_of(rint(sd * c.invB));
-    int64_t h = (int64_t)(s64 - (uint64_t)qi * c.B);
-    const int64_t half = (int64_t)(c.B >> 1);
-    if (h > half) { h -= (int64_t)c.B; qi += 1; }
-    else if (h <= -half) { h += (int64_t)c.B; qi -= 1; }
+    const int64_t qi = i64_of(rint(sd * c.invB));
     *q = qi;
-    return h;
+    return (int64_t)(s64 - (uint64_t)qi * c.B);
+}
+
+// z times the 64-bit constant (lo, hi) added to the 64-bit accumulator (acc, acch) mod 2^64: the
+// low product into acc (one 32x32+64 multiply-add), the high product's low word into acch.
+__device__ __forceinline__ void mac64(uint64_t &acc, uint32_t &acch, uint32_t z, uint32_t lo, uint32_t hi) {
+    acc += (uint64_t)z * lo;
+    acch += z * hi;
 }
 
 // Base-B pieces of the limb coefficient whose raw inverse-transform residues are a[0..3] (times 2
@@ -68,48 +73,66 @@ __device__ __forceinline__ int64_t split_B(uint64_t s64, double sd, const LimbCr
 __device__ __forceinline__ void limb_pieces(const LimbCrt &c, const uint32_t (&a)[4], bool dup, int64_t &h0,
                                             int64_t &h1, int64_t &h2) {
     uint32_t z[4];
+    double zd[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const ModS m{c.p[k], c.q[k]};
         z[k] = m.mulp(a[k], c.zc[k], c.zc_q[k]);
+        zd[k] = (double)z[k];
     }
     double ts = 0.0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) ts = fma((double)z[k], c.inv_p[k], ts);
+    for (int k = 0; k < 4; ++k) ts = fma(zd[k], c.inv_p[k], ts);
     const int32_t t = (int32_t)rint(ts);
+    const double td = (double)t;
     uint64_t s0 = (uint64_t)(-(int64_t)t) * (uint64_t)c.P0;
-    double s0d = -(double)t * c.P0d;
+    uint32_t s0h = 0;
+    double s0d = -td * c.P0d;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        s0 += (uint64_t)z[k] * (uint64_t)c.w0[k];
-        s0d = fma((double)z[k], c.w0d[k], s0d);
+        mac64(s0, s0h, z[k], c.w0l[k], c.w0h[k]);
+        s0d = fma(zd[k], c.w0d[k], s0d);
     }
     int64_t q0;
-    h0 = split_B(s0, s0d, c, &q0);
+    h0 = split_B(s0 + ((uint64_t)s0h << 32), s0d, c, &q0);
     uint64_t s1 = (uint64_t)q0 - (uint64_t)(int64_t)t * (uint64_t)c.P1;
-    double s1d = fma(-(double)t, c.P1d, (double)q0);
+    uint32_t s1h = 0;
+    double s1d = fma(-td, c.P1d, (double)q0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        s1 += (uint64_t)z[k] * (uint64_t)c.w1[k];
-        s1d = fma((double)z[k], c.w1d[k], s1d);
+        mac64(s1, s1h, z[k], c.w1l[k], c.w1h[k]);
+        s1d = fma(zd[k], c.w1d[k], s1d);
     }
     int64_t q1;
-    h1 = split_B(s1, s1d, c, &q1);
+    h1 = split_B(s1 + ((uint64_t)s1h << 32), s1d, c, &q1);
     int64_t s2 = q1 - (int64_t)t * c.P2;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) s2 += (int64_t)z[k] * c.w2[k];
+    for (int k = 0; k < 4; ++k) s2 += (int64_t)(int32_t)z[k] * c.w2s[k];  // z < 2^31
     h2 = s2;
     if (dup) {  // 2x = 2 h0 + 2 h1 B + 2 h2 B^2: pieces up to B stay within the limb bounds below
         h0 *= 2; h1 *= 2; h2 *= 2;
     }
 }
 
-// Two balanced digits of the limb value v and the carry out: v = d0 + b d1 + B c (div_bal is exact
-// for |v| < 2^63; its double estimate only seeds the quotient).
+// Two balanced digits of the limb value v and the carry out: v = d0 + b d1 + B c, digits in
+// [-(b-1)/2, b/2].  v is exact with |v| < 2^62, so (double)v is within 2^9 of it and each rounded
+// quotient below is within 1 of the balanced one (b >= 2^23): one correction step per digit, in
+// 32-bit arithmetic, since the remainders are below b in size; the first quotient's low word comes
+// from its double as in i64_of.
 __device__ __forceinline__ void limb_digits(int64_t v, uint32_t b, double binv, int32_t &d0, int32_t &d1, int64_t &c) {
-    int64_t q;
-    d0 = div_bal((uint64_t)v, (double)v, b, binv, &q);
-    d1 = div_bal((uint64_t)q, (double)q, b, binv, &c);
+    const int32_t hi = (int32_t)(b / 2), lo = -(int32_t)((b - 1) / 2), bs = (int32_t)b;
+    double q = rint((double)v * binv);                                     // |q| < 2^39
+    uint32_t ql = (uint32_t)__double_as_longlong(q + 0x1.8p52);
+    int32_t r = (int32_t)((uint32_t)v - ql * b);
+    const int32_t a1 = (r > hi) - (r < lo);                                // -1, 0 or 1
+    d0 = r - a1 * bs;
+    ql += (uint32_t)a1;
+    q += (double)a1;
+    int32_t c2 = (int32_t)rint(q * binv);                                  // |c2| < 2^17
+    int32_t r2 = (int32_t)(ql - (uint32_t)c2 * b);
+    const int32_t a2 = (r2 > hi) - (r2 < lo);
+    d1 = r2 - a2 * bs;
+    c = (int64_t)(c2 + a2);
 }
 
 // Adds the carries of a limb normalisation range boundary into the next range's first S digits:
@@ -282,9 +305,9 @@ __device__ __forceinline__ void norm_limb_rows(const LimbCrt &c, const uint32_t 
     for (int j = 0; j < 2 * S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);
     if (jt == tpr - 1) {
         const int64_t e0 = h1[S - 1] + h2[S - 2] + cr, e1 = h2[S - 1];
-        int64_t q, u2;
-        const int32_t u0 = div_bal((uint64_t)e0, (double)e0, b, binv, &q);
-        const int32_t u1 = div_bal((uint64_t)q, (double)q, b, binv, &u2);
+        int32_t u0, u1;
+        int64_t u2;
+        limb_digits(e0, b, binv, u0, u1, u2);
         uint32_t w = k * (C / NG) + blk + 1;
         int32_t sg = 1;
         if (w == nrange) { w = 0; sg = -1; }

@@ -61,15 +61,10 @@ def make_consts(b, n):
 
 
 def split_B(s64, sd, c):
+    """s = qi B + h with qi = rint(sd / B), no further balancing (|h| <= B/2 + 2^38)."""
     B = c["B"]
     qi = int(rint(sd * (1.0 / B)))
     h = i64(s64 - qi * B)
-    if h > B // 2:
-        h -= B
-        qi += 1
-    elif h <= -(B // 2):
-        h += B
-        qi -= 1
     return h, qi
 
 
@@ -104,6 +99,33 @@ def div_bal(f, b):
     if r > b // 2:
         r -= b
     return r, (f - r) // b
+
+
+def limb_digits(v, b):
+    """limb_digits as the device computes it -- rounded double quotients, one correction step each
+    in 32-bit arithmetic -- checked against the exact balanced divisions: v = d0 + b d1 + B c."""
+    assert abs(v) < 2 ** 62
+    binv = 1.0 / b
+    hi, lo = b // 2, -((b - 1) // 2)
+    q = rint(float(v) * binv)
+    assert abs(q) < 2 ** 51
+    ql = int(q) & 0xFFFFFFFF
+    r = ((v & 0xFFFFFFFF) - ql * b) & 0xFFFFFFFF
+    r = r - (1 << 32) if r >> 31 else r
+    a1 = (r > hi) - (r < lo)
+    d0 = r - a1 * b
+    ql = (ql + a1) & 0xFFFFFFFF
+    q = q + float(a1)
+    c2 = int(rint(q * binv))
+    r2 = (ql - c2 * b) & 0xFFFFFFFF
+    r2 = r2 - (1 << 32) if r2 >> 31 else r2
+    a2 = (r2 > hi) - (r2 < lo)
+    d1 = r2 - a2 * b
+    c = c2 + a2
+    e0, e1 = div_bal(v, b)
+    e1, ec = div_bal(e1, b)
+    assert (d0, d1, c) == (e0, e1, ec), (v, b)
+    return d0, d1, c
 
 
 def absorb(d, cin, b, e1=None):
@@ -142,9 +164,7 @@ def normalise(coeffs, b, n, dup):
             cr = 0
             for j in range(4):
                 lj = h[j][0] + (pb1 + pb2 if j == 0 else h[j - 1][1] + (pl2 if j == 1 else h[j - 2][2]))
-                assert abs(lj + cr) < 2 ** 62  # div_bal is exact for |f| < 2^63
-                d0, q = div_bal(lj + cr, b)
-                d1, cr = div_bal(q, b)
+                d0, d1, cr = limb_digits(lj + cr, b)
                 d += [d0, d1]
             carries.append(cr)
             outs.append(d)
@@ -200,7 +220,7 @@ def test_pieces_at_the_bound(b, n):
         z = [x % p * pow(P // p, -1, p) % p for p in PRIMES]
         h0, h1, h2 = pieces(z, c, False)
         assert h0 + h1 * B + h2 * B * B == x
-        assert -B // 2 < h0 <= B // 2 and -B // 2 < h1 <= B // 2
+        assert abs(h0) <= B // 2 + 2 ** 38 and abs(h1) <= B // 2 + 2 ** 38  # split_B does not rebalance
         assert abs(h2) <= (1 << n) // 4 + 2
 
 
@@ -272,9 +292,7 @@ def normalise_fused(coeffs, b, n, dup, ng, S):
             d, cr = [], 0
             for j in range(S):
                 lj = h[j][0] + (pb1 + pb2 if j == 0 else h[j - 1][1] + (pl2 if j == 1 else h[j - 2][2]))
-                assert abs(lj + cr) < 2 ** 62
-                d0, q = div_bal(lj + cr, b)
-                d1, cr = div_bal(q, b)
+                d0, d1, cr = limb_digits(lj + cr, b)
                 d += [d0, d1]
             outs.append(d)
             carries.append(cr)
@@ -286,8 +304,7 @@ def normalise_fused(coeffs, b, n, dup, ng, S):
         last = hs[(w + 1) * ng - S:(w + 1) * ng]
         e0 = last[S - 1][1] + last[S - 2][2] + carries[tpr - 1]
         e1 = last[S - 1][2]
-        u0, q = div_bal(e0, b)
-        u1, u2 = div_bal(q, b)
+        u0, u1, u2 = limb_digits(e0, b)
         nxt, sg = (w + 1, 1) if w + 1 < nrange else (0, -1)
         slots[nxt] = (sg * u0, sg * u1, sg * (u2 + e1))
         assert abs(u2) <= 2 and abs(u2 + e1) < 2 ** 31
@@ -353,3 +370,22 @@ def test_fused_normalisation_and_carries(b, n, dup, ng, S, seed):
         assert abs(l) <= bound, (j, l, bound)
     digits2, slots2 = normalise_fused(negacyclic_square(limbs), b, n, False, ng, S)
     assert value(flush_slots(digits2, slots2, b, ng), b) % mod == want * want % mod
+
+
+@pytest.mark.parametrize("b", [1 << 23, (1 << 23) + 1, 20204692, 43999999, 100000001, 249999998])
+def test_limb_digits_at_the_edges(b):
+    """The device's two-digit split (rounded double quotients, one 32-bit correction each) equals the
+    exact balanced division for values at quotient ties, at the balanced digits' ends and up to
+    2^61 in size (limb_digits asserts the equality)."""
+    B = b * b
+    rng = random.Random(b)
+    vs = [0, 1, -1, b // 2, -(b // 2), (b - 1) // 2 + 1, B // 2, -(B // 2), 2 ** 61 - 1, -(2 ** 61) + 1]
+    for k in (1, 3, 2 ** 20, 2 ** 30, 2 ** 37):
+        for off in (-1, 0, 1):
+            vs += [k * B + off, -k * B + off, k * b + b // 2 + off, -k * b - b // 2 + off,
+                   k * B + (B // 2) + off, k * B + b * (b // 2) + b // 2 + off]
+    vs += [rng.randint(-(2 ** 61), 2 ** 61) for _ in range(2000)]
+    for v in vs:
+        if abs(v) < 2 ** 61:
+            d0, d1, c = limb_digits(v, b)
+            assert d0 + b * d1 + B * c == v
