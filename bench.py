@@ -226,10 +226,14 @@ def _run_batch(G, bases, iters, plan):
         if bt.submit(b, iters, i) is None:
             raise RuntimeError("batch refused a candidate")
     got = {}
+    last = time.perf_counter()
     while len(got) < len(bases):
         bt.step()
         for r in bt.poll():
             got[r["tag"]] = r
+        if time.perf_counter() - last > 30.0:  # a long run reports its progress
+            last = time.perf_counter()
+            log(f"[bench] n=15 batch: {len(got)}/{len(bases)} candidates done")
     bt.close()
     return [got[i] for i in range(len(bases))]
 
