@@ -314,6 +314,7 @@ def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
         "speedup_vs_plain_serial": round(serial / summary["makespan_s"], 2),
         "plain_ms_per_squaring_by_n": {int(k): round(v, 5) for k, v in ms_by_n.items()},
         "pools_created": stats["pools_created"], "bytes_peak": stats["bytes_peak"], "wall_s": round(wall, 2),
+        "lone_runs": stats.get("solo", 0),
     }
 
 

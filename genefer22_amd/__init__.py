@@ -62,7 +62,7 @@ class ServerStats(ctypes.Structure):
                 ("queued", ctypes.c_uint64), ("bytes_budget", ctypes.c_uint64), ("bytes_reserved", ctypes.c_uint64),
                 ("bytes_peak", ctypes.c_uint64), ("pools", ctypes.c_uint32), ("pools_created", ctypes.c_uint32),
                 ("pools_released", ctypes.c_uint32), ("busy_slots", ctypes.c_uint32), ("pumps", ctypes.c_uint64),
-                ("blocks", ctypes.c_uint64)]
+                ("blocks", ctypes.c_uint64), ("solo", ctypes.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -100,6 +100,9 @@ ABI = [
     ("gfe_engine_plan", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, I32P]),
     ("gfe_candidate_plans", ctypes.c_int, [ctypes.c_uint32, I32P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("gfe_host_twiddles", ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, U32P]),
+    ("gfe_rebase", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("gfe_exponent_top", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, U64P, ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_uint32), U64P, U64P]),
     ("gfe_square_noalloc", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32,
                                           ctypes.c_uint64, ctypes.c_void_p]),
     ("gfe_run_prp_resumable", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_char_p, ctypes.c_uint64,
@@ -148,6 +151,7 @@ ABI = [
     ("gfe_server_pump", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
     ("gfe_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]),
     ("gfe_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    ("gfe_server_set_solo", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("gfe_serve_replay", ctypes.c_int, [U32P, U32P, U64P, U32P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_int,
                                         ctypes.c_uint64, U32P, ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
 ]
@@ -290,6 +294,12 @@ class Transform:
     def set_plan(self, plan) -> None:
         a = np.ascontiguousarray(plan, dtype=np.int32)
         _ck(lib().gfe_set_plan(self._h, _i32(a)))
+
+    def rebase(self, b: int) -> None:
+        """Moves the engine to base b of the same n and prime set (gfe_rebase); set the registers
+        before squaring."""
+        _ck(lib().gfe_rebase(self._h, b))
+        self.b = b
 
     def residue(self, reg: int = 0) -> dict:
         return residue(self.get_digits(reg), self.n, self.b)
@@ -583,6 +593,12 @@ class Server:
 
     def prepare(self, n: int, capacity: int = 0):
         _ck(lib().gfe_server_prepare(self._h, n, capacity))
+
+    def set_solo(self, max_rate: int):
+        """The lone-request path (a request admitted while no pool has work runs alone on a warm
+        single-candidate engine) while at most `max_rate` requests arrived in the last second
+        (default 4); 0 turns it off."""
+        _ck(lib().gfe_server_set_solo(self._h, int(max_rate)))
 
     def submit(self, n: int, b: int, iters: int = 0, tag: int = 0, depth: int = 0):
         """A quick test (or its first `iters` squarings), or with depth > 0 a proof request."""

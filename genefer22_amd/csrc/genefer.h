@@ -20,6 +20,7 @@
 #include <string>
 #include <vector>
 
+#include "exponent.h"
 #include "transform_hip.h"
 
 namespace gfe {
@@ -52,6 +53,22 @@ public:
         : _t(new transformHIP(b, n, device, num_regs)), _g(size_t(1) << n, b) {
         mpz_init(_E);
         mpz_ui_pow_ui(_E, b, 1ul << n);
+    }
+    // On an engine of (b, n) with num_regs registers that the caller keeps (a server's warm engine,
+    // gfe_rebase).
+    genefer(gfe_engine *engine, uint32_t b, uint32_t n, size_t num_regs)
+        : _t(new transformHIP(engine, b, n, num_regs)), _g(size_t(1) << n, b) {
+        mpz_init(_E);
+        mpz_ui_pow_ui(_E, b, 1ul << n);
+    }
+    // As above for a run of at most the PRP loop's first `top_bits` squarings: E holds its top bits
+    // (exponent_top) in place, zeros below them, which such a run never reads; esize() is exact.
+    genefer(gfe_engine *engine, uint32_t b, uint32_t n, size_t num_regs, uint64_t top_bits)
+        : _t(new transformHIP(engine, b, n, num_regs)), _g(size_t(1) << n, b) {
+        mpz_init(_E);
+        uint64_t S = 0, es = 0;
+        exponent_top(b, n, top_bits, _E, &S, &es);
+        mpz_mul_2exp(_E, _E, S);
     }
     ~genefer() { mpz_clear(_E); }
 

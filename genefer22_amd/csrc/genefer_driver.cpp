@@ -46,6 +46,26 @@ void fill_residue(gfe::gint &g, gfe_result *out) {
 
 extern "C" {
 
+int gfe_exponent_top(uint32_t b, uint32_t n, uint64_t k, uint64_t *words, uint32_t max_words, uint32_t *nwords,
+                     uint64_t *shift, uint64_t *esize) {
+    if (!words || !nwords || !shift || !esize) return gfe::set_error(GFE_EINVAL, "exponent_top arguments");
+    if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;
+    mpz_t T;
+    mpz_init(T);
+    gfe::exponent_top(b, n, k, T, shift, esize);
+    const size_t need = (mpz_sizeinbase(T, 2) + 63) / 64;
+    int rc = GFE_OK;
+    if (need > max_words) {
+        rc = gfe::set_error(GFE_EINVAL, "exponent_top: output too small");
+    } else {
+        size_t cnt = 0;
+        mpz_export(words, &cnt, -1, 8, 0, 0, T);
+        *nwords = (uint32_t)cnt;
+    }
+    mpz_clear(T);
+    return rc;
+}
+
 int gfe_run_quick(uint32_t b, uint32_t n, int device, gfe_result *out) {
     if (!out) return gfe::set_error(GFE_EINVAL, "null output");
     if (check_candidate(b, n) != GFE_OK) return GFE_EINVAL;

@@ -15,6 +15,8 @@
 
 #include <gmp.h>
 
+#include "exponent.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -275,7 +277,8 @@ struct Slot {
     int rns = 0, is32 = 0;
     uint64_t tag = 0, esize = 0, iters = 0;
     double t_submit = 0, t_done = 0;
-    std::vector<uint64_t> ebits;      // E = b^(2^n), little-endian 64-bit words
+    std::vector<uint64_t> ebits;      // E = b^(2^n) >> eshift, little-endian 64-bit words
+    uint64_t eshift = 0;              // truncated runs keep only E's top bits (exponent_top)
     std::vector<uint8_t> resbits;     // Gerbicz-Li exponent `res`, most significant bit first
     std::vector<uint8_t> powseq;      // round kinds of the queued power runs (RUN_POW)
     // Pietrzak-Li proof (proof batches): its items are queued phase by phase, each phase after the
@@ -292,7 +295,7 @@ struct Slot {
     int reads_pending = 0;            // result read-backs enqueued and not yet processed
 
     bool bit(uint64_t k) const {  // k-th squaring of the PRP loop (bit esize - 1 - k)
-        const uint64_t i = esize - 1 - k;
+        const uint64_t i = esize - 1 - k - eshift;
         return (ebits[i >> 6] >> (i & 63)) & 1;
     }
 };
@@ -889,12 +892,17 @@ int Batch::submit(uint32_t b, uint64_t iters, uint64_t tag, int32_t *slot_out, u
     S.t_submit = now_s() - t0;
     select_primes(b, S.n, &S.rns, &S.is32);
     S.is32 = transform_is32(S.rns, S.is32);
-    // E = b^(2^n) and its bits (src/genefer.h:1476)
+    // E = b^(2^n) and its bits (src/genefer.h:1476); the first `iters` squarings of a truncated run
+    // read only E's top bits, which exponent_top gives without computing E
     mpz_t E;
     mpz_init(E);
-    mpz_ui_pow_ui(E, b, 1ul << S.n);
-    S.esize = mpz_sizeinbase(E, 2);
-    S.ebits.assign((S.esize + 63) / 64, 0);
+    const bool want_all = iters == 0 || depth > 0;
+    exponent_top(b, S.n, want_all ? ~uint64_t(0) >> 8 : iters, E, &S.eshift, &S.esize);
+    if (!want_all && iters >= S.esize && S.eshift != 0) {  // the whole loop after all
+        mpz_ui_pow_ui(E, b, 1ul << S.n);
+        S.eshift = 0;
+    }
+    S.ebits.assign((mpz_sizeinbase(E, 2) + 63) / 64 + 1, 0);
     size_t cnt = 0;
     mpz_export(S.ebits.data(), &cnt, -1, 8, 0, 0, E);
     const bool full = (iters == 0 || iters >= S.esize) || depth > 0;  // a proof batch runs full tests

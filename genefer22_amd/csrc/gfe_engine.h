@@ -63,6 +63,7 @@ bool limb_ok(uint32_t b, uint32_t n);
 bool limb_fused_ok(uint32_t b, uint32_t n, int ng);
 // The plan an engine of (b, n) starts with (gfe_engine_plan).
 Plan engine_plan_for(uint32_t b, uint32_t n);
+size_t engine_bytes(int rns, size_t N, uint32_t num_regs, bool limb_tables);
 // Plans of the limb transform the autotuner times (n: digits exponent).
 std::vector<Plan> candidate_limb_plans(uint32_t n);
 Plan default_limb_plan(uint32_t n);

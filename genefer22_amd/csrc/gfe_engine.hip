@@ -412,6 +412,12 @@ bool limb_fused_ok(uint32_t b, uint32_t n, int ng) {
     return 2 * ranges * per_range < P / 4;
 }
 
+// Device bytes of an engine: registers, work space, multiplicand and twiddles of rns primes, the
+// carry buffer (N bytes) and, with limb_tables, the limb transform's twiddles (4 x N/2 words).
+size_t engine_bytes(int rns, size_t N, uint32_t num_regs, bool limb_tables) {
+    return (size_t)num_regs * N * 4 + 3 * (size_t)rns * N * 4 + N + (limb_tables ? 8 * N : 0);
+}
+
 // An engine's first plan: the limb transform where it is exact (with the inverse column pass and
 // normalisation fused where that bound holds), else the digit transform's default.
 Plan engine_plan_for(uint32_t b, uint32_t n) {
@@ -1220,7 +1226,7 @@ int gfe_set_digits(gfe_engine *e, uint32_t reg, const int32_t *in) {
 size_t gfe_mem_size(const gfe_engine *e) {
     if (!e) return 0;
     const Engine &g = e->eng;
-    return (size_t)g.num_regs * g.N * 4 + 3 * (size_t)g.rns * g.N * 4 + g.N / 2048 * 8 + (g.can_limb ? 2 * g.N * 4 : 0);
+    return gfe::engine_bytes(g.rns, g.N, g.num_regs, g.tw_limb != nullptr);
 }
 
 size_t gfe_cache_size(const gfe_engine *e) {
@@ -1506,6 +1512,35 @@ int gfe_host_twiddles(uint32_t n, int is32, int max_threads, uint32_t *out) {
     const int sets[3] = {is32, is32, is32}, primes[3] = {0, 1, 2};
     gfe::make_twiddle_tables(sets, primes, 3, n, out, max_threads);
     return GFE_OK;
+}
+
+// Moves the engine to base b of the same n: only the b-dependent state changes (the CRT constants,
+// the carry rounds, the limb transform's constants and whether it applies, the plan); the twiddle
+// tables and buffers stay.
+int gfe_rebase(gfe_engine *e, uint32_t b) {
+    GFE_GUARD(e);
+    Engine &g = e->eng;
+    if (b < 1024 || b > 2000000000u) return set_error(GFE_EINVAL, "b must be in [1024, 2000000000]");
+    int rns = 0, ref_is32 = 0;
+    gfe::select_primes(b, g.n, &rns, &ref_is32);
+    if (rns != g.rns || gfe::transform_is32(rns, ref_is32) != g.is32)
+        return set_error(GFE_EKIND, "rebase: b selects another prime set at this n");
+    if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "rebase sync");
+    g.b = b;
+    g.ref_is32 = ref_is32;
+    g.rounds = gfe::carry_rounds(b, g.n);
+    g.carry_pending = false;
+    const gfe::CrtConst c = gfe::make_crt(g.rns, g.is32, g.n, b);
+    if (hipMemcpy(g.crt, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "crt upload");
+    g.can_limb = g.tw_limb != nullptr && gfe::limb_ok(b, g.n) && gfe::default_limb_plan(g.n).limb;
+    if (g.can_limb) {
+        const gfe::LimbCrt lc = gfe::make_limb_crt(g.n, b);
+        if (hipMemcpy(g.limb_crt, &lc, sizeof(lc), hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(GFE_EHIP, "crt upload");
+        for (int k = 0; k < 4; ++k) g.bm4[k] = gfe::mulw_const(b % g.pr4.p[k], g.pr4.p[k]);
+    }
+    g.plan = g.can_limb ? gfe::engine_plan_for(b, g.n) : gfe::default_plan(g.n);
+    return g.check("rebase");
 }
 
 int gfe_set_plan(gfe_engine *e, const int32_t plan[6]) {

@@ -246,16 +246,25 @@ struct LimbColNormArgs {
     int dup;
 };
 
+// The 64-bit value of the lane one below within each row of 16 lanes (DPP row_shr:1, a VALU move
+// per word); lane 0 of a row reads 0.
+__device__ __forceinline__ int64_t lane_below(int64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(uint64_t)v, 0x111, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)((uint64_t)v >> 32), 0x111, 0xF, 0xF, false);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 // The normalisation half: rows of NG limbs whose four primes' residues are at
-// NL[p rows NG + k NG + j]; thread t takes S consecutive limbs of row t / (NG / S), pieces and the
-// digit carry pass between the threads of one row through xs (3 blockDim.x int64 of LDS, apart
-// from NL), as in norm_limb_body.  blockDim.x = rows NG / S.
+// NL[p rows NG + k NG + j]; thread t takes S consecutive limbs of row t / (NG / S).  The NG / S
+// threads of a row are consecutive lanes of one 16-lane DPP row (NG / S <= 16), so the pieces of
+// the previous thread's last limbs and its digit carry come across by DPP moves, with no LDS
+// round trip and no workgroup barrier.  blockDim.x = rows NG / S.
 template <int S>
 __device__ __forceinline__ void norm_limb_rows(const LimbCrt &c, const uint32_t *NL, uint32_t NG, uint32_t rows,
                                                uint32_t C, unsigned blk, int32_t *__restrict__ dst,
-                                               int4 *__restrict__ slots, uint32_t nrange, bool dup, int64_t *xs) {
+                                               int4 *__restrict__ slots, uint32_t nrange, bool dup) {
     static_assert(S >= 2 && S % 2 == 0, "two or more limbs per thread");
-    const uint32_t tid = threadIdx.x, nt = blockDim.x, tpr = NG / S;
+    const uint32_t tid = threadIdx.x, tpr = NG / S;
     const uint32_t k = tid / tpr, jt = tid % tpr, j0 = jt * S;
     uint32_t a[4][S];
 #pragma unroll
@@ -281,25 +290,20 @@ __device__ __forceinline__ void norm_limb_rows(const LimbCrt &c, const uint32_t 
         const uint32_t r[4] = {a[0][j], a[1][j], a[2][j], a[3][j]};
         limb_pieces(c, r, dup, h0[j], h1[j], h2[j]);
     }
-    xs[tid] = h1[S - 1];
-    xs[nt + tid] = h2[S - 2];
-    xs[2 * nt + tid] = h2[S - 1];
-    __syncthreads();
-    int64_t pb1 = 0, pb2 = 0, pl2 = 0;  // the row's previous thread: h1 of its last limb, h2 of its last two
-    if (jt > 0) { pb1 = xs[tid - 1]; pb2 = xs[nt + tid - 1]; pl2 = xs[2 * nt + tid - 1]; }
+    // the row's previous thread: h1 of its last limb, h2 of its last two (a row's first thread: none)
+    const int64_t pb1 = lane_below(h1[S - 1]), pb2 = lane_below(h2[S - 2]), pl2 = lane_below(h2[S - 1]);
+    const bool first = jt == 0;
     const uint32_t b = c.b;
     const double binv = c.binv;
     int32_t d[2 * S];
     int64_t cr = 0;
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-        const int64_t lj = h0[j] + (j == 0 ? pb1 + pb2 : h1[j - 1] + (j == 1 ? pl2 : h2[j - 2]));
+        const int64_t lj = h0[j] + (j == 0 ? (first ? 0 : pb1 + pb2) : h1[j - 1] + (j == 1 ? (first ? 0 : pl2) : h2[j - 2]));
         limb_digits(lj + cr, b, binv, d[2 * j], d[2 * j + 1], cr);
     }
-    __syncthreads();
-    xs[tid] = cr;
-    __syncthreads();
-    if (jt > 0) absorb_carry<2 * S>(d, xs[tid - 1], b, binv);
+    const int64_t cin = lane_below(cr);
+    if (!first) absorb_carry<2 * S>(d, cin, b, binv);
     int32_t *o = dst + 2 * ((size_t)k * C + (size_t)blk * NG + j0);
 #pragma unroll
     for (int j = 0; j < 2 * S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);

@@ -538,22 +538,22 @@ __host__ __device__ inline bool spec_col_ok(int logg, int loge, int ng) {
 
 // The limb transform's fused inverse column pass and normalisation (ks_col_inv_norm_limb) runs the
 // four primes' groups of T = ng 2^logg / 2^loge threads side by side: 4 T threads, whole waves per
-// prime, within the workgroup limit (512 at 32 points per thread), so logg - loge <= 5.  LDS: the
-// four exchange regions, reused for the rows (4 ng 2^logg words) and the normalisation's
-// neighbour exchange (3 int64 per thread).
+// prime, within the workgroup limit (512 at 32 points per thread), so logg - loge <= 5.  The
+// normalisation's threads of one row (ng / (2^loge / 4) of them) must share a 16-lane DPP row.
+// LDS: the four exchange regions, reused for the rows (4 ng 2^logg words).
 constexpr int LIMB_FUSED_MAX_SPAN = 5;
 
 __host__ __device__ inline size_t limb_col_norm_lds_bytes(int logg, int loge, int ng) {
-    const size_t T = ((size_t)ng << logg) >> loge;
     const size_t ex = 4 * col_pass_lds_words(logg, loge, ng);
-    const size_t rows = 4 * ((size_t)ng << logg) + 6 * 4 * T;
+    const size_t rows = 4 * ((size_t)ng << logg);
     return 4 * (ex > rows ? ex : rows);
 }
 
 __host__ __device__ inline bool limb_col_norm_ok(int logg, int loge, int ng) {
     if (!spec_col_ok(logg, loge, ng) || logg - loge > LIMB_FUSED_MAX_SPAN) return false;
     const int T = (ng << logg) >> loge;
-    return T % 64 == 0 && 4 * T <= (loge >= 5 ? 512 : 1024) && limb_col_norm_lds_bytes(logg, loge, ng) <= 160 * 1024;
+    return T % 64 == 0 && 4 * T <= (loge >= 5 ? 512 : 1024) && ng / ((1 << loge) / 4) <= 16 &&
+           limb_col_norm_lds_bytes(logg, loge, ng) <= 160 * 1024;
 }
 
 }  // namespace gfe

@@ -1,14 +1,17 @@
 // gfe_serve.cpp -- the serving loop (include/gfe_serve.h): continuous batching of candidate
-// requests of mixed n over pools of gfe_batch, admission by device-memory footprint.
+// requests of mixed n over pools of gfe_batch, admission by device-memory footprint, and a
+// lone-request path on warm single-candidate engines.
 //
-// Host code only: it drives the batch engine through its C ABI and owns no device memory itself
-// (hipMemGetInfo for the default budget is its only HIP call).
+// Host code only: it drives the batch and single-candidate engines through their C ABI and owns no
+// device memory itself (hipMemGetInfo for the default budget is its only HIP call).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -17,10 +20,8 @@
 #include "../../include/gfe.h"
 #include "../../include/gfe_batch.h"
 #include "../../include/gfe_serve.h"
-
-namespace gfe {
-int set_error(int code, const std::string &msg);
-}
+#include "gfe_engine.h"
+#include "genefer.h"
 
 namespace {
 
@@ -51,6 +52,32 @@ struct Admitted {
     double t_admit;
 };
 
+// A warm single-candidate engine of the lone-request path, for one (n, prime set computed in,
+// limb tables) and any b of that kind (gfe_rebase).
+struct Warm {
+    gfe_engine *e = nullptr;
+    uint32_t n = 0;
+    int rns = 0, is32 = 0;
+    bool limb = false;
+    uint64_t bytes = 0;
+};
+
+// The lone-request path: a quick or truncated test that arrives while no pool has work runs alone
+// on a warm engine, on a host thread, as gfe_run_quick / gfe_run_prp run it (the same residues as
+// in a pool); requests that arrive meanwhile go to the pools.
+struct Solo {
+    std::thread th;
+    std::atomic<bool> finished{false};
+    bool running = false;
+    Admitted job;
+    gfe_engine *e = nullptr;   // the warm engine in use, or null: the worker creates `created`
+    Warm created;
+    gfe_serve_result out;
+    int rc = GFE_OK;
+    std::string err;
+    double t_done = 0;
+};
+
 struct Pool {
     gfe_batch *bt = nullptr;
     uint32_t n = 0, cap = 0;
@@ -72,8 +99,154 @@ struct gfe_server {
     std::unordered_map<uint64_t, Admitted> inflight;  // by batch tag
     std::deque<gfe_serve_result> done;
     std::vector<gfe_batch_result> buf;
+    int solo_rate = 4;            // the lone-request path while at most this many arrivals in the last second
+    std::deque<double> recent;    // arrival times of the last second
+    Solo solo;
+    std::vector<Warm> warm;
+
+    // Light load: few enough arrivals in the last second that a lone run does not hold up the pools.
+    bool light() {
+        const double now = clock();
+        while (!recent.empty() && recent.front() < now - 1.0) recent.pop_front();
+        return solo_rate > 0 && (int)recent.size() <= solo_rate;
+    }
 
     double clock() const { return steady_s() - t0; }
+
+    static Warm warm_kind(uint32_t b, uint32_t n) {
+        Warm w;
+        w.n = n;
+        gfe::select_primes(b, n, &w.rns, &w.is32);
+        w.is32 = gfe::transform_is32(w.rns, w.is32);
+        w.limb = gfe::limb_ok(b, n) && gfe::default_limb_plan(n).limb;
+        w.bytes = gfe::engine_bytes(w.rns, size_t(1) << n, 3, w.limb);
+        return w;
+    }
+    long find_warm(const Warm &k) const {
+        for (size_t i = 0; i < warm.size(); ++i)
+            if (warm[i].n == k.n && warm[i].rns == k.rns && warm[i].is32 == k.is32 && warm[i].limb == k.limb) return (long)i;
+        return -1;
+    }
+    // Frees warm engines not in use until `need` more bytes fit in the budget.
+    bool make_room(uint64_t need) {
+        for (size_t i = warm.size(); i-- > 0 && st.bytes_reserved + need > st.bytes_budget;) {
+            if (solo.running && warm[i].e == solo.e) continue;
+            gfe_destroy(warm[i].e);
+            st.bytes_reserved -= warm[i].bytes;
+            warm.erase(warm.begin() + (long)i);
+        }
+        return st.bytes_reserved + need <= st.bytes_budget;
+    }
+    bool idle() const {
+        for (const Pool &P : pools)
+            if (P.busy) return false;
+        return inflight.empty();
+    }
+
+    // Runs on the lone-request thread: touches only `solo` and its engine.
+    void solo_work() {
+        Solo &S = solo;
+        const Request rq = S.job.rq;
+        try {
+            gfe_engine *e = S.e;
+            if (!e) {
+                if (gfe_create(rq.b, rq.n, 3, device, &e) != GFE_OK) throw std::runtime_error(gfe_last_error());
+                S.created.e = e;
+            } else if (gfe_rebase(e, rq.b) != GFE_OK) {
+                throw std::runtime_error(gfe_last_error());
+            }
+            // a truncated run needs only E's top bits (exponent_top), not E
+            std::unique_ptr<gfe::genefer> Gp(rq.iters ? new gfe::genefer(e, rq.b, rq.n, 3, rq.iters)
+                                                      : new gfe::genefer(e, rq.b, rq.n, 3));
+            gfe::genefer &G = *Gp;
+            const size_t es = G.esize();
+            const bool full = rq.iters == 0 || rq.iters >= es;
+            const int B_GL = full ? gfe::genefer::B_GerbiczLi(es) : 0;
+            G.prp(B_GL, 0, false, full ? 0 : (long)(es - rq.iters));
+            G.t().getInt(G.gi());
+            uint64_t r = 0, o = 0;
+            const bool one = G.gi().isOne(r, o);
+            memset(&S.out, 0, sizeof(S.out));
+            S.out.tag = rq.tag; S.out.n = rq.n; S.out.b = rq.b;
+            S.out.res64 = r; S.out.old64 = o; S.out.hash64 = one ? 0 : G.gi().gethash64();
+            S.out.esize = es; S.out.iters = full ? es : rq.iters; S.out.is_prp = one ? 1 : 0;
+            S.out.gl_ok = full ? (G.GL(B_GL) ? 1 : 0) : -1;
+        } catch (const std::exception &ex) {
+            S.rc = GFE_EHIP;
+            S.err = ex.what();
+        }
+        S.t_done = clock();
+        S.finished.store(true, std::memory_order_release);
+    }
+
+    // Starts `rq` alone on a warm engine of its kind (one is created on the thread when none is
+    // warm and it fits the budget); false: the request takes the pool path.
+    bool start_solo(const Request &rq) {
+        const Warm k = warm_kind(rq.b, rq.n);
+        const long wi = find_warm(k);
+        if (wi < 0) {
+            if (!make_room(k.bytes)) return false;
+            st.bytes_reserved += k.bytes;
+            st.bytes_peak = std::max(st.bytes_peak, st.bytes_reserved);
+        }
+        solo.job = Admitted{rq, clock()};
+        solo.e = wi >= 0 ? warm[(size_t)wi].e : nullptr;
+        solo.created = k;
+        solo.rc = GFE_OK;
+        solo.err.clear();
+        solo.finished.store(false);
+        try {
+            solo.th = std::thread(&gfe_server::solo_work, this);
+        } catch (const std::exception &) {
+            if (wi < 0) st.bytes_reserved -= k.bytes;
+            return false;
+        }
+        solo.running = true;
+        return true;
+    }
+
+    int collect_solo() {
+        if (!solo.running || !solo.finished.load(std::memory_order_acquire)) return GFE_OK;
+        solo.th.join();
+        solo.running = false;
+        if (!solo.e) {  // the worker was to create the engine
+            if (solo.created.e && solo.rc == GFE_OK) {
+                warm.push_back(solo.created);
+            } else {
+                if (solo.created.e) gfe_destroy(solo.created.e);
+                st.bytes_reserved -= solo.created.bytes;
+            }
+        }
+        if (solo.rc != GFE_OK) return gfe::set_error(solo.rc, "serve: lone run: " + solo.err);
+        gfe_serve_result o = solo.out;
+        o.t_arrival = solo.job.rq.t_arrival; o.t_admit = solo.job.t_admit; o.t_done = solo.t_done;
+        done.push_back(o);
+        st.completed++;
+        st.solo++;
+        return GFE_OK;
+    }
+
+    // A warm engine for the quick tests of n ahead of the first request: the kind of the NTT3
+    // candidates of n (with the limb transform's tables where some b of n uses them).
+    int warm_up(uint32_t n) {
+        uint32_t pick = 0;
+        for (int pass = 0; pass < 2 && !pick; ++pass)
+            for (double x = 8388608.0; x < 2.0e9 && !pick; x *= 1.01) {
+                const uint32_t b = (uint32_t)x & ~1u;
+                const Warm k = warm_kind(b, n);
+                if (k.rns == 3 && (k.limb || pass == 1)) pick = b;
+            }
+        if (!pick) return GFE_OK;
+        const Warm k = warm_kind(pick, n);
+        if (find_warm(k) >= 0 || !make_room(k.bytes)) return GFE_OK;
+        Warm w = k;
+        const int rc = gfe_create(pick, n, 3, device, &w.e);
+        if (rc != GFE_OK) return rc;
+        warm.push_back(w);
+        st.bytes_reserved += w.bytes;
+        st.bytes_peak = std::max(st.bytes_peak, st.bytes_reserved);
+        return GFE_OK;
+    }
 
     void release(size_t i) {
         gfe_batch_destroy(pools[i].bt);
@@ -100,6 +273,7 @@ struct gfe_server {
         auto fits = [&](uint32_t c) {
             return st.bytes_reserved + gfe_batch_proof_device_bytes(n, c, depth) <= st.bytes_budget;
         };
+        if (!fits(1)) make_room(gfe_batch_proof_device_bytes(n, 1, depth));
         if (!fits(1)) {
             for (size_t i = pools.size(); i-- > 0;)
                 if (pools[i].busy == 0 && (pools[i].n != n || pools[i].depth != depth || pools[i].mixed != mixed)) {
@@ -130,6 +304,12 @@ struct gfe_server {
     int admit() {
         for (auto it = queue.begin(); it != queue.end();) {
             const Request &rq = *it;
+            if (!solo.running && rq.depth == 0 && idle() && light() && start_solo(rq)) {
+                st.admitted++;
+                st.queued--;
+                it = queue.erase(it);
+                continue;
+            }
             long pi = -1;
             for (size_t i = 0; i < pools.size(); ++i)
                 if (pool_takes(pools[i], rq.n, rq.depth) && pools[i].busy < pools[i].cap) { pi = (long)i; break; }
@@ -154,6 +334,8 @@ struct gfe_server {
     }
 
     int collect() {
+        const int rs = collect_solo();
+        if (rs != GFE_OK) return rs;
         for (Pool &P : pools) {
             if (P.busy == 0) continue;
             if (buf.size() < P.cap) buf.resize(P.cap);
@@ -201,7 +383,7 @@ struct gfe_server {
         }
         rc = collect();
         if (rc != GFE_OK) return rc;
-        uint32_t busy = 0;
+        uint32_t busy = solo.running ? 1u : 0u;
         for (const Pool &P : pools) busy += P.busy;
         st.busy_slots = busy;
         if (!stepped && busy > 0) std::this_thread::sleep_for(std::chrono::microseconds(50));
@@ -235,16 +417,28 @@ int gfe_server_create(int device, uint64_t hbm_budget, gfe_server **out) {
 
 int gfe_server_destroy(gfe_server *s) {
     if (!s) return GFE_OK;
+    if (s->solo.running) {
+        s->solo.th.join();
+        if (!s->solo.e && s->solo.created.e) gfe_destroy(s->solo.created.e);
+    }
+    for (Warm &w : s->warm) gfe_destroy(w.e);
     for (Pool &P : s->pools) gfe_batch_destroy(P.bt);
     delete s;
     return GFE_OK;
 }
 
+int gfe_server_set_solo(gfe_server *s, int max_rate) {
+    if (!s) return gfe::set_error(GFE_EINVAL, "null server");
+    if (max_rate < 0) return gfe::set_error(GFE_EINVAL, "serve: negative arrival rate");
+    s->solo_rate = max_rate;
+    return GFE_OK;
+}
+
 double gfe_server_clock(const gfe_server *s) { return s ? s->clock() : 0.0; }
 
-int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity) {
-    if (!s) return gfe::set_error(GFE_EINVAL, "null server");
-    if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
+namespace {
+
+int prepare_pool(gfe_server *s, uint32_t n, uint32_t capacity) {
     for (const Pool &P : s->pools)
         if (gfe_server::pool_takes(P, n, 0)) return GFE_OK;  // n <= 13: the mixed pool already serves n
     long pi = -1;
@@ -252,6 +446,16 @@ int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity) {
     if (rc != GFE_OK) return rc;
     if (pi < 0) return gfe::set_error(GFE_ENOMEM, "serve: pool does not fit in the budget");
     return GFE_OK;
+}
+
+}  // namespace
+
+int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity) {
+    if (!s) return gfe::set_error(GFE_EINVAL, "null server");
+    if (n < 12 || n > 23) return gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
+    const int rc = prepare_pool(s, n, capacity);
+    if (rc != GFE_OK || s->solo_rate == 0) return rc;
+    return s->warm_up(n);
 }
 
 int gfe_server_submit(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint64_t tag) {
@@ -266,6 +470,7 @@ int gfe_server_submit_proof(gfe_server *s, uint32_t n, uint32_t b, uint64_t iter
     if (gfe_batch_proof_device_bytes(n, 1, depth) > s->st.bytes_budget)
         return gfe::set_error(GFE_ENOMEM, "serve: one candidate of this n exceeds the budget");
     s->queue.push_back(Request{tag, n, b, depth ? 0 : iters, depth, s->clock()});
+    s->recent.push_back(s->queue.back().t_arrival);
     s->st.submitted++;
     s->st.queued++;
     return GFE_OK;
@@ -304,7 +509,13 @@ int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters
     gfe_server *s = nullptr;
     int rc = gfe_server_create(device, hbm_budget, &s);
     if (rc != GFE_OK) return rc;
-    for (uint32_t i = 0; i < prepare_count && rc == GFE_OK; ++i) rc = gfe_server_prepare(s, prepare_n[i], 0);
+    // every pool first, then the warm engines: streams share the device's few hardware queues in
+    // creation order, and the pools, which run together, should each get one of their own
+    for (uint32_t i = 0; i < prepare_count && rc == GFE_OK; ++i) {
+        if (prepare_n[i] < 12 || prepare_n[i] > 23) rc = gfe::set_error(GFE_EINVAL, "n must be in [12, 23]");
+        else rc = prepare_pool(s, prepare_n[i], 0);
+    }
+    for (uint32_t i = 0; i < prepare_count && rc == GFE_OK; ++i) rc = s->warm_up(prepare_n[i]);
     const double start = s->clock();
     uint32_t next = 0, finished = 0;
     std::vector<gfe_serve_result> got(256);

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_inv_norm_limb(L
                                      lds + prime * cells);
     __syncthreads();
     norm_limb_rows<S>(*A.crt, lds, (uint32_t)A.ng, 1u << LOGG, C, blk, A.dst, A.slots, (uint32_t)(M / (uint32_t)A.ng),
-                      A.dup != 0, reinterpret_cast<int64_t *>(lds + 4 * cells));
+                      A.dup != 0);
 }
 
 template <class M, int LOGE, int LOGG>

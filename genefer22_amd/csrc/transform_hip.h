@@ -133,7 +133,12 @@ public:
         : transform(size_t(1) << n, n, b), _num_regs(num_regs) {
         ck(gfe_create(b, n, (uint32_t)num_regs, device, &_e));
     }
-    ~transformHIP() override { gfe_destroy(_e); }
+    // On an engine the caller keeps (num_regs registers of n): not destroyed with the transform.
+    transformHIP(gfe_engine *borrowed, uint32_t b, uint32_t n, size_t num_regs)
+        : transform(size_t(1) << n, n, b), _e(borrowed), _num_regs(num_regs), _own(false) {}
+    ~transformHIP() override {
+        if (_own) gfe_destroy(_e);
+    }
     transformHIP(const transformHIP &) = delete;
     transformHIP &operator=(const transformHIP &) = delete;
 
@@ -174,6 +179,7 @@ protected:
 private:
     gfe_engine *_e = nullptr;
     size_t _num_regs;
+    bool _own = true;
     static void ck(int rc) {
         if (rc != GFE_OK) throw std::runtime_error(std::string("gfe: ") + gfe_last_error());
     }

@@ -81,6 +81,14 @@ double gfe_get_error(const gfe_engine *e);
 /* transform::info, src/transform.h:72 -- prime set computed in (and the reference's set when it
  * differs) and pass plan as text. */
 int gfe_info(const gfe_engine *e, char *buf, size_t len);
+/* No counterpart in the reference (one candidate per process there): moves the engine to base b of
+ * the same n, for servers that keep engines between candidates instead of building their twiddle
+ * tables again.  b must select the prime set the engine computes in (GFE_EKIND otherwise; create
+ * an engine for it); the CRT constants, the carry rounds and the plan become b's -- the two-digit-
+ * limb transform where it is exact for b and the engine was created with its tables (for a b that
+ * has it), else the digit transform's default.  The registers keep their digits: the caller sets
+ * them (gfe_set, gfe_set_digits) before squaring. */
+int gfe_rebase(gfe_engine *e, uint32_t b);
 /* Waits for the engine's queued work. */
 int gfe_sync(gfe_engine *e);
 /* The reference's prime-set selection without creating an engine: *rns = 2 or 3, *is32 = 0 (31-bit)

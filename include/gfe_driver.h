@@ -24,6 +24,13 @@ typedef struct gfe_result {
     double seconds;    /* wall time of the run */
 } gfe_result;
 
+/* Host only (no device): the top bits of the PRP exponent E = b^(2^n) that a run of the PRP loop's
+ * first k squarings reads (src/genefer.h:479-552), without computing E -- T = floor(E / 2^shift)
+ * with at least k bits, exact, as little-endian 64-bit words (*nwords of them, at most max_words),
+ * and *esize = bits of E.  The batch and the serving loop use it for truncated runs. */
+int gfe_exponent_top(uint32_t b, uint32_t n, uint64_t k, uint64_t *words, uint32_t max_words, uint32_t *nwords,
+                     uint64_t *shift, uint64_t *esize);
+
 /* genefer's quick test (quick(), src/genefer.h:736-748): PRP of b^(2^n)+1 in base 2 with the
  * Gerbicz-Li check at the reference's cadence B_GL = B_GerbiczLi(esize). */
 int gfe_run_quick(uint32_t b, uint32_t n, int device, gfe_result *out);

@@ -13,6 +13,15 @@
  * squaring rounds on each pool whose device backlog is below two blocks, so the pools' work
  * overlaps on the device.  A candidate's residues do not depend on what else is admitted.
  *
+ * Lone requests: at light load (at most `max_rate` arrivals in the last second, gfe_server_set_solo;
+ * 4 by default) a quick or truncated test that is admitted while no pool has work runs alone on a
+ * warm single-candidate engine of its kind (one per n and prime set, moved between bases with
+ * gfe_rebase; gfe_server_prepare warms the one for n's NTT3 candidates), on a host thread, exactly
+ * as gfe_run_quick / gfe_run_prp run it: the request then takes its time alone on the plain
+ * engine instead of a batch's at one active slot.  Requests that arrive meanwhile go to the pools;
+ * under heavier load every request does, since a lone run would hold the device while the pools
+ * fill.  Warm engines count in the HBM budget and are freed when a pool needs the room.
+ *
  * Conventions of gfe.h: GFE_OK or a negative code, gfe_last_error() for the message; one server
  * must not be used from two threads at once.
  */
@@ -50,6 +59,7 @@ typedef struct gfe_server_stats {
     uint64_t bytes_budget, bytes_reserved, bytes_peak;
     uint32_t pools, pools_created, pools_released, busy_slots;
     uint64_t pumps, blocks;    /* pump calls; squaring blocks enqueued */
+    uint64_t solo;             /* requests run alone on a warm single-candidate engine */
 } gfe_server_stats;
 
 /* A server on `device` whose pools may hold at most `hbm_budget` device bytes (0: 3/4 of the
@@ -59,8 +69,13 @@ int gfe_server_destroy(gfe_server *s);
 /* Seconds since the server was created (steady clock). */
 double gfe_server_clock(const gfe_server *s);
 /* Creates a pool for n ahead of its first request (`capacity` slots, 0 = default for n); for
- * n <= 13 the mixed pool, unless one exists. */
+ * n <= 13 the mixed pool, unless one exists; and, with the lone-request path on, a warm engine for
+ * n's NTT3 candidates. */
 int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity);
+/* The lone-request path while at most `max_rate` requests arrived in the last second (default 4,
+ * about two fifths of one plain engine's capacity for a mix of n = 12..20); 0: every request
+ * through a pool. */
+int gfe_server_set_solo(gfe_server *s, int max_rate);
 /* Queues a request (arrival time = now). */
 int gfe_server_submit(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint64_t tag);
 /* Queues a proof request: the full test and the Pietrzak-Li proof of `depth` (1 ..

@@ -68,3 +68,45 @@ def test_server_api_full_quick_test(gfe):
     st = s.stats()
     assert st["completed"] == 2 and st["pools"] >= 1
     s.close()
+
+
+def test_lone_requests_run_alone_on_warm_engines(gfe):
+    """A request admitted while no pool has work runs alone on a warm single-candidate engine
+    (stats['solo']); its result equals the plain driver's, for truncated and full quick tests, for
+    bases of the limb transform, of the digit transform and of NTT2 (an engine created on demand)."""
+    reqs = [(16, 20204692, 500), (16, 1500000000, 500), (16, 4098, 300), (12, 1000000000, 0), (18, 30000000, 400)]
+    s = gfe.Server()
+    s.set_solo(100)  # requests that follow each other within a second still count as light load
+    s.prepare(16)
+    got = []
+    for k, (n, b, iters) in enumerate(reqs):
+        s.submit(n, b, iters, tag=k)
+        for _ in range(2000000):
+            s.pump()
+            r = s.poll()
+            if r:
+                got += r
+                break
+    st = s.stats()
+    s.close()
+    assert st["solo"] == len(reqs) and st["completed"] == len(reqs)
+    for (n, b, iters), g in zip(reqs, sorted(got, key=lambda r: r["tag"])):
+        want = gfe.run_quick(b, n) if iters == 0 else gfe.run_prp(b, n, iters=iters, b_gl=-1)
+        assert (g["res64"], g["old64"], g["hash64"]) == (want["res64"], want["old64"], want["hash64"]), (n, b)
+        assert g["gl_ok"] == (1 if iters == 0 else -1)
+
+
+def test_lone_path_off_runs_every_request_in_a_pool(gfe):
+    s = gfe.Server()
+    s.set_solo(0)
+    s.submit(14, 20204692, 300, tag=1)
+    got = []
+    for _ in range(2000000):
+        s.pump()
+        got += s.poll()
+        if got:
+            break
+    st = s.stats()
+    s.close()
+    assert st["solo"] == 0 and st["pools_created"] >= 1
+    assert got[0]["res64"] == gfe.run_prp(20204692, 14, iters=300, b_gl=-1)["res64"]
