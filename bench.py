@@ -269,17 +269,25 @@ def batch_end_to_end_c3(G, bases, plan):
             "mismatches": bad[:8], "match": not bad}
 
 
-def plain_ms_by_n(G, reqs) -> dict:
-    """ms per squaring of one candidate on the plain path with the library's default plan, per n."""
-    b_of_n = {q.n: q.b for q in reqs}
-    out = {}
-    for n in sorted(b_of_n):
-        t = G.Transform(b_of_n[n], n)
+def engine_kind(G, b: int, n: int) -> tuple:
+    """What a plain engine of (b, n) runs: n, its plan (digit or limb transform, fused or not) and
+    the prime set the reference's rule selects."""
+    return (n, tuple(G.engine_plan(b, n)), tuple(G.select(b, n)))
+
+
+def plain_ms_by_kind(G, reqs, cache: dict) -> dict:
+    """ms per squaring of one candidate on the plain path, per engine kind of the requests (one
+    representative b each, the library's default plan), added to `cache`."""
+    for q in reqs:
+        k = engine_kind(G, q.b, q.n)
+        if k in cache:
+            continue
+        t = G.Transform(q.b, q.n)
         t.set(1)
         t.time_squarings(32)
-        out[n] = t.time_squarings(256) / 256
+        cache[k] = t.time_squarings(256) / 256
         t.close()
-    return out
+    return cache
 
 
 def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
@@ -288,18 +296,20 @@ def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
     through the native serving loop; p50/p99 time-to-result and requests/hour, against the same
     requests run one after another on the plain one-candidate path (per-n ms/squaring measured on
     the library's default plans) and against each request's own plain run time (p50 of
-    iters x ms/squaring of its n: the latency floor of one request alone on the plain path)."""
+    iters x ms/squaring of its engine kind -- n, plan and prime set, which differ between bases of one
+    n: the latency floor of one request alone on the plain path, set-up and read-out excluded)."""
     from genefer22_amd import serve
     reqs = W.serving_requests(count, rate, iters=iters)
     t0 = time.perf_counter()
     _, summary, stats = serve.run(reqs)
     wall = time.perf_counter() - t0
-    missing = sorted({q.n for q in reqs} - set(ms_cache))
-    if missing:
-        ms_cache.update(plain_ms_by_n(G, [q for q in reqs if q.n in missing]))
-    ms_by_n = {n: ms_cache[n] for n in sorted({q.n for q in reqs})}
-    serial = serve.plain_serial_seconds(reqs, ms_by_n)
-    alone = sorted(q.iters * ms_by_n[q.n] / 1e3 for q in reqs)
+    plain_ms_by_kind(G, reqs, ms_cache)
+    ms_of = [ms_cache[engine_kind(G, q.b, q.n)] for q in reqs]
+    serial = sum(q.iters * m for q, m in zip(reqs, ms_of)) / 1e3
+    alone = sorted(q.iters * m / 1e3 for q, m in zip(reqs, ms_of))
+    ms_by_n = {}
+    for q, m in zip(reqs, ms_of):
+        ms_by_n.setdefault(q.n, []).append(m)
     return {
         "requests": count, "arrival_rate_per_s": rate, "squarings_per_request": iters,
         "p50_s": summary["p50_s"], "p99_s": summary["p99_s"], "makespan_s": summary["makespan_s"],
@@ -312,7 +322,8 @@ def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
                       f"take 13-27 M squarings each)",
         "plain_serial_requests_per_hour": round(count / serial * 3600.0, 1),
         "speedup_vs_plain_serial": round(serial / summary["makespan_s"], 2),
-        "plain_ms_per_squaring_by_n": {int(k): round(v, 5) for k, v in ms_by_n.items()},
+        "plain_ms_per_squaring_by_n": {int(k): round(sum(v) / len(v), 5) for k, v in sorted(ms_by_n.items())},
+        "plain_kinds_measured": len(ms_cache),
         "pools_created": stats["pools_created"], "bytes_peak": stats["bytes_peak"], "wall_s": round(wall, 2),
         "lone_runs": stats.get("solo", 0),
     }

@@ -52,8 +52,9 @@ struct Admitted {
     double t_admit;
 };
 
-// A warm single-candidate engine of the lone-request path, for one (n, prime set computed in,
-// limb tables) and any b of that kind (gfe_rebase).
+// A warm single-candidate engine of the lone-request path, for one (n, prime set computed in) and
+// any b of that set (gfe_rebase); one created with the limb transform's tables serves both the
+// bases that use it and those that do not.
 struct Warm {
     gfe_engine *e = nullptr;
     uint32_t n = 0;
@@ -89,6 +90,9 @@ struct Pool {
 
 }  // namespace
 
+// A lone run of at most this many squarings is short enough for light-load requests to wait for.
+constexpr uint64_t SOLO_SHORT = 65536;
+
 struct gfe_server {
     int device = 0;
     double t0 = 0;
@@ -122,9 +126,12 @@ struct gfe_server {
         w.bytes = gfe::engine_bytes(w.rns, size_t(1) << n, 3, w.limb);
         return w;
     }
+    // A warm engine that can take kind k: same n and prime set, and the limb transform's tables
+    // when k uses them (an engine with them serves the other bases of its set too).
     long find_warm(const Warm &k) const {
         for (size_t i = 0; i < warm.size(); ++i)
-            if (warm[i].n == k.n && warm[i].rns == k.rns && warm[i].is32 == k.is32 && warm[i].limb == k.limb) return (long)i;
+            if (warm[i].n == k.n && warm[i].rns == k.rns && warm[i].is32 == k.is32 && (warm[i].limb || !k.limb))
+                return (long)i;
         return -1;
     }
     // Frees warm engines not in use until `need` more bytes fit in the budget.
@@ -304,11 +311,19 @@ struct gfe_server {
     int admit() {
         for (auto it = queue.begin(); it != queue.end();) {
             const Request &rq = *it;
-            if (!solo.running && rq.depth == 0 && idle() && light() && start_solo(rq)) {
-                st.admitted++;
-                st.queued--;
-                it = queue.erase(it);
-                continue;
+            if (rq.depth == 0 && idle() && light()) {
+                // at light load a request waits for a short lone run rather than share the device
+                // with it from a pool: the wait is shorter than the slowdown of both
+                if (solo.running && solo.job.rq.iters != 0 && solo.job.rq.iters <= SOLO_SHORT) {
+                    ++it;
+                    continue;
+                }
+                if (!solo.running && start_solo(rq)) {
+                    st.admitted++;
+                    st.queued--;
+                    it = queue.erase(it);
+                    continue;
+                }
             }
             long pi = -1;
             for (size_t i = 0; i < pools.size(); ++i)
