@@ -153,7 +153,8 @@ ABI = [
     ("gfe_server_get_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
     ("gfe_server_set_solo", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("gfe_serve_replay", ctypes.c_int, [U32P, U32P, U64P, U32P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint32, ctypes.c_int,
-                                        ctypes.c_uint64, U32P, ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+                                        ctypes.c_uint64, U32P, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.POINTER(ServerStats)]),
 ]
 
 
@@ -597,7 +598,7 @@ class Server:
     def set_solo(self, max_rate: int):
         """The lone-request path (a request admitted while no pool has work runs alone on a warm
         single-candidate engine) while at most `max_rate` requests arrived in the last second
-        (default 4); 0 turns it off."""
+        (default 8); 0 turns it off."""
         _ck(lib().gfe_server_set_solo(self._h, int(max_rate)))
 
     def submit(self, n: int, b: int, iters: int = 0, tag: int = 0, depth: int = 0):
@@ -621,9 +622,10 @@ class Server:
         return st.as_dict()
 
 
-def serve_replay(requests, device: int = 0, hbm_budget: int = 0, prepare=()) -> tuple[list[dict], dict]:
+def serve_replay(requests, device: int = 0, hbm_budget: int = 0, prepare=(), solo_rate: int = -1) -> tuple[list[dict], dict]:
     """Open-loop replay of (n, b, iters, arrival_seconds[, proof depth]) requests through the
-    serving loop.  Returns (results in request order, server statistics)."""
+    serving loop (solo_rate: the lone-request path's arrival-rate limit, -1 the default, 0 off).
+    Returns (results in request order, server statistics)."""
     k = len(requests)
     n = np.ascontiguousarray([r[0] for r in requests], dtype=np.uint32)
     b = np.ascontiguousarray([r[1] for r in requests], dtype=np.uint32)
@@ -636,5 +638,5 @@ def serve_replay(requests, device: int = 0, hbm_budget: int = 0, prepare=()) -> 
     _ck(lib().gfe_serve_replay(n.ctypes.data_as(U32P), b.ctypes.data_as(U32P), it.ctypes.data_as(U64P),
                                dp.ctypes.data_as(U32P), t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k, device,
                                hbm_budget,
-                               pn.ctypes.data_as(U32P), len(prepare), res, ctypes.byref(st)))
+                               pn.ctypes.data_as(U32P), len(prepare), int(solo_rate), res, ctypes.byref(st)))
     return [res[i].as_dict() for i in range(k)], st.as_dict()

@@ -65,7 +65,10 @@ struct Warm {
 
 // The lone-request path: a quick or truncated test that arrives while no pool has work runs alone
 // on a warm engine, on a host thread, as gfe_run_quick / gfe_run_prp run it (the same residues as
-// in a pool); requests that arrive meanwhile go to the pools.
+// in a pool).  SOLO_WORKERS such runs may go side by side: a single candidate below n = 18 is
+// bound by launch latency and leaves most of the device idle.
+constexpr int SOLO_WORKERS = 2;
+
 struct Solo {
     std::thread th;
     std::atomic<bool> finished{false};
@@ -103,10 +106,21 @@ struct gfe_server {
     std::unordered_map<uint64_t, Admitted> inflight;  // by batch tag
     std::deque<gfe_serve_result> done;
     std::vector<gfe_batch_result> buf;
-    int solo_rate = 4;            // the lone-request path while at most this many arrivals in the last second
+    int solo_rate = 8;            // the lone-request path while at most this many arrivals in the last second
     std::deque<double> recent;    // arrival times of the last second
-    Solo solo;
+    Solo solos[SOLO_WORKERS];
     std::vector<Warm> warm;
+
+    bool in_use(const gfe_engine *e) const {
+        for (const Solo &S : solos)
+            if (S.running && S.e == e) return true;
+        return false;
+    }
+    int solos_running() const {
+        int r = 0;
+        for (const Solo &S : solos) r += S.running ? 1 : 0;
+        return r;
+    }
 
     // Light load: few enough arrivals in the last second that a lone run does not hold up the pools.
     bool light() {
@@ -130,14 +144,15 @@ struct gfe_server {
     // when k uses them (an engine with them serves the other bases of its set too).
     long find_warm(const Warm &k) const {
         for (size_t i = 0; i < warm.size(); ++i)
-            if (warm[i].n == k.n && warm[i].rns == k.rns && warm[i].is32 == k.is32 && (warm[i].limb || !k.limb))
+            if (warm[i].n == k.n && warm[i].rns == k.rns && warm[i].is32 == k.is32 && (warm[i].limb || !k.limb) &&
+                !in_use(warm[i].e))
                 return (long)i;
         return -1;
     }
     // Frees warm engines not in use until `need` more bytes fit in the budget.
     bool make_room(uint64_t need) {
         for (size_t i = warm.size(); i-- > 0 && st.bytes_reserved + need > st.bytes_budget;) {
-            if (solo.running && warm[i].e == solo.e) continue;
+            if (in_use(warm[i].e)) continue;
             gfe_destroy(warm[i].e);
             st.bytes_reserved -= warm[i].bytes;
             warm.erase(warm.begin() + (long)i);
@@ -150,9 +165,9 @@ struct gfe_server {
         return inflight.empty();
     }
 
-    // Runs on the lone-request thread: touches only `solo` and its engine.
-    void solo_work() {
-        Solo &S = solo;
+    // Runs on a lone-request thread: touches only its Solo and its engine.
+    void solo_work(Solo *Sp) {
+        Solo &S = *Sp;
         const Request rq = S.job.rq;
         try {
             gfe_engine *e = S.e;
@@ -188,7 +203,7 @@ struct gfe_server {
 
     // Starts `rq` alone on a warm engine of its kind (one is created on the thread when none is
     // warm and it fits the budget); false: the request takes the pool path.
-    bool start_solo(const Request &rq) {
+    bool start_solo(Solo &solo, const Request &rq) {
         const Warm k = warm_kind(rq.b, rq.n);
         const long wi = find_warm(k);
         if (wi < 0) {
@@ -203,7 +218,7 @@ struct gfe_server {
         solo.err.clear();
         solo.finished.store(false);
         try {
-            solo.th = std::thread(&gfe_server::solo_work, this);
+            solo.th = std::thread(&gfe_server::solo_work, this, &solo);
         } catch (const std::exception &) {
             if (wi < 0) st.bytes_reserved -= k.bytes;
             return false;
@@ -212,7 +227,7 @@ struct gfe_server {
         return true;
     }
 
-    int collect_solo() {
+    int collect_solo(Solo &solo) {
         if (!solo.running || !solo.finished.load(std::memory_order_acquire)) return GFE_OK;
         solo.th.join();
         solo.running = false;
@@ -312,16 +327,22 @@ struct gfe_server {
         for (auto it = queue.begin(); it != queue.end();) {
             const Request &rq = *it;
             if (rq.depth == 0 && idle() && light()) {
-                // at light load a request waits for a short lone run rather than share the device
-                // with it from a pool: the wait is shorter than the slowdown of both
-                if (solo.running && solo.job.rq.iters != 0 && solo.job.rq.iters <= SOLO_SHORT) {
-                    ++it;
-                    continue;
+                Solo *free_slot = nullptr;
+                bool short_run = false;
+                for (Solo &S : solos) {
+                    if (!S.running) free_slot = free_slot ? free_slot : &S;
+                    else if (S.job.rq.iters != 0 && S.job.rq.iters <= SOLO_SHORT) short_run = true;
                 }
-                if (!solo.running && start_solo(rq)) {
+                if (free_slot && start_solo(*free_slot, rq)) {
                     st.admitted++;
                     st.queued--;
                     it = queue.erase(it);
+                    continue;
+                }
+                // at light load a request waits for a short lone run rather than share the device
+                // with it from a pool: the wait is shorter than the slowdown of both
+                if (!free_slot && short_run) {
+                    ++it;
                     continue;
                 }
             }
@@ -349,8 +370,10 @@ struct gfe_server {
     }
 
     int collect() {
-        const int rs = collect_solo();
-        if (rs != GFE_OK) return rs;
+        for (Solo &S : solos) {
+            const int rs = collect_solo(S);
+            if (rs != GFE_OK) return rs;
+        }
         for (Pool &P : pools) {
             if (P.busy == 0) continue;
             if (buf.size() < P.cap) buf.resize(P.cap);
@@ -398,7 +421,7 @@ struct gfe_server {
         }
         rc = collect();
         if (rc != GFE_OK) return rc;
-        uint32_t busy = solo.running ? 1u : 0u;
+        uint32_t busy = (uint32_t)solos_running();
         for (const Pool &P : pools) busy += P.busy;
         st.busy_slots = busy;
         if (!stepped && busy > 0) std::this_thread::sleep_for(std::chrono::microseconds(50));
@@ -432,10 +455,11 @@ int gfe_server_create(int device, uint64_t hbm_budget, gfe_server **out) {
 
 int gfe_server_destroy(gfe_server *s) {
     if (!s) return GFE_OK;
-    if (s->solo.running) {
-        s->solo.th.join();
-        if (!s->solo.e && s->solo.created.e) gfe_destroy(s->solo.created.e);
-    }
+    for (Solo &S : s->solos)
+        if (S.running) {
+            S.th.join();
+            if (!S.e && S.created.e) gfe_destroy(S.created.e);
+        }
     for (Warm &w : s->warm) gfe_destroy(w.e);
     for (Pool &P : s->pools) gfe_batch_destroy(P.bt);
     delete s;
@@ -517,13 +541,14 @@ int gfe_server_get_stats(const gfe_server *s, gfe_server_stats *out) {
 
 int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters, const uint32_t *depth,
                      const double *arrival, uint32_t count, int device, uint64_t hbm_budget, const uint32_t *prepare_n,
-                     uint32_t prepare_count, gfe_serve_result *results, gfe_server_stats *stats) {
+                     uint32_t prepare_count, int solo_rate, gfe_serve_result *results, gfe_server_stats *stats) {
     if (!n || !b || !iters || !arrival || !results || count == 0) return gfe::set_error(GFE_EINVAL, "replay arguments");
     for (uint32_t k = 1; k < count; ++k)
         if (arrival[k] < arrival[k - 1]) return gfe::set_error(GFE_EINVAL, "replay: arrivals must be non-decreasing");
     gfe_server *s = nullptr;
     int rc = gfe_server_create(device, hbm_budget, &s);
     if (rc != GFE_OK) return rc;
+    if (solo_rate >= 0) s->solo_rate = solo_rate;
     // every pool first, then the warm engines: streams share the device's few hardware queues in
     // creation order, and the pools, which run together, should each get one of their own
     for (uint32_t i = 0; i < prepare_count && rc == GFE_OK; ++i) {

@@ -14,11 +14,13 @@
  * overlaps on the device.  A candidate's residues do not depend on what else is admitted.
  *
  * Lone requests: at light load (at most `max_rate` arrivals in the last second, gfe_server_set_solo;
- * 4 by default) a quick or truncated test that is admitted while no pool has work runs alone on a
+ * 8 by default) a quick or truncated test that is admitted while no pool has work runs alone on a
  * warm single-candidate engine of its kind (one per n and prime set, moved between bases with
  * gfe_rebase; gfe_server_prepare warms the one for n's NTT3 candidates), on a host thread, exactly
  * as gfe_run_quick / gfe_run_prp run it: the request then takes its time alone on the plain
- * engine instead of a batch's at one active slot.  Requests that arrive meanwhile go to the pools;
+ * engine instead of a batch's at one active slot.  Two such runs may go side by side (a single
+ * candidate below n = 18 leaves most of the device idle); a request that finds both busy waits for
+ * them when one is short (at most 65,536 squarings), else it goes to the pools;
  * under heavier load every request does, since a lone run would hold the device while the pools
  * fill.  Warm engines count in the HBM budget and are freed when a pool needs the room.
  *
@@ -72,9 +74,9 @@ double gfe_server_clock(const gfe_server *s);
  * n <= 13 the mixed pool, unless one exists; and, with the lone-request path on, a warm engine for
  * n's NTT3 candidates. */
 int gfe_server_prepare(gfe_server *s, uint32_t n, uint32_t capacity);
-/* The lone-request path while at most `max_rate` requests arrived in the last second (default 4,
- * about two fifths of one plain engine's capacity for a mix of n = 12..20); 0: every request
- * through a pool. */
+/* The lone-request path while at most `max_rate` requests arrived in the last second (default 8,
+ * about four fifths of one plain engine's capacity for a mix of n = 12..20; a saturated mix sees
+ * two to three times that); 0: every request through a pool. */
 int gfe_server_set_solo(gfe_server *s, int max_rate);
 /* Queues a request (arrival time = now). */
 int gfe_server_submit(gfe_server *s, uint32_t n, uint32_t b, uint64_t iters, uint64_t tag);
@@ -92,11 +94,13 @@ int gfe_server_get_stats(const gfe_server *s, gfe_server_stats *out);
 
 /* Open-loop replay: request k (n[k], b[k], iters[k], proof depth depth[k] -- depth may be null:
  * no proofs) is submitted when the server clock reaches arrival[k] seconds (non-decreasing); runs
- * until every request has finished.  results[k] is the
- * result of request k; stats (optional) receives the final statistics. */
+ * until every request has finished.  prepare_n lists the n whose pools (and warm engines) are made
+ * before the clock starts; solo_rate is gfe_server_set_solo's max_rate (-1: the default, 0: every
+ * request through a pool).  results[k] is the result of request k; stats (optional) receives the
+ * final statistics. */
 int gfe_serve_replay(const uint32_t *n, const uint32_t *b, const uint64_t *iters, const uint32_t *depth,
                      const double *arrival, uint32_t count, int device, uint64_t hbm_budget, const uint32_t *prepare_n,
-                     uint32_t prepare_count, gfe_serve_result *results, gfe_server_stats *stats);
+                     uint32_t prepare_count, int solo_rate, gfe_serve_result *results, gfe_server_stats *stats);
 
 #ifdef __cplusplus
 }

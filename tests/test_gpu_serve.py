@@ -38,7 +38,7 @@ def test_serve_alone_equals_mix_under_a_small_budget(gfe):
     iters = 700
     budget = gfe.lib().gfe_batch_device_bytes(14, 2) + 4096
     reqs = [(n, b, iters, 0.002 * k) for k, (n, b) in enumerate(mix)]
-    results, stats = gfe.serve_replay(reqs, hbm_budget=budget)
+    results, stats = gfe.serve_replay(reqs, hbm_budget=budget, solo_rate=0)  # pools only
     assert stats["bytes_peak"] <= budget
     assert stats["pools_released"] >= 1
     for (n, b), g in zip(mix, results):
@@ -46,10 +46,13 @@ def test_serve_alone_equals_mix_under_a_small_budget(gfe):
         assert (g["res64"], g["old64"], g["hash64"]) == (alone["res64"], alone["old64"], alone["hash64"]), (n, b)
 
 
-def test_server_api_full_quick_test(gfe):
+@pytest.mark.parametrize("solo_rate", [0, 8])
+def test_server_api_full_quick_test(gfe, solo_rate):
     """submit / pump / poll: a full quick test (PRP + Gerbicz-Li) through the server equals the
-    reference's known answer (src/main.cpp:424) and the plain driver's run."""
+    reference's known answer (src/main.cpp:424) and the plain driver's run, through the pools
+    (solo_rate 0) and through the lone-request path (the default)."""
     s = gfe.Server()
+    s.set_solo(solo_rate)
     s.submit(12, 1000000000, 0, tag=7)
     s.submit(13, 2000000000, 500, tag=8)
     got = []
@@ -66,7 +69,8 @@ def test_server_api_full_quick_test(gfe):
     r8 = next(r for r in got if r["tag"] == 8)
     assert r8["res64"] == gfe.run_prp(2000000000, 13, iters=500, b_gl=-1)["res64"]
     st = s.stats()
-    assert st["completed"] == 2 and st["pools"] >= 1
+    assert st["completed"] == 2
+    assert (st["pools"] >= 1 and st["solo"] == 0) if solo_rate == 0 else st["solo"] == 2
     s.close()
 
 
