@@ -821,6 +821,10 @@ unsigned Engine::resident_blocks(const void *fn, int threads, size_t lds) {
 
 // Forward column pass from register `src` into `dst` (ws or zp).
 void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
+    // the fused limb plans' forward pass stages the four primes' digits once (ks_col_fwd_limb_all)
+    // and takes only the range carries
+    const bool staged = plan.limb && plan.fused && plan.spec && spec_col_fwd_limb_all(plan.col_loge, plan.a) != nullptr;
+    if (staged && carry_pending && pend_kind != 2) flush_carry();
     // the pending carries enter this pass when it reads register 0 for a squaring or product and
     // knows their form (the digit transform: k_norm's; the limb transform: either limb form)
     const bool absorb = carry_pending && src == 0 && defer_carry() && (pend_kind != 0) == (plan.limb != 0) &&
@@ -843,6 +847,11 @@ void Engine::launch_col_fwd(uint32_t src, uint32_t *dst) {
         }
         const uint32_t C = 1u << (tn() - plan.a);
         const int T = plan.col_ng * (1 << plan.a) / (1 << plan.col_loge);
+        if (staged) {
+            hipLaunchKernelGGL(spec_col_fwd_limb_all(plan.col_loge, plan.a), dim3(C / plan.col_ng, 1, 1), dim3(4 * T),
+                               limb_col_fwd_all_lds_bytes(plan.a, plan.col_loge, plan.col_ng), stream, A);
+            return;
+        }
         hipLaunchKernelGGL(spec_col_fwd_limb(plan.col_loge, plan.a), dim3(C / plan.col_ng, 4, 1), dim3(T),
                            col_pass_lds_words(plan.a, plan.col_loge, plan.col_ng) * 4, stream, A);
         return;

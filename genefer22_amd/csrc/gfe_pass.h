@@ -441,6 +441,78 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
     });
 }
 
+// The limb transform's forward column pass for the four primes in one workgroup (ks_col_fwd_limb_all):
+// the group's 2^LOGG x NG digit pairs are loaded once, with wide loads along the rows, into LDS as
+// D[k NG + j] = limb k C + blk NG + j, the range carries of the fused limb normalisation added to each
+// row's first two limbs on the way (`ls`, when not null: the carry into range k C / NG + blk is
+// ls[.] = (u0, u1, v, 0), (u0, u1) onto limb 0's digit pair and v onto limb 1's first digit); each
+// prime's threads then read their column elements from LDS.  S = NG 2^LOGG / blockDim.x limbs per
+// thread, S even.
+template <int S>
+__device__ __forceinline__ void stage_limb_digits(const int32_t *__restrict__ src, const int4 *__restrict__ ls,
+                                                  uint32_t C, uint32_t NG, unsigned blk, int2 *D) {
+    static_assert(S >= 2 && S % 2 == 0, "whole int4 loads of two limbs");
+    const uint32_t tid = threadIdx.x, tpr = NG / S;
+    const uint32_t k = tid / tpr, j0 = (tid % tpr) * S;
+    const int4 *g = reinterpret_cast<const int4 *>(src + 2 * ((size_t)k * C + (size_t)blk * NG + j0));
+    int4 cy = make_int4(0, 0, 0, 0);
+    if (ls && j0 == 0) cy = ls[(size_t)k * (C / NG) + blk];
+    int4 v[S / 2];
+#pragma unroll
+    for (int i = 0; i < S / 2; ++i) v[i] = g[i];
+    v[0].x += cy.x;
+    v[0].y += cy.y;
+    v[0].z += cy.z;
+    int4 *o = reinterpret_cast<int4 *>(D + (size_t)k * NG + j0);
+#pragma unroll
+    for (int i = 0; i < S / 2; ++i) o[i] = v[i];
+}
+
+// As col_fwd_pass (LIMB) for prime-group thread t on its own exchange region L, the first chunk's
+// digit pairs read from the staged D.  D overlaps the exchange regions: the first chunk's LDS
+// writes wait for a workgroup barrier after every group's reads of D.
+template <int LOGE, int LOGG>
+__device__ __forceinline__ void col_fwd_limb_staged(const ModS &md, const int2 *D, uint32_t *__restrict__ dst,
+                                                    const uint32_t *__restrict__ tw, uint32_t C, int NG, unsigned blk,
+                                                    uint32_t *L, uint32_t t, uint32_t bm) {
+    using GM = Geo<LOGG, LOGE>;
+    constexpr int E = 1 << LOGE;
+    const uint32_t q = t & (uint32_t)(NG - 1), s = t / (uint32_t)NG;
+    const uint32_t c = blk * (uint32_t)NG + q;
+    L += q * col_pass_stride(LOGG, LOGE, NG);
+    uint32_t x[E], w[E];
+    static_for<0, GM::NCH>([&](auto cc) {
+        constexpr int ci = decltype(cc)::value;
+        constexpr int lam0 = GM::lam0(ci), NE = GM::ne(ci), YL = GM::ylog(ci);
+        const uint32_t hi = s >> YL, base = (hi << (LOGG - lam0)) + (s & ((1u << YL) - 1u));
+        load_fwd_tw<LOGE>(w, tw, (1u << lam0) + hi, NE);
+        if constexpr (ci == 0) {
+            const int2 *p = D + (size_t)base * NG + q;
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) {
+                const int2 v = p[(size_t)((uint32_t)rr << YL) * NG];
+                x[rr] = md.add(to_res(v.x, md.p), md.mulw(to_res(v.y, md.p), bm));
+            }
+        } else {
+            __syncthreads();
+            const uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) x[rr] = p[chunk_off<YL, LOGE>(rr)];
+        }
+        fwd_chunk<ModS, LOGE, NE>(md, x, w);
+        if constexpr (ci == GM::NCH - 1) {
+            const uint32_t off = (base * C + c) * 4u;
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) *at_bytes(dst + (size_t)((uint32_t)rr << YL) * C, off) = x[rr];
+        } else {
+            __syncthreads();  // chunk 0: after every group's reads of D
+            uint32_t *p = L + base + (base >> LOGE);
+#pragma unroll
+            for (int rr = 0; rr < E; ++rr) p[chunk_off<YL, LOGE>(rr)] = x[rr];
+        }
+    });
+}
+
 // Inverse levels a-1..0 of one prime, in place on the residues `data`.
 template <class M, int LOGE, int LOGG>
 __device__ __forceinline__ void col_inv_pass(const M &md, uint32_t *__restrict__ data, const uint32_t *__restrict__ tw,
@@ -547,6 +619,13 @@ __host__ __device__ inline size_t limb_col_norm_lds_bytes(int logg, int loge, in
     const size_t ex = 4 * col_pass_lds_words(logg, loge, ng);
     const size_t rows = 4 * ((size_t)ng << logg);
     return 4 * (ex > rows ? ex : rows);
+}
+
+// LDS of ks_col_fwd_limb_all: the staged digit pairs (8 bytes per limb) or the four exchange regions.
+__host__ __device__ inline size_t limb_col_fwd_all_lds_bytes(int logg, int loge, int ng) {
+    const size_t ex = 16 * col_pass_lds_words(logg, loge, ng);
+    const size_t st = 8 * ((size_t)ng << logg);
+    return ex > st ? ex : st;
 }
 
 __host__ __device__ inline bool limb_col_norm_ok(int logg, int loge, int ng) {

@@ -18,6 +18,9 @@ typedef void (*SpecBatchRowFn)(BArgs);
 SpecColFn spec_col_fwd(int is32, int loge, int logg);
 SpecColFn spec_col_inv(int is32, int loge, int logg);
 SpecColFn spec_col_fwd_limb(int loge, int logg);  // two-digit-limb transform, 31-bit primes
+// Its four primes in one workgroup (the fused plans' shape: grid column groups, 4 ng 2^logg / 2^loge
+// threads, limb_col_fwd_all_lds_bytes of LDS; null outside limb_col_norm_ok).
+SpecColFn spec_col_fwd_limb_all(int loge, int logg);
 // The limb transform's inverse column pass fused with its normalisation (grid: column groups,
 // 4 ng 2^logg / 2^loge threads, limb_col_norm_lds_bytes of LDS; null outside limb_col_norm_ok).
 SpecLimbNormFn spec_col_inv_norm_limb(int loge, int logg);

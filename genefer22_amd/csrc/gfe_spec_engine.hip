@@ -29,6 +29,28 @@ __global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd_limb(ColArg
                                          xcd_block(blockIdx.x, gridDim.x), lds, A.wc, A.b, A.bm[prime], A.ls, A.rlog);
 }
 
+// Forward column pass of the two-digit-limb transform for the four primes in one workgroup (the
+// fused plans' shape: threads [p T, (p + 1) T) for prime p): the group's digit pairs are loaded once
+// into LDS with the range carries of the last normalisation (A.ls) added, then each prime's threads
+// run its forward column levels from there.
+template <int LOGE, int LOGG>
+__global__ void __launch_bounds__(LOGE >= 5 ? 512 : 1024) ks_col_fwd_limb_all(ColArgs A) {
+    extern __shared__ uint32_t lds[];
+    constexpr int S = (1 << LOGE) / 4;
+    const uint32_t T = ((uint32_t)A.ng << LOGG) >> LOGE;
+    const uint32_t prime = threadIdx.x / T, t = threadIdx.x - prime * T;
+    const ModS md{A.pr.p[prime], A.pr.q[prime]};
+    const size_t M = (size_t)1 << A.n;
+    const uint32_t C = (uint32_t)(M >> LOGG);
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    int2 *D = reinterpret_cast<int2 *>(lds);
+    stage_limb_digits<S>(A.src, A.ls, C, (uint32_t)A.ng, blk, D);
+    __syncthreads();
+    const uint32_t exw = (uint32_t)col_pass_lds_words(LOGG, LOGE, A.ng);
+    col_fwd_limb_staged<LOGE, LOGG>(md, D, A.ws + prime * M, A.tw + prime * M, C, A.ng, blk, lds + prime * exw, t,
+                                    A.bm[prime]);
+}
+
 // Inverse column pass of the two-digit-limb transform fused with its normalisation (gfe_limb.h):
 // one workgroup per group of ng columns runs the four primes' inverse column transforms side by side
 // (threads [p T, (p + 1) T) for prime p, T = ng 2^LOGG / 2^LOGE, each prime on its own exchange
@@ -115,6 +137,19 @@ static SpecLimbNormFn col_inv_norm_limb_lookup(int logg) {
     }
 }
 
+template <int LOGE, int LOGG = SPEC_COL_MIN>
+static SpecColFn col_fwd_limb_all_lookup(int logg) {
+    if constexpr (LOGG > SPEC_COL_MAX || LOGG > LOGE + LIMB_FUSED_MAX_SPAN) {
+        return nullptr;
+    } else {
+        if (logg == LOGG) {
+            if constexpr (LOGG >= LOGE) return ks_col_fwd_limb_all<LOGE, LOGG>;
+            else return nullptr;
+        }
+        return col_fwd_limb_all_lookup<LOGE, LOGG + 1>(logg);
+    }
+}
+
 template <class M, int LOGE, int LOGG = SPEC_ROW_MIN>
 static SpecRowFn row_lookup(int logg) {
     if constexpr (LOGG > SPEC_ROW_MAX) {
@@ -150,6 +185,15 @@ SpecColFn spec_col_fwd_limb(int loge, int logg) {
     }
 }
 
+SpecColFn spec_col_fwd_limb_all(int loge, int logg) {
+    switch (loge) {
+        case 3: return col_fwd_limb_all_lookup<3>(logg);
+        case 4: return col_fwd_limb_all_lookup<4>(logg);
+        case 5: return col_fwd_limb_all_lookup<5>(logg);
+        default: return nullptr;
+    }
+}
+
 SpecLimbNormFn spec_col_inv_norm_limb(int loge, int logg) {
     switch (loge) {
         case 3: return col_inv_norm_limb_lookup<3>(logg);
@@ -179,6 +223,8 @@ void spec_set_lds_attributes() {
                     if (SpecColFn f = is32 ? nullptr : spec_col_fwd_limb(loge, g))
                         (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                     if (SpecLimbNormFn f = is32 ? nullptr : spec_col_inv_norm_limb(loge, g))
+                        (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+                    if (SpecColFn f = is32 ? nullptr : spec_col_fwd_limb_all(loge, g))
                         (void)hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
                 }
                 for (int g = SPEC_ROW_MIN; g <= SPEC_ROW_MAX; ++g)
