@@ -1534,11 +1534,11 @@ int gfe_rebase(gfe_engine *e, uint32_t b) {
     gfe::select_primes(b, g.n, &rns, &ref_is32);
     if (rns != g.rns || gfe::transform_is32(rns, ref_is32) != g.is32)
         return set_error(GFE_EKIND, "rebase: b selects another prime set at this n");
+    g.flush_carry();  // register 0's pending carries, in the old base
     if (hipStreamSynchronize(g.stream) != hipSuccess) return set_error(GFE_EHIP, "rebase sync");
     g.b = b;
     g.ref_is32 = ref_is32;
     g.rounds = gfe::carry_rounds(b, g.n);
-    g.carry_pending = false;
     const gfe::CrtConst c = gfe::make_crt(g.rns, g.is32, g.n, b);
     if (hipMemcpy(g.crt, &c, sizeof(c), hipMemcpyHostToDevice) != hipSuccess) return set_error(GFE_EHIP, "crt upload");
     g.can_limb = g.tw_limb != nullptr && gfe::limb_ok(b, g.n) && gfe::default_limb_plan(g.n).limb;

@@ -447,25 +447,30 @@ __device__ __forceinline__ void col_fwd_pass(const M &md, const int32_t *__restr
 // row's first two limbs on the way (`ls`, when not null: the carry into range k C / NG + blk is
 // ls[.] = (u0, u1, v, 0), (u0, u1) onto limb 0's digit pair and v onto limb 1's first digit); each
 // prime's threads then read their column elements from LDS.  S = NG 2^LOGG / blockDim.x limbs per
-// thread, S even.
+// thread, S even.  Load i of thread t moves 16-byte word f = i blockDim.x + t of D (row f / (NG/2)):
+// each wave's loads read whole runs of rows and its LDS stores are 16 consecutive bytes per lane.
 template <int S>
 __device__ __forceinline__ void stage_limb_digits(const int32_t *__restrict__ src, const int4 *__restrict__ ls,
                                                   uint32_t C, uint32_t NG, unsigned blk, int2 *D) {
     static_assert(S >= 2 && S % 2 == 0, "whole int4 loads of two limbs");
-    const uint32_t tid = threadIdx.x, tpr = NG / S;
-    const uint32_t k = tid / tpr, j0 = (tid % tpr) * S;
-    const int4 *g = reinterpret_cast<const int4 *>(src + 2 * ((size_t)k * C + (size_t)blk * NG + j0));
-    int4 cy = make_int4(0, 0, 0, 0);
-    if (ls && j0 == 0) cy = ls[(size_t)k * (C / NG) + blk];
-    int4 v[S / 2];
+    const uint32_t NT = blockDim.x, tid = threadIdx.x;
+    const uint32_t hl = 31u - (uint32_t)__clz((int)(NG / 2));  // NG / 2 int4 per row, a power of two
+    const int4 *g = reinterpret_cast<const int4 *>(src) + (size_t)blk * (NG / 2);
+    int4 v[S / 2], cy[S / 2];
 #pragma unroll
-    for (int i = 0; i < S / 2; ++i) v[i] = g[i];
-    v[0].x += cy.x;
-    v[0].y += cy.y;
-    v[0].z += cy.z;
-    int4 *o = reinterpret_cast<int4 *>(D + (size_t)k * NG + j0);
+    for (int i = 0; i < S / 2; ++i) {
+        const uint32_t f = (uint32_t)i * NT + tid, k = f >> hl, m = f & ((1u << hl) - 1u);
+        v[i] = g[(size_t)k * (C / 2) + m];
+        cy[i] = (ls && m == 0) ? ls[(size_t)k * (C / NG) + blk] : make_int4(0, 0, 0, 0);
+    }
+    int4 *o = reinterpret_cast<int4 *>(D);
 #pragma unroll
-    for (int i = 0; i < S / 2; ++i) o[i] = v[i];
+    for (int i = 0; i < S / 2; ++i) {
+        v[i].x += cy[i].x;
+        v[i].y += cy[i].y;
+        v[i].z += cy[i].z;
+        o[(uint32_t)i * NT + tid] = v[i];
+    }
 }
 
 // As col_fwd_pass (LIMB) for prime-group thread t on its own exchange region L, the first chunk's

@@ -86,8 +86,9 @@ int gfe_info(const gfe_engine *e, char *buf, size_t len);
  * tables again.  b must select the prime set the engine computes in (GFE_EKIND otherwise; create
  * an engine for it); the CRT constants, the carry rounds and the plan become b's -- the two-digit-
  * limb transform where it is exact for b and the engine was created with its tables (for a b that
- * has it), else the digit transform's default.  The registers keep their digits: the caller sets
- * them (gfe_set, gfe_set_digits) before squaring. */
+ * has it), else the digit transform's default.  The registers keep their digits (carries pending
+ * from the last squaring are added first, in the old base): the caller sets them (gfe_set,
+ * gfe_set_digits) before squaring. */
 int gfe_rebase(gfe_engine *e, uint32_t b);
 /* Waits for the engine's queued work. */
 int gfe_sync(gfe_engine *e);

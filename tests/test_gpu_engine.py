@@ -270,14 +270,19 @@ def test_set_rejects_values_beyond_the_balanced_bound(gfe, oracle):
 
 def test_rebase_equals_a_fresh_engine(gfe, oracle):
     """gfe_rebase moves an engine to another b of its n and prime set: the squarings then equal a
-    fresh engine's for that b (the limb transform's and the digit transform's bases alike), and a b
-    of another prime set is refused."""
+    fresh engine's for that b (the limb transform's and the digit transform's bases alike), the
+    registers keep the last value's digits (carries still pending from its last squaring included),
+    and a b of another prime set is refused."""
     n = 16
     t = gfe.Transform(20204692, n)
     assert t.get_plan()[5] & 32
     bits = [1, 0, 1, 1, 0, 0, 1, 0, 1, 1] * 3
+    last = None  # (b, the oracle's digits) of the value the engine holds
     for b in (33554434, 1500000000, 20204692):  # limb transform; NTT3 beyond the limb bound; back
         t.rebase(b)
+        if last is not None:
+            pb, want = last
+            assert np.array_equal(canon(oracle, t.get_digits(0), n, pb), want), (pb, b)
         fresh = gfe.Transform(b, n)
         assert t.get_plan() == fresh.get_plan() and t.info() == fresh.info(), b
         o = oracle.Engine(b, n)
@@ -285,6 +290,9 @@ def test_rebase_equals_a_fresh_engine(gfe, oracle):
         for bit in bits:
             t.square_dup(bit); o.square_dup(bit)
         assert np.array_equal(canon(oracle, t.get_digits(0), n, b), canon(oracle, o.get_digits(0), n, b)), b
+        for bit in bits[:3]:  # leaves the last squaring's carries pending for the rebase
+            t.square_dup(bit); o.square_dup(bit)
+        last = (b, canon(oracle, o.get_digits(0), n, b))
         fresh.close()
     with pytest.raises(gfe.GfeError):
         t.rebase(4098)  # NTT2 at n = 16: another prime set
