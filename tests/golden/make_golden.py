@@ -20,6 +20,8 @@ Outputs (all values are the reference's, hex RES64 / old64):
   ref_c3i32.json : C3, 16 candidates, RES64 after 2048 squarings through the "i32" transform
   ref_c4.json  : C4 (n=22, b in [1e7, 3.5e7]), 1 candidate, RES64 every 2000 squarings to 100,000
                  and the SHA-256 of the canonical digits at 100,000
+  ref_c4max.json : C4 at the top of its range (b = 35,000,000, n = 22), RES64 every 2000 squarings to
+                 20,000 and the SHA-256 of the canonical digits there
   ref_c5.json  : C5 serving mix, the first 48 requests, RES64 after their 4096 squarings
 """
 from __future__ import annotations
@@ -98,7 +100,7 @@ def write(name, obj, quiet=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=8)
-    ap.add_argument("--only", default="c1,c1i32,c3i32,c2,c3t,c3f,c4,c5")
+    ap.add_argument("--only", default="c1,c1i32,c3i32,c2,c3t,c3f,c4,c4max,c5")
     ap.add_argument("--c3f-count", type=int, default=512)
     args = ap.parse_args()
     only = set(args.only.split(","))
@@ -138,6 +140,9 @@ def main():
         b = W.config_bases(W.C4, 1)[0]
         r = job_trace(b, 22, 100_000, 2000, args.jobs, True)
         write("ref_c4.json", {"source": src, "config": "C4", "results": [r]})
+    if "c4max" in only:
+        r = job_trace(W.C4["hi"] - 1, 22, 20_000, 2000, args.jobs, True)
+        write("ref_c4max.json", {"source": src, "config": "C4, largest b", "results": [r]})
     ex.shutdown()
 
 

@@ -89,6 +89,21 @@ def test_c4_checkpoints_to_100000(gfe):
         assert canonical_sha256(gfe, g["digits"], 22, r["b"]) == r["sha256"], r["b"]
 
 
+def test_c4_top_of_range(gfe):
+    """C4 at its largest b (35,000,000, n = 22): the default plan is the fused two-digit-limb plan,
+    whose CRT has the least slack at the top of the range; RES64 at every 2000th of the first 20,000
+    squarings and the whole canonical residue there (SHA-256) equal the reference's."""
+    for r in golden("ref_c4max.json"):
+        t = gfe.Transform(r["b"], 22)
+        assert t.get_plan()[5] == 42, t.get_plan()
+        t.close()
+        tr = gfe.run_prp_trace(r["b"], 22, r["iters"], r["every"])
+        assert [h(a) for a, _ in tr] == r["res64"], r["b"]
+        g = gfe.run_prp(r["b"], 22, iters=r["iters"], b_gl=2000, digits=True)
+        assert h(g["res64"]) == r["res64"][-1], r["b"]
+        assert canonical_sha256(gfe, g["digits"], 22, r["b"]) == r["sha256"], r["b"]
+
+
 def test_c5_serving_sample(gfe):
     """C5: the first 48 serving requests (n in 12..20), RES64 after each one's 4096 squarings,
     batched per n."""
