@@ -24,8 +24,10 @@ def canon(O, d, n, b):
 
 
 # (b, n): NTT3 candidates inside the limb bound (b >= 2^23, N l_max^2 < P/4), the largest b of n = 12
-# included; C4's range at n = 22
-CASES = [(50000000, 12), (249999998, 12), (100000000, 13), (60000000, 15), (20204692, 22)]
+# included; the lone-request sizes n = 18 (rows of 16 points per thread, two per workgroup) and
+# n = 20 at the top of the serving mix's range (2^8-point columns); C4's range at n = 22
+CASES = [(50000000, 12), (249999998, 12), (100000000, 13), (60000000, 15), (30000000, 18), (54999998, 20),
+         (20204692, 22)]
 
 
 @pytest.mark.parametrize("b,n", CASES)
@@ -38,7 +40,7 @@ def test_engine_starts_with_the_limb_plan(gfe, b, n):
 
 @pytest.mark.parametrize("b,n", CASES)
 def test_limb_squarings_match_oracle(gfe, oracle, b, n):
-    iters = {12: 200, 13: 120, 15: 40, 22: 3}[n]
+    iters = {12: 200, 13: 120, 15: 40, 18: 16, 20: 6, 22: 3}[n]
     bits = prp_bits(b, n, iters)
     t = gfe.Transform(b, n)
     o = oracle.Engine(b, n)
