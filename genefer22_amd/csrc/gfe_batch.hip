@@ -1519,13 +1519,38 @@ int gfe_batch_tune(gfe_batch *b, uint32_t reps) {
         screen.emplace_back(ms, k);
     }
     std::sort(screen.begin(), screen.end());
+    std::vector<std::pair<double, size_t>> fin;
     for (size_t j = 0; j < std::min<size_t>(8, screen.size()); ++j) {
         B.plan = plans[screen[j].second];
         if (B.zero_carries() != GFE_OK) break;
         double ms = 0;
         if (gfe_batch_time_rounds(b, 1, &ms) != GFE_OK) continue;
         if (gfe_batch_time_rounds(b, 8 * reps, &ms) != GFE_OK) continue;
+        fin.emplace_back(ms, screen[j].second);
         if (ms < best_ms) { best_ms = ms; best = plans[screen[j].second]; }
+    }
+    // the three fastest again as the batch runs its blocks (captured 64-round graphs, when on),
+    // in turn twice, each kept at its better time: eager timings rank the plans, but graph
+    // launches save each plan a different share of its launch overhead
+    if (graphs && fin.size() > 1) {
+        std::sort(fin.begin(), fin.end());
+        const size_t m = std::min<size_t>(3, fin.size());
+        std::vector<double> t(m, 1e30);
+        B.use_graphs = true;
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t i = 0; i < m; ++i) {
+                B.plan = plans[fin[i].second];
+                B.drop_graphs();
+                if (B.zero_carries() != GFE_OK) break;
+                double ms = 0;
+                if (gfe_batch_time_rounds(b, 64, &ms) != GFE_OK) continue;
+                if (gfe_batch_time_rounds(b, 128, &ms) != GFE_OK) continue;
+                t[i] = std::min(t[i], ms);
+            }
+        size_t bi = 0;
+        for (size_t i = 1; i < m; ++i)
+            if (t[i] < t[bi]) bi = i;
+        if (t[bi] < 1e30) best = plans[fin[bi].second];
     }
     B.plan = best;
     B.use_graphs = graphs;

@@ -68,8 +68,10 @@ int gfe_batch_destroy(gfe_batch *bt);
 size_t gfe_batch_slot_bytes(uint32_t n);
 /* As gfe_batch_slot_bytes for a proof batch of `depth` (3 + 2^depth registers per slot). */
 size_t gfe_batch_proof_slot_bytes(uint32_t n, uint32_t depth);
-/* Chooses the pass plan by timing the admissible ones on `reps` rounds of the batch's busy slots
- * (their registers are saved and restored, so their programs continue unchanged); returns
+/* Chooses the pass plan by timing the admissible ones on `reps` rounds of the batch's busy slots,
+ * the fastest eight again on 8 x reps and, when the batch runs its blocks as graphs, the fastest
+ * three on captured 64-round blocks (their registers are saved and restored, so their programs
+ * continue unchanged); returns
  * GFE_EINVAL when no slot is busy: submit the candidates first.  The reference's tune,
  * src/transformGPU.h:851-900. */
 int gfe_batch_tune(gfe_batch *bt, uint32_t reps);
