@@ -369,11 +369,11 @@ def cpu_baseline(O, b: int, threads: int, budget_s: float):
         ref.square_dup(count & 1)
         count += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or count >= 5000:
+        if el >= budget_s or count >= 20000:
             break
     return {"value": round(el / count * 1e9, 1), "unit": "ns/squaring", "cores": threads, "kind": "reference",
             "sample": f"{count} squarings of one n=22 candidate (b={b}) with the reference's '{ref.impl}' CPU "
-                      f"transform, {threads} thread(s)"}
+                      f"transform, {threads} thread(s), {el:.1f} s"}
 
 
 def bench_small_configs(G, O, cores: int, budget_s: float = 2.0):
