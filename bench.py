@@ -22,7 +22,9 @@ One JSON line on stdout (progress on stderr):
                  serving loop, and candidates/hour (each truncated to its first 4096 squarings)
                  against the serial plain path: the 2000-request C5 workload at about 80 % of capacity
                  (serving) and 40 requests at 2/s (serving_light: requests seldom overlap, so
-                 time-to-result is a lone request's service time in the batched path);
+                 time-to-result is a lone request's service time); every served RES64 / old64 is
+                 compared with the reference's for the same request (serving.parity, from
+                 tests/golden/ref_c5all.json when present; the run fails on a mismatch);
   * parity     : RES64 after P PRP iterations on the GPU vs the reference's own CPU transform;
   * cpu_baseline: the reference's CPU transform (compiled from its sources into oracle/_ref) timed on
                  this host's cores on a bounded sample of the same workload;
@@ -301,8 +303,11 @@ def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
     from genefer22_amd import serve
     reqs = W.serving_requests(count, rate, iters=iters)
     t0 = time.perf_counter()
-    _, summary, stats = serve.run(reqs)
+    results, summary, stats = serve.run(reqs)
     wall = time.perf_counter() - t0
+    parity = serving_parity(reqs, results)
+    if parity and not parity["match"]:
+        raise RuntimeError(f"served residues differ from the reference: {parity}")
     plain_ms_by_kind(G, reqs, ms_cache)
     ms_of = [ms_cache[engine_kind(G, q.b, q.n)] for q in reqs]
     serial = sum(q.iters * m for q, m in zip(reqs, ms_of)) / 1e3
@@ -326,7 +331,28 @@ def bench_serving(G, count: int, rate: float, iters: int, ms_cache: dict):
         "plain_kinds_measured": len(ms_cache),
         "pools_created": stats["pools_created"], "bytes_peak": stats["bytes_peak"], "wall_s": round(wall, 2),
         "lone_runs": stats.get("solo", 0),
+        "parity": parity,
     }
+
+
+def serving_parity(reqs, results, path=None):
+    """Every served RES64 / old64 against the reference CPU transform's for the same request
+    (tests/golden/ref_c5all.json, by request id: the (n, b) of a request id do not depend on the
+    arrival rate); None when the fixture is absent."""
+    path = path or os.path.join(ROOT, "tests", "golden", "ref_c5all.json")
+    if not os.path.exists(path):
+        return None
+    ref = {r["rid"]: r for r in json.load(open(path))["results"]}
+    checked, bad = 0, []
+    for q, g in zip(reqs, results):
+        r = ref.get(q.rid)
+        if r is None or (r["n"], r["b"], r["iters"]) != (q.n, q.b, q.iters):
+            continue
+        checked += 1
+        if (f"{g['res64']:016X}", f"{g['old64']:016X}") != (r["res64"], r["old64"]):
+            bad.append(q.rid)
+    return {"checked": checked, "of": len(reqs), "mismatches": bad[:16], "match": not bad,
+            "reference": "tests/golden/ref_c5all.json (the reference's CPU transform)"}
 
 
 def parity_n22(G, O, b: int, iters: int, threads: int):

@@ -23,6 +23,9 @@ Outputs (all values are the reference's, hex RES64 / old64):
   ref_c4max.json : C4 at the top of its range (b = 35,000,000, n = 22), RES64 every 2000 squarings to
                  20,000 and the SHA-256 of the canonical digits there
   ref_c5.json  : C5 serving mix, the first 48 requests, RES64 after their 4096 squarings
+  ref_c5all.json : C5, all 2000 requests (the same (n, b) for any arrival rate), RES64 and old64
+                 after their 4096 squarings; written after every finished request and resumed from
+                 what the file already holds
 """
 from __future__ import annotations
 
@@ -90,6 +93,28 @@ def _full_resumable(ex, name, cfg_name, bases, n, src):
         print(f"{name}: {len(done)}/{len(bases)}", flush=True)
 
 
+def _trunc_resumable(ex, name, cfg_name, reqs, src):
+    """Truncated runs of the serving requests `reqs`, keyed by request id, the file rewritten (in
+    request order) after each finished run and the runs it already holds skipped."""
+    from concurrent.futures import as_completed
+    path = os.path.join(HERE, name)
+    done = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            for r in json.load(f)["results"]:
+                done[r["rid"]] = r
+    # the large n first, so the pool ends on short runs
+    todo = sorted((q for q in reqs if q.rid not in done), key=lambda q: -q.n)
+    futs = {ex.submit(job_trunc, q.b, q.n, q.iters, 1): q for q in todo}
+    for k, fut in enumerate(as_completed(futs)):
+        q = futs[fut]
+        done[q.rid] = dict(fut.result(), rid=q.rid)
+        if k % 16 == 15 or k == len(futs) - 1:
+            write(name, {"source": src, "config": cfg_name,
+                         "results": [done[r.rid] for r in reqs if r.rid in done]}, quiet=True)
+            print(f"{name}: {len(done)}/{len(reqs)}", flush=True)
+
+
 def write(name, obj, quiet=False):
     with open(os.path.join(HERE, name), "w") as f:
         json.dump(obj, f, indent=1)
@@ -120,6 +145,9 @@ def main():
         futs = [ex.submit(job_trunc, q.b, q.n, q.iters, 1, True) for q in reqs]
         write("ref_c5.json", {"source": src, "config": "C5 serving sample (first 48 requests, seed 5, rate 100/s)",
                               "results": [dict(f.result(), rid=q.rid) for f, q in zip(futs, reqs)]})
+    if "c5all" in only:
+        _trunc_resumable(ex, "ref_c5all.json", "C5 serving workload (all 2000 requests, seed 5)",
+                         W.serving_requests(2000, 24.0), src)
     if "c2" in only:
         bases = W.config_bases(W.C2, 2)
         futs = [ex.submit(job_trace, b, 16, 200_000, 1000, 4) for b in bases]

@@ -77,3 +77,28 @@ def test_measured_counters_cover_the_default_plan():
     row = next(l for l in text.splitlines() if l.startswith(f"| `{short}`") and "MiB" not in l and "M |" in l)
     assert f"{m['fetch_mib']}" in row and f"{m['write_mib']}" in row, row
     assert f"{m['valu_instr_per_butterfly']} VALU instructions" in text
+
+
+def test_serving_parity_finds_a_changed_residue(tmp_path):
+    """bench.py's serving.parity: every served RES64 / old64 is looked up by request id in the
+    reference fixture; requests the fixture does not hold (or holds for another n, b or length) are
+    not counted, a changed residue is reported and fails the comparison, a missing fixture gives None."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    from genefer22_amd import workloads
+    reqs = workloads.serving_requests(6, 24.0)
+    assert [(q.n, q.b) for q in reqs] == [(q.n, q.b) for q in workloads.serving_requests(6, 2.0)]
+    ref = [{"rid": q.rid, "n": q.n, "b": q.b, "iters": q.iters, "res64": f"{0x1000 + q.rid:016X}",
+            "old64": f"{0x2000 + q.rid:016X}"} for q in reqs[:5]]
+    ref[4]["b"] += 2  # another candidate under the same id: not compared
+    path = tmp_path / "ref.json"
+    path.write_text(json.dumps({"results": ref}))
+    results = [{"res64": 0x1000 + q.rid, "old64": 0x2000 + q.rid} for q in reqs]
+    p = bench.serving_parity(reqs, results, str(path))
+    assert p["checked"] == 4 and p["of"] == 6 and p["match"] and p["mismatches"] == []
+    results[2]["old64"] ^= 1
+    p = bench.serving_parity(reqs, results, str(path))
+    assert not p["match"] and p["mismatches"] == [reqs[2].rid]
+    assert bench.serving_parity(reqs, results, str(tmp_path / "absent.json")) is None
