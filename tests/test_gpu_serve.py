@@ -11,21 +11,29 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def test_serve_replay_matches_reference_fixtures(gfe):
-    """The first 48 seeded serving requests (n in 12..20, Poisson arrivals at 100/s) replayed on the
-    wall clock: every RES64/old64 equals the reference CPU transform's (tests/golden/ref_c5.json)."""
+    """The first 200 seeded serving requests (n in 12..20, Poisson arrivals at 100/s: the loop runs
+    saturated, pools and lone runs side by side) replayed on the wall clock: every RES64/old64 equals
+    the reference CPU transform's for the same request (tests/golden/ref_c5all.json, all 2000
+    requests; ref_c5.json, the first 48, where that file is absent)."""
     from genefer22_amd import serve, workloads
-    path = os.path.join(HERE, "golden", "ref_c5.json")
-    if not os.path.exists(path):
-        pytest.skip("ref_c5.json not generated")
-    ref = json.load(open(path))["results"]
-    reqs = workloads.serving_requests(48, 100.0)
+    ref = None
+    for name, count in (("ref_c5all.json", 200), ("ref_c5.json", 48)):
+        path = os.path.join(HERE, "golden", name)
+        if os.path.exists(path):
+            ref = json.load(open(path))["results"][:count]
+            break
+    if ref is None:
+        pytest.skip("no C5 fixture generated")
+    count = len(ref)
+    reqs = workloads.serving_requests(count, 100.0)
     assert [(q.n, q.b) for q in reqs] == [(r["n"], r["b"]) for r in ref]
+    print(f"[serve] replaying {count} requests", flush=True)
     results, summary, stats = serve.run(reqs)
     for q, r, g in zip(reqs, ref, results):
         assert (g["n"], g["b"], g["iters"]) == (q.n, q.b, q.iters)
         assert (f"{g['res64']:016X}", f"{g['old64']:016X}") == (r["res64"], r["old64"]), (q.rid, q.n, q.b)
         assert g["t_arrival"] <= g["t_admit"] <= g["t_done"]
-    assert stats["completed"] == 48 and stats["queued"] == 0
+    assert stats["completed"] == count and stats["queued"] == 0
     assert summary["p50_s"] <= summary["p99_s"]
 
 
