@@ -135,6 +135,28 @@ __device__ __forceinline__ void limb_digits(int64_t v, uint32_t b, double binv, 
     c = (int64_t)(c2 + a2);
 }
 
+// Adds the carry c of the thread below into balanced digits d (c = 0 leaves them): while |c| is at
+// most 8, far below b / 2, every digit needs at most one correction by b and the carry on stays in
+// {-1, 0, 1}, so 32-bit compares do what div_bal does; a larger c takes absorb_carry.
+template <int S>
+__device__ __forceinline__ void absorb_small_carry(int32_t (&d)[S], int64_t c, uint32_t b, double binv) {
+    if (c > 8 || c < -8) {
+        absorb_carry<S>(d, c, b, binv);
+        return;
+    }
+    const int32_t hi = (int32_t)(b / 2), lo = -(int32_t)((b - 1) / 2), bs = (int32_t)b;
+    int32_t k = (int32_t)c;
+#pragma unroll
+    for (int j = 0; j < S - 1; ++j) {
+        if (k != 0) {
+            const int32_t f = d[j] + k;
+            k = (f > hi) - (f < lo);
+            d[j] = f - k * bs;
+        }
+    }
+    d[S - 1] += k;
+}
+
 // Adds the carries of a limb normalisation range boundary into the next range's first S digits:
 // e0 at digit 0, e1 at digit 2 (the limb after), digit by digit while a carry remains.
 template <int S>
@@ -302,8 +324,10 @@ __device__ __forceinline__ void norm_limb_rows(const LimbCrt &c, const uint32_t 
         const int64_t lj = h0[j] + (j == 0 ? (first ? 0 : pb1 + pb2) : h1[j - 1] + (j == 1 ? (first ? 0 : pl2) : h2[j - 2]));
         limb_digits(lj + cr, b, binv, d[2 * j], d[2 * j + 1], cr);
     }
+    // |cr| <= 2: a limb is below 2B + 2^40 + N/2 + 2 in size (doubled pieces), so its carry over
+    // B >= 2^46 after two balanced digits is below 2.52 + |carry in| / B
     const int64_t cin = lane_below(cr);
-    if (!first) absorb_carry<2 * S>(d, cin, b, binv);
+    if (!first) absorb_small_carry<2 * S>(d, cin, b, binv);
     int32_t *o = dst + 2 * ((size_t)k * C + (size_t)blk * NG + j0);
 #pragma unroll
     for (int j = 0; j < 2 * S; j += 4) *reinterpret_cast<int4 *>(o + j) = make_int4(d[j], d[j + 1], d[j + 2], d[j + 3]);

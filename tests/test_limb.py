@@ -294,6 +294,9 @@ def normalise_fused(coeffs, b, n, dup, ng, S):
                 lj = h[j][0] + (pb1 + pb2 if j == 0 else h[j - 1][1] + (pl2 if j == 1 else h[j - 2][2]))
                 d0, d1, cr = limb_digits(lj + cr, b)
                 d += [d0, d1]
+            # the carry a thread hands to the next within its range: the device adds it with one
+            # correction per digit (absorb_small_carry), exact while |cr| <= 8
+            assert abs(cr) <= 2, (w, jt, cr)
             outs.append(d)
             carries.append(cr)
         for jt in range(tpr):
