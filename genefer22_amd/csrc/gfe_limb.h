@@ -55,9 +55,12 @@ __device__ __forceinline__ int64_t i64_of(double v) {
 // Division by B of the exact integer s, given as s mod 2^64 and an estimate sd with |sd - s| < 2^38:
 // s = q B + h with q = rint(sd / B), so |h| <= B/2 + 2^38 (B >= 2^46: within B/2 (1 + 2^-8)), not
 // balanced further -- every later step is exact for any h of that size.
-__device__ __forceinline__ int64_t split_B(uint64_t s64, double sd, const LimbCrt &c, int64_t *q) {
-    const int64_t qi = i64_of(rint(sd * c.invB));
+// *qd receives q as a double (exact: the pieces of P/p_k are at most B/2, so |q| <= 2^32 + 2).
+__device__ __forceinline__ int64_t split_B(uint64_t s64, double sd, const LimbCrt &c, int64_t *q, double *qd) {
+    const double qr = rint(sd * c.invB);
+    const int64_t qi = i64_of(qr);
     *q = qi;
+    *qd = qr;
     return (int64_t)(s64 - (uint64_t)qi * c.B);
 }
 
@@ -94,17 +97,19 @@ __device__ __forceinline__ void limb_pieces(const LimbCrt &c, const uint32_t (&a
         s0d = fma(zd[k], c.w0d[k], s0d);
     }
     int64_t q0;
-    h0 = split_B(s0 + ((uint64_t)s0h << 32), s0d, c, &q0);
+    double q0d;
+    h0 = split_B(s0 + ((uint64_t)s0h << 32), s0d, c, &q0, &q0d);
     uint64_t s1 = (uint64_t)q0 - (uint64_t)(int64_t)t * (uint64_t)c.P1;
     uint32_t s1h = 0;
-    double s1d = fma(-td, c.P1d, (double)q0);
+    double s1d = fma(-td, c.P1d, q0d);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         mac64(s1, s1h, z[k], c.w1l[k], c.w1h[k]);
         s1d = fma(zd[k], c.w1d[k], s1d);
     }
     int64_t q1;
-    h1 = split_B(s1 + ((uint64_t)s1h << 32), s1d, c, &q1);
+    double q1d;
+    h1 = split_B(s1 + ((uint64_t)s1h << 32), s1d, c, &q1, &q1d);
     int64_t s2 = q1 - (int64_t)t * c.P2;
 #pragma unroll
     for (int k = 0; k < 4; ++k) s2 += (int64_t)(int32_t)z[k] * c.w2s[k];  // z < 2^31
@@ -276,6 +281,11 @@ __device__ __forceinline__ int64_t lane_below(int64_t v) {
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+// The 32-bit value of the lane one below, as lane_below.
+__device__ __forceinline__ int32_t lane_below32(int32_t v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);
+}
+
 // The normalisation half: rows of NG limbs whose four primes' residues are at
 // NL[p rows NG + k NG + j]; thread t takes S consecutive limbs of row t / (NG / S).  The NG / S
 // threads of a row are consecutive lanes of one 16-lane DPP row (NG / S <= 16), so the pieces of
@@ -326,7 +336,7 @@ __device__ __forceinline__ void norm_limb_rows(const LimbCrt &c, const uint32_t 
     }
     // |cr| <= 2: a limb is below 2B + 2^40 + N/2 + 2 in size (doubled pieces), so its carry over
     // B >= 2^46 after two balanced digits is below 2.52 + |carry in| / B
-    const int64_t cin = lane_below(cr);
+    const int32_t cin = lane_below32((int32_t)cr);  // |cr| < 2^17 + 1 (limb_digits) in any case
     if (!first) absorb_small_carry<2 * S>(d, cin, b, binv);
     int32_t *o = dst + 2 * ((size_t)k * C + (size_t)blk * NG + j0);
 #pragma unroll
